@@ -1,0 +1,313 @@
+"""ctypes view of the CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py``, as the checker. The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref_hash_dag.so")
+
+# Stream ids of the workload generators (shared with the product).
+STREAM_KEYS = 1
+STREAM_VALUES = 2
+STREAM_SCAN = 3
+STREAM_HASH = 4
+STREAM_BRUSH = 5
+STREAM_BATCH = 6
+STREAM_ARRIVAL = 7
+
+
+class Terrain(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("base", C.c_uint32), ("amp", C.c_uint32),
+                ("period", C.c_uint32), ("octaves", C.c_uint32)]
+
+
+def terrain_for_world(world: int, seed: int, octaves: int = 4) -> "Terrain":
+    """The 512^3 heightfield (base 256, amplitude 64, period 64) scaled to `world`."""
+    return Terrain(seed, world // 2, world // 8, world // 8, octaves)
+
+
+class Brush(C.Structure):
+    _fields_ = [("cx", C.c_int32), ("cy", C.c_int32), ("cz", C.c_int32),
+                ("hx", C.c_int32), ("hy", C.c_int32), ("hz", C.c_int32),
+                ("kind", C.c_uint32), ("material", C.c_uint32)]
+
+
+BRUSH_DTYPE = np.dtype([("cx", "<i4"), ("cy", "<i4"), ("cz", "<i4"), ("hx", "<i4"),
+                        ("hy", "<i4"), ("hz", "<i4"), ("kind", "<u4"), ("material", "<u4")])
+
+_lib = None
+_ref = None
+
+
+def build() -> None:
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        u32, u64, i32 = C.c_uint32, C.c_uint64, C.c_int32
+        sig = {
+            "vxo_mix64": (u64, [u64]),
+            "vxo_rand": (u64, [u64, u64, u64]),
+            "vxo_gen_u32": (None, [u64, u64, u64, u32, u32, P]),
+            "vxo_gen_morton": (None, [u64, u64, u64, u32, P]),
+            "vxo_gen_unique_u64": (None, [u64, u64, u64, P]),
+            "vxo_morton3": (u64, [u32, u32, u32]),
+            "vxo_exclusive_sum_u32": (None, [P, u64, P]),
+            "vxo_inclusive_sum_u32": (None, [P, u64, P]),
+            "vxo_exclusive_max_u32": (None, [P, u64, P]),
+            "vxo_inclusive_max_u32": (None, [P, u64, P]),
+            "vxo_sort_pairs_u32": (None, [P, P, u64, u32]),
+            "vxo_sort_pairs_u64": (None, [P, P, u64, u32]),
+            "vxo_set_contains_u64": (None, [P, u64, P, u64, P]),
+            "vxo_terrain_height": (i32, [P, u32, u32]),
+            "vxo_gen_brushes": (None, [u64, u64, u32, u32, u32, P]),
+            "vxo_brush_voxels": (u64, [P, u32]),
+            "vxo_dag_create": (P, [u32, C.c_int]),
+            "vxo_dag_destroy": (None, [P]),
+            "vxo_dag_build_terrain": (None, [P, P]),
+            "vxo_dag_apply_brushes": (None, [P, P, u32]),
+            "vxo_dag_apply_voxels": (None, [P, P, u32, u32]),
+            "vxo_dag_box": (None, [P, u32, u32, u32, u32, u32, u32, u32]),
+            "vxo_dag_get": (u32, [P, u32, u32, u32]),
+            "vxo_dag_export": (u64, [P, P, u64]),
+            "vxo_dag_stored_nodes": (u64, [P]),
+            "vxo_dag_stored_leaves": (u64, [P]),
+            "vxo_dense_export": (u64, [P, u32, C.c_int, P, u64]),
+            "vxo_digest": (u64, [P, u64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The reference's own CPU HashDAG, compiled from its sources (oracle/_ref)."""
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_PATH)
+        P = C.c_void_p
+        u32, u64 = C.c_uint32, C.c_uint64
+        sig = {
+            "vxr_create": (P, [u32]),
+            "vxr_destroy": (None, [P]),
+            "vxr_box": (C.c_int, [P, u32, u32, u32, u32, u32, u32, C.c_int]),
+            "vxr_set_voxel": (C.c_int, [P, u32, u32, u32, C.c_int]),
+            "vxr_get_voxel": (C.c_int, [P, u32, u32, u32]),
+            "vxr_node_count": (u64, [P]),
+            "vxr_leaf_count": (u64, [P]),
+            "vxr_collect_garbage": (None, [P]),
+            "vxr_export": (u64, [P, P, u64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _ref = L
+    return _ref
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---- generators ---------------------------------------------------------------
+def gen_u32(seed, stream, n, lo, hi):
+    out = np.empty(n, np.uint32)
+    lib().vxo_gen_u32(seed, stream, n, lo, hi, _ptr(out))
+    return out
+
+
+def gen_morton(seed, stream, n, bits):
+    out = np.empty(n, np.uint64)
+    lib().vxo_gen_morton(seed, stream, n, bits, _ptr(out))
+    return out
+
+
+def gen_unique_u64(seed, stream, n):
+    out = np.empty(n, np.uint64)
+    lib().vxo_gen_unique_u64(seed, stream, n, _ptr(out))
+    return out
+
+
+def gen_brushes(seed, stream, n, world, mix):
+    out = np.zeros(n, BRUSH_DTYPE)
+    lib().vxo_gen_brushes(seed, stream, n, world, mix, _ptr(out))
+    return out
+
+
+def brush_voxels(brushes: np.ndarray, world: int) -> int:
+    b = np.ascontiguousarray(brushes, BRUSH_DTYPE)
+    return sum(int(lib().vxo_brush_voxels(C.c_void_p(b.ctypes.data + i * 32), world)) for i in range(len(b)))
+
+
+def terrain_height(t: Terrain, x, z):
+    return lib().vxo_terrain_height(C.byref(t), x, z)
+
+
+# ---- primitives ---------------------------------------------------------------
+def exclusive_sum(a):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.empty_like(a)
+    lib().vxo_exclusive_sum_u32(_ptr(a), a.size, _ptr(out))
+    return out
+
+
+def inclusive_sum(a):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.empty_like(a)
+    lib().vxo_inclusive_sum_u32(_ptr(a), a.size, _ptr(out))
+    return out
+
+
+def exclusive_max(a):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.empty_like(a)
+    lib().vxo_exclusive_max_u32(_ptr(a), a.size, _ptr(out))
+    return out
+
+
+def inclusive_max(a):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.empty_like(a)
+    lib().vxo_inclusive_max_u32(_ptr(a), a.size, _ptr(out))
+    return out
+
+
+def sort_pairs(keys, vals=None, key_bits=None):
+    keys = np.array(keys, copy=True)
+    if keys.dtype == np.uint32:
+        f, kb = lib().vxo_sort_pairs_u32, 32
+    elif keys.dtype == np.uint64:
+        f, kb = lib().vxo_sort_pairs_u64, 64
+    else:
+        raise TypeError(keys.dtype)
+    v = None if vals is None else np.array(vals, np.uint32, copy=True)
+    f(_ptr(keys), None if v is None else _ptr(v), keys.size, kb if key_bits is None else key_bits)
+    return keys, v
+
+
+def set_contains(keys, queries):
+    keys = np.ascontiguousarray(keys, np.uint64)
+    queries = np.ascontiguousarray(queries, np.uint64)
+    out = np.empty(queries.size, np.uint8)
+    lib().vxo_set_contains_u64(_ptr(keys), keys.size, _ptr(queries), queries.size, _ptr(out))
+    return out
+
+
+def digest(words: np.ndarray) -> int:
+    w = np.ascontiguousarray(words, np.uint32)
+    return int(lib().vxo_digest(_ptr(w), w.size))
+
+
+# ---- DAG ----------------------------------------------------------------------
+class Dag:
+    """CPU Hash DAG (restatement of svt/hash_dag.hpp, extended with materials)."""
+
+    def __init__(self, inner_depth: int, materials: bool):
+        self.h = lib().vxo_dag_create(inner_depth, int(materials))
+        if not self.h:
+            raise ValueError("bad depth")
+        self.depth = inner_depth
+        self.world = 4 << inner_depth
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().vxo_dag_destroy(self.h)
+            self.h = None
+
+    def build_terrain(self, t: Terrain):
+        lib().vxo_dag_build_terrain(self.h, C.byref(t))
+
+    def apply_brushes(self, brushes):
+        b = np.ascontiguousarray(brushes, BRUSH_DTYPE)
+        lib().vxo_dag_apply_brushes(self.h, _ptr(b), len(b))
+
+    def apply_voxels(self, xyz, material):
+        a = np.ascontiguousarray(xyz, np.uint32).reshape(-1, 3)
+        lib().vxo_dag_apply_voxels(self.h, _ptr(a), len(a), material)
+
+    def box(self, lo, hi, material):
+        lib().vxo_dag_box(self.h, *lo, *hi, material)
+
+    def get(self, x, y, z) -> int:
+        return int(lib().vxo_dag_get(self.h, x, y, z))
+
+    def export(self) -> np.ndarray:
+        n = lib().vxo_dag_export(self.h, None, 0)
+        out = np.empty(n, np.uint32)
+        lib().vxo_dag_export(self.h, _ptr(out), n)
+        return out
+
+    def stored(self):
+        return int(lib().vxo_dag_stored_nodes(self.h)), int(lib().vxo_dag_stored_leaves(self.h))
+
+
+def dense_export(grid: np.ndarray, inner_depth: int, materials: bool) -> np.ndarray:
+    """Canonical export of a dense grid[z, y, x] of u8 materials (brute force)."""
+    g = np.ascontiguousarray(grid, np.uint8)
+    n = lib().vxo_dense_export(_ptr(g), inner_depth, int(materials), None, 0)
+    out = np.empty(n, np.uint32)
+    lib().vxo_dense_export(_ptr(g), inner_depth, int(materials), _ptr(out), n)
+    return out
+
+
+class RefDag:
+    """The reference's own HashDAG (occupancy only), from oracle/_ref."""
+
+    def __init__(self, inner_depth: int):
+        self.h = ref().vxr_create(inner_depth)
+        if not self.h:
+            raise ValueError("bad depth")
+        self.world = 4 << inner_depth
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref().vxr_destroy(self.h)
+            self.h = None
+
+    def box(self, lo, hi, fill: bool):
+        rc = ref().vxr_box(self.h, *lo, *hi, int(fill))
+        if rc:
+            raise RuntimeError("reference box edit failed")
+
+    def set_voxel(self, x, y, z, value: bool):
+        rc = ref().vxr_set_voxel(self.h, x, y, z, int(value))
+        if rc:
+            raise RuntimeError("reference set_voxel failed")
+
+    def get(self, x, y, z) -> int:
+        return int(ref().vxr_get_voxel(self.h, x, y, z))
+
+    def counts(self):
+        return int(ref().vxr_node_count(self.h)), int(ref().vxr_leaf_count(self.h))
+
+    def collect_garbage(self):
+        ref().vxr_collect_garbage(self.h)
+
+    def export(self) -> np.ndarray:
+        n = ref().vxr_export(self.h, None, 0)
+        out = np.empty(n, np.uint32)
+        ref().vxr_export(self.h, _ptr(out), n)
+        return out

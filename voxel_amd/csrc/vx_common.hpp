@@ -1,0 +1,113 @@
+// Shared device helpers for the gfx950 kernels (wave64, CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/vx.h"
+
+namespace vx {
+
+constexpr uint32_t kWave = 64;
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
+
+inline vx_status from_hip(hipError_t e) { return e == hipSuccess ? VX_OK : VX_DEVICE_ERROR; }
+inline vx_status last_launch() { return from_hip(hipGetLastError()); }
+
+// Carves consecutive 256-byte aligned regions out of a caller workspace.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  explicit Carver(void* p) : base(static_cast<char*>(p)) {}
+  template <class T>
+  T* take(size_t count) {
+    off = align_up(off, 256);
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += count * sizeof(T);
+    return p;
+  }
+  size_t size() const { return align_up(off, 256); }
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / kWave; }
+
+// Number of set bits of `mask` below this lane (mbcnt).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+template <class T>
+__device__ __forceinline__ T shfl(T v, int src) { return __shfl(v, src, kWave); }
+
+// Inclusive wave scan (sum) over 64 lanes.
+template <class T>
+__device__ __forceinline__ T wave_inclusive_sum(T v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T o = __shfl_up(v, d, kWave);
+    if (lane >= uint32_t(d)) v += o;
+  }
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_inclusive_max(T v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T o = __shfl_up(v, d, kWave);
+    if (lane >= uint32_t(d)) v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Device-coherent (agent scope) relaxed atomics: every XCD sees the value,
+// which plain loads through a per-XCD L2 would not guarantee.
+template <class T>
+__device__ __forceinline__ T ld_agent(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_agent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Counter-based splitmix64 stream shared by all generators.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ __forceinline__ uint64_t stream_base(uint64_t seed, uint64_t stream) {
+  return mix64(seed ^ mix64(stream));
+}
+__host__ __device__ __forceinline__ uint64_t stream_at(uint64_t base, uint64_t i) {
+  return mix64(base + i * 0x9E3779B97F4A7C15ull);
+}
+__host__ __device__ __forceinline__ uint32_t uniform_u32(uint64_t r, uint32_t lo, uint32_t hi) {
+  const uint64_t span = uint64_t(hi) - lo + 1;
+  return lo + uint32_t(((r >> 32) * span) >> 32);
+}
+
+// Zero a u32 range (graph-capturable replacement for hipMemsetAsync).
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v);
+vx_status fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+// Scan building blocks used by other translation units.
+namespace scan {
+// Returns bytes of workspace the single-pass scan needs for n elements.
+size_t workspace_bytes(uint32_t n);
+enum Op : uint32_t { kSum = 0, kMax = 1 };
+vx_status run(uint32_t* data, uint32_t n, bool inclusive, Op op, void* ws, hipStream_t s);
+}  // namespace scan
+
+}  // namespace vx

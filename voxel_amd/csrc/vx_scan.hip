@@ -1,0 +1,283 @@
+// Single-pass prefix scan with decoupled look-back, written for wave64.
+//
+// Replaces algo::cuda::scan's default BlockBased<DecoupledLookback<
+// WarpShuffleBlock<512,4>>> (cuda/scan/scan.cuh:22-23,
+// cuda/scan/block_based/decoupled_lookback.cuh:116-222) and the fused
+// transform -> scan -> post-scan kernel used for stream compaction
+// (cuda/scan/fused.cuh:102-159).
+//
+// Layout: a tile is kBlock threads x 16 elements. Each wave owns a contiguous
+// 1024-element segment that it reads as 4 rounds of 64 lanes x uint4, so every
+// load instruction moves 1 KiB of consecutive bytes. A tile publishes its
+// aggregate, then wave 0 inspects up to 64 predecessor records per step
+// (ballot over the 64 lanes) instead of walking back one tile at a time.
+#include "vx_scan.hpp"
+
+namespace vx {
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+vx_status fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (n == 0) return VX_OK;
+  const uint32_t blocks = uint32_t(std::min<uint64_t>(ceil_div(n, 256), 4096));
+  fill_u32_kernel<<<blocks, 256, 0, s>>>(p, n, v);
+  return last_launch();
+}
+
+namespace scan {
+
+namespace {
+
+template <Op OP>
+__device__ __forceinline__ uint32_t combine(uint32_t a, uint32_t b) {
+  if constexpr (OP == kSum) return a + b;
+  else return a > b ? a : b;
+}
+
+template <Op OP>
+__device__ __forceinline__ uint32_t wave_incl(uint32_t v) {
+  if constexpr (OP == kSum) return wave_inclusive_sum(v);
+  else return wave_inclusive_max(v);
+}
+
+// Scan of a tile whose kBlock*16 elements were loaded as 4 uint4 per lane in
+// wave-segment order; returns, per lane and round, the exclusive prefix of the
+// first element of the vector within the tile, and the tile aggregate.
+template <Op OP, uint32_t kBlock>
+struct TileScan {
+  static constexpr uint32_t kWaves = kBlock / kWave;
+  uint32_t* s_wave;  // [kWaves] LDS
+
+  __device__ uint32_t run(const uint4 (&v)[4], uint32_t (&excl)[4]) {
+    const uint32_t lane = lane_id(), wave = wave_id();
+    uint32_t carry = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t s = combine<OP>(combine<OP>(v[k].x, v[k].y), combine<OP>(v[k].z, v[k].w));
+      const uint32_t inc = wave_incl<OP>(s);
+      uint32_t ex = __shfl_up(inc, 1, kWave);
+      if (lane == 0) ex = 0;
+      excl[k] = combine<OP>(carry, ex);
+      carry = combine<OP>(carry, shfl(inc, 63));
+    }
+    if (lane == 0) s_wave[wave] = carry;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) {
+      const uint32_t t = s_wave[w];
+      if (w < wave) before = combine<OP>(before, t);
+      total = combine<OP>(total, t);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) excl[k] = combine<OP>(before, excl[k]);
+    return total;
+  }
+};
+
+}  // namespace
+
+// Decoupled look-back (lookback_window in vx_scan.hpp): records pack status
+// (bits 32..33: 1 aggregate, 2 inclusive prefix) over a 32-bit value, so one
+// 64-bit agent-scope load observes both, and wave 0 inspects 64 predecessors
+// per step.
+
+namespace {
+
+template <Op OP, bool INCL, uint32_t kBlock>
+__global__ __launch_bounds__(kBlock) void scan_vec_kernel(uint32_t* __restrict__ data, uint32_t n,
+                                                         uint32_t* counter, uint64_t* rec) {
+  constexpr uint32_t kTile = kBlock * 16;
+  __shared__ uint32_t s_wave[kBlock / kWave];
+  __shared__ uint32_t s_tile, s_prefix;
+  const bool multi = gridDim.x > 1;
+  if (threadIdx.x == 0) s_tile = multi ? atomicAdd(counter, 1u) : 0u;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t lane = lane_id(), wave = wave_id();
+  const uint64_t seg = uint64_t(tile) * kTile + wave * 1024u;
+
+  uint4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t i = seg + k * 256u + lane * 4u;
+    if (i + 3 < n) {
+      v[k] = *reinterpret_cast<const uint4*>(data + i);
+    } else {
+      v[k].x = i < n ? data[i] : 0u;
+      v[k].y = i + 1 < n ? data[i + 1] : 0u;
+      v[k].z = i + 2 < n ? data[i + 2] : 0u;
+      v[k].w = 0u;
+    }
+  }
+  uint32_t excl[4];
+  TileScan<OP, kBlock> ts{s_wave};
+  const uint32_t agg = ts.run(v, excl);
+
+  if (wave == 0) {
+    uint32_t prefix = 0;
+    if (multi) {
+      if (tile == 0) {
+        if (lane == 0) st_agent(&rec[0], (uint64_t(2) << 32) | agg);
+      } else {
+        if (lane == 0) st_agent(&rec[tile], (uint64_t(1) << 32) | agg);
+        prefix = lookback_window<OP>(rec, tile);
+        if (lane == 0) st_agent(&rec[tile], (uint64_t(2) << 32) | combine<OP>(prefix, agg));
+      }
+    }
+    if (lane == 0) s_prefix = prefix;
+  }
+  __syncthreads();
+  const uint32_t prefix = s_prefix;
+
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t i = seg + k * 256u + lane * 4u;
+    uint32_t a = combine<OP>(prefix, excl[k]);
+    uint4 o;
+    if constexpr (INCL) {
+      a = combine<OP>(a, v[k].x); o.x = a;
+      a = combine<OP>(a, v[k].y); o.y = a;
+      a = combine<OP>(a, v[k].z); o.z = a;
+      a = combine<OP>(a, v[k].w); o.w = a;
+    } else {
+      o.x = a; a = combine<OP>(a, v[k].x);
+      o.y = a; a = combine<OP>(a, v[k].y);
+      o.z = a; a = combine<OP>(a, v[k].z);
+      o.w = a;
+    }
+    if (i + 3 < n) {
+      *reinterpret_cast<uint4*>(data + i) = o;
+    } else {
+      if (i < n) data[i] = o.x;
+      if (i + 1 < n) data[i + 1] = o.y;
+      if (i + 2 < n) data[i + 2] = o.z;
+    }
+  }
+}
+
+constexpr uint32_t kScanBlock = 256;
+constexpr uint32_t kScanTile = kScanBlock * 16;
+
+template <Op OP, bool INCL>
+vx_status launch(uint32_t* data, uint32_t n, void* ws, hipStream_t s) {
+  const uint32_t tiles = ceil_div(n, kScanTile);
+  uint32_t* counter = nullptr;
+  uint64_t* rec = nullptr;
+  if (tiles > 1) {
+    Carver c(ws);
+    counter = c.take<uint32_t>(64);
+    rec = c.take<uint64_t>(tiles);
+    const vx_status st = fill_u32(counter, (reinterpret_cast<char*>(rec + tiles) - reinterpret_cast<char*>(counter)) / 4, 0, s);
+    if (st != VX_OK) return st;
+  }
+  // Unaligned pointers fall back to element loads inside the kernel only at
+  // the tail; the body requires 16-byte alignment.
+  scan_vec_kernel<OP, INCL, kScanBlock><<<tiles, kScanBlock, 0, s>>>(data, n, counter, rec);
+  return last_launch();
+}
+
+}  // namespace
+
+size_t workspace_bytes(uint32_t n) {
+  const uint32_t tiles = ceil_div(n, kScanTile);
+  if (tiles <= 1) return 0;
+  Carver c(nullptr);
+  c.take<uint32_t>(64);
+  c.take<uint64_t>(tiles);
+  return c.size();
+}
+
+vx_status run(uint32_t* data, uint32_t n, bool inclusive, Op op, void* ws, hipStream_t s) {
+  if (n == 0) return VX_OK;
+  if (reinterpret_cast<uintptr_t>(data) & 15) return generic_inplace(data, n, inclusive, op, ws, s);
+  if (op == kSum) return inclusive ? launch<kSum, true>(data, n, ws, s) : launch<kSum, false>(data, n, ws, s);
+  return inclusive ? launch<kMax, true>(data, n, ws, s) : launch<kMax, false>(data, n, ws, s);
+}
+
+// ---- generic fused scan (transform -> scan -> post-scan) ---------------------
+namespace {
+
+struct InplaceLoad {
+  const uint32_t* p;
+  __device__ uint32_t operator()(uint32_t i) const { return p[i]; }
+};
+
+template <Op OP>
+struct InplaceStoreOp {
+  uint32_t* p;
+  bool inclusive;
+  __device__ void operator()(uint32_t i, uint32_t, uint32_t excl, uint32_t incl) const { p[i] = inclusive ? incl : excl; }
+};
+
+struct FlagLoad {
+  const uint32_t* flags;
+  __device__ uint32_t operator()(uint32_t i) const { return flags[i] != 0u; }
+};
+struct CompactStore {
+  const uint32_t* in;
+  uint32_t* out;
+  uint32_t* count;
+  uint32_t n;
+  __device__ void operator()(uint32_t i, uint32_t v, uint32_t excl, uint32_t incl) const {
+    if (v) out[excl] = in[i];
+    if (i == n - 1) *count = incl;
+  }
+};
+
+}  // namespace
+
+vx_status generic_inplace(uint32_t* data, uint32_t n, bool inclusive, Op op, void* ws, hipStream_t s) {
+  if (op == kSum) return fused<kSum>(n, InplaceLoad{data}, InplaceStoreOp<kSum>{data, inclusive}, ws, s);
+  return fused<kMax>(n, InplaceLoad{data}, InplaceStoreOp<kMax>{data, inclusive}, ws, s);
+}
+
+vx_status compact(const uint32_t* in, const uint32_t* flags, uint32_t n, uint32_t* out, uint32_t* d_count, void* ws,
+                  hipStream_t s) {
+  if (n == 0) return fill_u32(d_count, 1, 0, s);
+  return fused<kSum>(n, FlagLoad{flags}, CompactStore{in, out, d_count, n}, ws, s);
+}
+
+}  // namespace scan
+}  // namespace vx
+
+// ---- C ABI -------------------------------------------------------------------
+using vx::scan::Op;
+
+extern "C" vx_status vx_scan_workspace_size(uint32_t n, size_t* bytes) {
+  if (!bytes) return VX_INVALID_VALUE;
+  *bytes = std::max(vx::scan::workspace_bytes(n), vx::scan::fused_workspace_bytes(n));
+  return VX_OK;
+}
+
+static vx_status scan_entry(uint32_t* data, uint32_t n, void* ws, size_t ws_bytes, void* stream, bool incl, Op op) {
+  if (!data && n) return VX_INVALID_VALUE;
+  size_t need = 0;
+  vx_scan_workspace_size(n, &need);
+  if (need && (!ws || ws_bytes < need)) return VX_INVALID_VALUE;
+  return vx::scan::run(data, n, incl, op, ws, static_cast<hipStream_t>(stream));
+}
+
+extern "C" vx_status vx_exclusive_sum_u32(uint32_t* d, uint32_t n, void* ws, size_t b, void* s) {
+  return scan_entry(d, n, ws, b, s, false, vx::scan::kSum);
+}
+extern "C" vx_status vx_inclusive_sum_u32(uint32_t* d, uint32_t n, void* ws, size_t b, void* s) {
+  return scan_entry(d, n, ws, b, s, true, vx::scan::kSum);
+}
+extern "C" vx_status vx_exclusive_max_u32(uint32_t* d, uint32_t n, void* ws, size_t b, void* s) {
+  return scan_entry(d, n, ws, b, s, false, vx::scan::kMax);
+}
+extern "C" vx_status vx_inclusive_max_u32(uint32_t* d, uint32_t n, void* ws, size_t b, void* s) {
+  return scan_entry(d, n, ws, b, s, true, vx::scan::kMax);
+}
+
+extern "C" vx_status vx_compact_u32(const uint32_t* in, const uint32_t* flags, uint32_t n, uint32_t* out,
+                                    uint32_t* d_count, void* ws, size_t ws_bytes, void* stream) {
+  if (!d_count || (n && (!in || !flags || !out))) return VX_INVALID_VALUE;
+  size_t need = 0;
+  vx_scan_workspace_size(n, &need);
+  if (need && (!ws || ws_bytes < need)) return VX_INVALID_VALUE;
+  return vx::scan::compact(in, flags, n, out, d_count, ws, static_cast<hipStream_t>(stream));
+}

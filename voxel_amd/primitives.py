@@ -1,0 +1,148 @@
+"""Host mirror of the reference's primitive API, over the C ABI.
+
+Names and argument meaning follow ``algo::cuda::scan`` (cuda/scan/scan.cuh:79-161)
+and ``algo::cuda::sort`` (cuda/sort/sort.cuh:100-167): operations are in place
+on device buffers, take an optional caller workspace and a stream, and raise
+``InvalidValue`` where the reference returns ``cudaErrorInvalidValue``.
+Device buffers are torch CUDA tensors (plumbing for HIP memory and streams).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from ._lib import check, lib
+
+
+def _stream(stream):
+    if stream is None:
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, torch.cuda.Stream):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(stream)
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _workspace(nbytes: int, workspace, device):
+    if nbytes == 0:
+        return None, 0
+    if workspace is None:
+        workspace = torch.empty(nbytes, dtype=torch.uint8, device=device)
+    return workspace, workspace.numel() * workspace.element_size()
+
+
+# ---- scan ---------------------------------------------------------------------
+def required_workspace_size(count: int) -> int:
+    """Bytes of workspace the scan needs for `count` elements (0 = none)."""
+    n = C.c_size_t()
+    check(lib().vx_scan_workspace_size(count, C.byref(n)), "vx_scan_workspace_size")
+    return n.value
+
+
+def _scan(fn, data: torch.Tensor, workspace, stream):
+    if data.dtype not in (torch.int32, torch.uint32):
+        raise TypeError("scan supports 32-bit unsigned data")
+    n = data.numel()
+    ws, wsb = _workspace(required_workspace_size(n), workspace, data.device)
+    check(fn(_ptr(data), n, _ptr(ws), wsb, _stream(stream)), fn.__name__)
+    return data
+
+
+def exclusive_sum(data, workspace=None, stream=None):
+    return _scan(lib().vx_exclusive_sum_u32, data, workspace, stream)
+
+
+def inclusive_sum(data, workspace=None, stream=None):
+    return _scan(lib().vx_inclusive_sum_u32, data, workspace, stream)
+
+
+def exclusive_max(data, workspace=None, stream=None):
+    return _scan(lib().vx_exclusive_max_u32, data, workspace, stream)
+
+
+def inclusive_max(data, workspace=None, stream=None):
+    return _scan(lib().vx_inclusive_max_u32, data, workspace, stream)
+
+
+def compact(values: torch.Tensor, flags: torch.Tensor, workspace=None, stream=None):
+    """Keep values[i] where flags[i] != 0, in order. Returns (out, device count)."""
+    n = values.numel()
+    out = torch.empty_like(values)
+    count = torch.zeros(1, dtype=torch.int32, device=values.device)
+    ws, wsb = _workspace(required_workspace_size(n), workspace, values.device)
+    check(lib().vx_compact_u32(_ptr(values), _ptr(flags), n, _ptr(out), _ptr(count), _ptr(ws), wsb, _stream(stream)),
+          "vx_compact_u32")
+    return out, count
+
+
+# ---- sort ---------------------------------------------------------------------
+def _key_info(keys: torch.Tensor):
+    if keys.element_size() == 4:
+        return 4, lib().vx_sort_pairs_u32
+    if keys.element_size() == 8:
+        return 8, lib().vx_sort_pairs_u64
+    raise TypeError("sort keys must be 32- or 64-bit")
+
+
+def required_sort_workspace_size(count: int, key_bytes: int, key_bits: int | None = None,
+                                 with_values: bool = True) -> int:
+    n = C.c_size_t()
+    kb = key_bits if key_bits is not None else key_bytes * 8
+    check(lib().vx_sort_workspace_size(count, key_bytes, kb, int(with_values), C.byref(n)), "vx_sort_workspace_size")
+    return n.value
+
+
+def sort_by_key(keys: torch.Tensor, values: torch.Tensor | None, key_bits: int | None = None, workspace=None,
+                stream=None):
+    """Stable in-place sort of keys (and one u32 value array) by the low key_bits bits."""
+    kbytes, fn = _key_info(keys)
+    kb = key_bits if key_bits is not None else kbytes * 8
+    n = keys.numel()
+    if values is not None and values.numel() != n:
+        raise ValueError("values must match keys")
+    ws, wsb = _workspace(required_sort_workspace_size(n, kbytes, kb, values is not None), workspace, keys.device)
+    check(fn(_ptr(keys), _ptr(values), n, kb, _ptr(ws), wsb, _stream(stream)), fn.__name__)
+    return keys, values
+
+
+def sort_keys(keys, key_bits=None, workspace=None, stream=None):
+    return sort_by_key(keys, None, key_bits, workspace, stream)[0]
+
+
+def set_sort_tuning(key_bytes: int, radix_bits: int, keys_per_thread: int, block: int) -> None:
+    check(lib().vx_sort_set_tuning(key_bytes, radix_bits, keys_per_thread, block), "vx_sort_set_tuning")
+
+
+def get_sort_tuning(key_bytes: int):
+    a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    check(lib().vx_sort_get_tuning(key_bytes, C.byref(a), C.byref(b), C.byref(c)), "vx_sort_get_tuning")
+    return a.value, b.value, c.value
+
+
+# ---- generators ---------------------------------------------------------------
+def gen_u32(seed, stream_id, n, lo, hi, device="cuda"):
+    out = torch.empty(n, dtype=torch.int32, device=device)
+    check(lib().vx_gen_u32(seed, stream_id, n, lo, hi, _ptr(out), _stream(None)), "vx_gen_u32")
+    return out
+
+
+def gen_morton(seed, stream_id, n, coord_bits, device="cuda"):
+    out = torch.empty(n, dtype=torch.int64, device=device)
+    check(lib().vx_gen_morton(seed, stream_id, n, coord_bits, _ptr(out), _stream(None)), "vx_gen_morton")
+    return out
+
+
+def gen_unique_u64(seed, stream_id, first, n, device="cuda"):
+    out = torch.empty(n, dtype=torch.int64, device=device)
+    check(lib().vx_gen_unique_u64(seed, stream_id, first, n, _ptr(out), _stream(None)), "vx_gen_unique_u64")
+    return out
+
+
+def iota(n, device="cuda"):
+    out = torch.empty(n, dtype=torch.int32, device=device)
+    check(lib().vx_iota_u32(_ptr(out), n, _stream(None)), "vx_iota_u32")
+    return out
