@@ -103,6 +103,96 @@ vx_status vx_gen_unique_u64(uint64_t seed, uint64_t stream_id, uint64_t first,
                             uint32_t n, uint64_t* out, void* stream);
 vx_status vx_iota_u32(uint32_t* out, uint32_t n, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Hash DAG. Replaces algo::svt::hash_dag_gpu (HashDagGpu storage.cuh:16-105,
+ * apply/place/destroy_voxel_edits edit.cuh:16-61, get_voxel query.cuh:12-39)
+ * with a runtime depth: world side 4 << inner_depth (inner_depth 1..13, i.e.
+ * up to 32768^3), 64-bit Morton leaf keys, and an optional per-voxel u8
+ * material payload.
+ *   materials == 0: occupancy DAG, exactly the reference's canonical form —
+ *     refs 0 EMPTY / 1 FULL, node item [child_mask | fill_mask << 8, refs of
+ *     table-backed children] (hash_dag_gpu/config.cuh:30-45, detail/node.cuh:210-289),
+ *     leaves = u64 occupancy (bit x + 4y + 16z).
+ *   materials != 0: leaves are 64 u8 materials (0 = empty) in 16 u32 words;
+ *     FULL is never used (a solid region is a shared subtree of materials).
+ * Nodes and leaves are interned by content in two device hash tables (u64
+ * slots: 32-bit fingerprint | pool offset) over two word pools; every
+ * distinct subtree is stored once. A failed edit (pool or table full) returns
+ * VX_OUT_OF_CAPACITY and leaves the old root published.
+ * ------------------------------------------------------------------------- */
+typedef struct vx_dag vx_dag;
+
+typedef struct vx_dag_capacity {
+  uint64_t node_words; /* node pool, u32 words */
+  uint64_t leaf_words; /* leaf pool, u32 words */
+  uint64_t node_slots; /* node table slots (rounded up to a power of two) */
+  uint64_t leaf_slots; /* leaf table slots (rounded up to a power of two) */
+} vx_dag_capacity;
+
+typedef struct vx_voxel { uint32_t x, y, z; } vx_voxel; /* = hash_dag_gpu::VoxelEdit */
+
+/* Brush: sphere (kind 0, radius hx) or box (kind 1, half extents hx, hy, hz)
+ * around an integer centre; covers voxel (x,y,z) iff dx^2+dy^2+dz^2 <= r^2 or
+ * |dx|<=hx && |dy|<=hy && |dz|<=hz. material 0 removes, 1..255 adds (the
+ * occupancy DAG treats any non-zero material as "place"). Brushes of one call
+ * apply in array order: per voxel, the last brush wins. */
+typedef struct vx_brush {
+  int32_t cx, cy, cz;
+  int32_t hx, hy, hz;
+  uint32_t kind;
+  uint32_t material;
+} vx_brush;
+
+/* Heightfield: solid iff y <= base + floor(amp * sum_o 2^-o * n_o(x, z)),
+ * n_o = value noise of period (period >> o) over lattice values uniform in
+ * [-1, 1) from the seeded stream; material 1 + (y mod 255). */
+typedef struct vx_terrain {
+  uint64_t seed;
+  uint32_t base, amp, period, octaves;
+} vx_terrain;
+
+typedef struct vx_dag_stats {
+  uint64_t node_words_used, leaf_words_used;
+  uint64_t node_slots, leaf_slots;
+  uint32_t root;
+  uint32_t inner_depth;
+  uint32_t leaf_words; /* 2 (occupancy) or 16 (materials) */
+  uint32_t reserved;
+} vx_dag_stats;
+
+vx_status vx_dag_create(uint32_t inner_depth, int32_t materials, const vx_dag_capacity* cap, vx_dag** out);
+vx_status vx_dag_destroy(vx_dag* dag);
+/* Clear every table and pool, root = EMPTY (storage.cuh:62-73). */
+vx_status vx_dag_reset(vx_dag* dag, void* stream);
+vx_status vx_dag_stats_get(const vx_dag* dag, vx_dag_stats* out);
+/* Replace the scene by a heightfield (reset + hierarchical build). */
+vx_status vx_dag_build_terrain(vx_dag* dag, const vx_terrain* terrain, void* stream);
+
+/* Drop-in for apply_voxel_edits_workspace_size / apply_voxel_edits
+ * (edit.cuh:16-41): op 0 = Place, 1 = Destroy; material is the placed
+ * material for a materials DAG. Out-of-world voxels are ignored; count 0 is a
+ * no-op; NULL edits with count > 0 or a short workspace is VX_INVALID_VALUE. */
+vx_status vx_dag_voxel_edits_workspace_size(const vx_dag* dag, uint32_t count, size_t* bytes);
+vx_status vx_dag_apply_voxel_edits(vx_dag* dag, const vx_voxel* d_edits, uint32_t count, uint32_t op,
+                                   uint32_t material, void* ws, size_t ws_bytes, void* stream);
+
+/* Brush edit batch; h_brushes is HOST memory (it is small and the host needs
+ * it to size the work). */
+vx_status vx_dag_brush_workspace_size(const vx_dag* dag, const vx_brush* h_brushes, uint32_t n, size_t* bytes);
+vx_status vx_dag_apply_brushes(vx_dag* dag, const vx_brush* h_brushes, uint32_t n, void* ws, size_t ws_bytes,
+                               void* stream);
+
+/* get_voxel for n device coordinates: d_out[i] = material (1 = solid in an
+ * occupancy DAG), 0 if empty or out of the world (query.cuh:12-39). */
+vx_status vx_dag_get_voxels(const vx_dag* dag, const vx_voxel* d_xyz, uint32_t n, uint32_t* d_out, void* stream);
+
+/* Canonical depth-first export (host buffer of u32 words):
+ *   [magic 0x56584443, inner_depth, leaf_words, root kind (0 EMPTY, 1 FULL,
+ *    2 node), node count, leaf count], then per node in first-visit order
+ *   [level, child_mask | fill_mask << 8, child ids...], then leaf payloads.
+ * *n_words receives the size; words are written only if cap suffices. */
+vx_status vx_dag_export_canonical(const vx_dag* dag, uint32_t* out, uint64_t cap, uint64_t* n_words);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,174 @@
+"""GPU Hash DAG edits vs the CPU oracle (canonical export, bit-exact) and the
+reference's own known-answer tests (test/cuda/test_cuda_hash_dag_gpu.cu)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _dag(depth, materials, **kw):
+    from voxel_amd.dag import HashDagGpu
+    return HashDagGpu(depth, materials, **kw)
+
+
+def _dev_xyz(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, np.uint32).reshape(-1, 3).view(np.int32)).to(dev)
+
+
+def spread_edits(count):
+    """make_spread_edits (test_cuda_hash_dag_gpu.cu:142-156): every 19th edit is out of range."""
+    e = []
+    for i in range(count):
+        if i % 19 == 0:
+            e.append((1024 + i, 0, 0))
+        else:
+            e.append(((i * 37) & 1023, (i * 91 + 3) & 1023, (i * 53 + 7) & 1023))
+    return np.array(e, np.uint32)
+
+
+def test_single_voxel_place_destroy(gpu):
+    d = _dag(8, False)
+    d.place_voxel_edits(_dev_xyz([[0, 0, 0], [1, 1, 1], [1023, 1023, 1023]], gpu))
+    q = _dev_xyz([[0, 0, 0], [1, 1, 1], [1, 0, 0], [2, 0, 0], [1023, 1023, 1023], [1022, 1023, 1023], [1024, 0, 0]], gpu)
+    assert d.get_voxels(q).cpu().tolist() == [1, 1, 0, 0, 1, 0, 0]
+    d.destroy_voxel_edits(_dev_xyz([[0, 0, 0]], gpu))
+    assert d.get_voxels(q).cpu().tolist() == [0, 1, 0, 0, 1, 0, 0]
+
+
+def test_eight_partial_siblings_make_a_nine_word_parent(gpu):
+    d = _dag(8, False)
+    pts = [(x * 4, y * 4, z * 4) for z in range(2) for y in range(2) for x in range(2)]
+    d.place_voxel_edits(_dev_xyz(pts, gpu))
+    e = d.export()
+    ref = O.Dag(8, False)
+    ref.apply_voxels(np.array(pts, np.uint32), 1)
+    assert np.array_equal(e, ref.export())
+    # deepest node (level 7) has 8 table-backed children: header + 8 refs
+    nodes = e[4]
+    k, widths = 6, []
+    for _ in range(nodes):
+        lvl, hdr = e[k], e[k + 1]
+        cm = hdr & 0xFF
+        widths.append((lvl, 1 + bin(cm).count("1")))
+        k += 2 + bin(cm).count("1")
+    assert (7, 9) in widths
+
+
+@pytest.mark.parametrize("count", [31, 32, 33, 255, 256, 257, 4099])
+def test_boundary_batch_sizes_match_truth(gpu, count):
+    d = _dag(8, False)
+    edits = spread_edits(count)
+    d.place_voxel_edits(_dev_xyz(edits, gpu))
+    valid = edits[(edits < 1024).all(axis=1)]
+    truth = {tuple(v) for v in valid.tolist()}
+    extra = np.array([((i * 13 + 5) & 1023, (i * 17 + 11) & 1023, (i * 29 + 23) & 1023) for i in range(32)] +
+                     [(1024, 0, 0)], np.uint32)
+    q = np.concatenate([edits, extra])
+    got = d.get_voxels(_dev_xyz(q, gpu)).cpu().numpy()
+    exp = np.array([1 if tuple(v) in truth else 0 for v in q.tolist()])
+    assert np.array_equal(got, exp)
+    ref = O.Dag(8, False)
+    ref.apply_voxels(edits, 1)
+    assert np.array_equal(d.export(), ref.export())
+
+
+def test_invalid_arguments_keep_existing_dag(gpu):
+    from voxel_amd import InvalidValue
+    from voxel_amd._lib import lib
+    import ctypes as C
+    d = _dag(8, False)
+    d.place_voxel_edits(_dev_xyz([[0, 0, 0]], gpu))
+    e = _dev_xyz([[2, 0, 0]], gpu)
+    need = d.apply_voxel_edits_workspace_size(1)
+    ws = torch.empty(need, dtype=torch.uint8, device=gpu)
+    assert lib().vx_dag_apply_voxel_edits(d.h, None, 1, 0, 1, C.c_void_p(ws.data_ptr()), need, None) == 1
+    assert lib().vx_dag_apply_voxel_edits(d.h, C.c_void_p(e.data_ptr()), 1, 0, 1, None, need, None) == 1
+    assert lib().vx_dag_apply_voxel_edits(d.h, C.c_void_p(e.data_ptr()), 1, 0, 1, C.c_void_p(ws.data_ptr()),
+                                          need - 1, None) == 1
+    with pytest.raises(InvalidValue):
+        d.apply_voxel_edits(e, 7)
+    assert d.get_voxels(_dev_xyz([[0, 0, 0], [2, 0, 0]], gpu)).cpu().tolist() == [1, 0]
+
+
+def test_overflow_does_not_publish_partial_root(gpu):
+    """HashDagGpu<1,0> analogue: tiny pools must fail with the old root kept."""
+    from voxel_amd import OutOfCapacity
+    d = _dag(8, False, node_words=64, leaf_words=64, node_slots=64, leaf_slots=64)
+    with pytest.raises(OutOfCapacity):
+        d.place_voxel_edits(_dev_xyz(spread_edits(512), gpu))
+    q = _dev_xyz([[37, 94, 60], [74, 185, 113], [111, 276, 166], [148, 367, 219]], gpu)
+    assert d.get_voxels(q).cpu().tolist() == [0, 0, 0, 0]
+    assert d.export()[3] == 0
+
+
+@pytest.mark.parametrize("materials", [False, True])
+@pytest.mark.parametrize("depth,n,mix,seed", [(4, 30, 0, 1), (6, 64, 0, 2), (7, 64, 0, 1), (8, 200, 1, 5)])
+def test_brushes_on_empty_scene_match_oracle(gpu, materials, depth, n, mix, seed):
+    d = _dag(depth, materials)
+    br = O.gen_brushes(seed, O.STREAM_BRUSH, n, 4 << depth, mix)
+    d.apply_brushes(br)
+    ref = O.Dag(depth, materials)
+    ref.apply_brushes(br)
+    assert np.array_equal(d.export(), ref.export())
+
+
+@pytest.mark.parametrize("materials", [False, True])
+def test_successive_batches_match_oracle(gpu, materials):
+    depth = 7
+    d = _dag(depth, materials)
+    ref = O.Dag(depth, materials)
+    for seed in range(4):
+        br = O.gen_brushes(10 + seed, O.STREAM_BRUSH, 40, 4 << depth, 0)
+        d.apply_brushes(br)
+        ref.apply_brushes(br)
+        assert np.array_equal(d.export(), ref.export()), seed
+    vox = O.gen_u32(3, O.STREAM_KEYS, 3000 * 3, 0, (4 << depth) + 20)  # some out of range
+    d.apply_voxel_edits(_dev_xyz(vox, gpu), 0, material=77)
+    ref.apply_voxels(vox, 77)
+    assert np.array_equal(d.export(), ref.export())
+    d.apply_voxel_edits(_dev_xyz(vox[: 1500 * 3], gpu), 1)
+    ref.apply_voxels(vox[: 1500 * 3], 0)
+    assert np.array_equal(d.export(), ref.export())
+
+
+def test_get_voxels_matches_oracle(gpu):
+    depth = 6
+    d = _dag(depth, True)
+    ref = O.Dag(depth, True)
+    br = O.gen_brushes(4, O.STREAM_BRUSH, 60, 4 << depth, 0)
+    d.apply_brushes(br)
+    ref.apply_brushes(br)
+    q = O.gen_u32(9, O.STREAM_KEYS, 5000 * 3, 0, (4 << depth) - 1).reshape(-1, 3)
+    got = d.get_voxels(_dev_xyz(q, gpu)).cpu().numpy()
+    exp = np.array([ref.get(*map(int, v)) for v in q])
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("materials", [False, True])
+@pytest.mark.parametrize("depth,seed", [(3, 1), (5, 2), (7, 1), (8, 10)])
+def test_terrain_matches_oracle(gpu, materials, depth, seed):
+    from voxel_amd.dag import terrain_for_world
+    world = 4 << depth
+    d = _dag(depth, materials, node_words=1 << 24, leaf_words=1 << 25, node_slots=1 << 21, leaf_slots=1 << 21)
+    d.build_terrain(terrain_for_world(world, seed))
+    ref = O.Dag(depth, materials)
+    ref.build_terrain(O.terrain_for_world(world, seed))
+    assert np.array_equal(d.export(), ref.export())
+
+
+@pytest.mark.parametrize("materials", [False, True])
+def test_config0_terrain_plus_spheres_matches_oracle(gpu, materials):
+    """512^3 heightfield (seed 1) + one batch of 64 spheres r[4,16] 50/50."""
+    from voxel_amd.dag import terrain_for_world
+    depth, world = 7, 512
+    d = _dag(depth, materials, node_words=1 << 24, leaf_words=1 << 25, node_slots=1 << 21, leaf_slots=1 << 21)
+    d.build_terrain(terrain_for_world(world, 1))
+    br = O.gen_brushes(1, O.STREAM_BRUSH, 64, world, 0)
+    d.apply_brushes(br)
+    ref = O.Dag(depth, materials)
+    ref.build_terrain(O.terrain_for_world(world, 1))
+    ref.apply_brushes(br)
+    assert np.array_equal(d.export(), ref.export())

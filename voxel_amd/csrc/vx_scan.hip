@@ -201,27 +201,31 @@ vx_status run(uint32_t* data, uint32_t n, bool inclusive, Op op, void* ws, hipSt
 namespace {
 
 struct InplaceLoad {
+  using Payload = NoPayload;
   const uint32_t* p;
-  __device__ uint32_t operator()(uint32_t i) const { return p[i]; }
+  __device__ uint32_t operator()(uint32_t i, Payload&) const { return p[i]; }
 };
 
 template <Op OP>
 struct InplaceStoreOp {
   uint32_t* p;
   bool inclusive;
-  __device__ void operator()(uint32_t i, uint32_t, uint32_t excl, uint32_t incl) const { p[i] = inclusive ? incl : excl; }
+  __device__ void operator()(uint32_t i, const NoPayload&, uint32_t, uint32_t excl, uint32_t incl) const {
+    p[i] = inclusive ? incl : excl;
+  }
 };
 
 struct FlagLoad {
+  using Payload = NoPayload;
   const uint32_t* flags;
-  __device__ uint32_t operator()(uint32_t i) const { return flags[i] != 0u; }
+  __device__ uint32_t operator()(uint32_t i, Payload&) const { return flags[i] != 0u; }
 };
 struct CompactStore {
   const uint32_t* in;
   uint32_t* out;
   uint32_t* count;
   uint32_t n;
-  __device__ void operator()(uint32_t i, uint32_t v, uint32_t excl, uint32_t incl) const {
+  __device__ void operator()(uint32_t i, const NoPayload&, uint32_t v, uint32_t excl, uint32_t incl) const {
     if (v) out[excl] = in[i];
     if (i == n - 1) *count = incl;
   }

@@ -1,8 +1,11 @@
-// Fused single-pass scan: Load(i) -> scan -> Store(i, value, exclusive,
-// inclusive), the building block for stream compaction and for the DAG
-// pipeline's run/level compactions. Counterpart of the reference's
-// decoupled_lookback_fused_kernel (cuda/scan/fused.cuh:102-159), but with a
-// dynamic tile id and a wave-wide look-back window.
+// Fused single-pass scan: Load(i, payload) -> scan -> Store(i, payload,
+// value, exclusive, inclusive), the building block for stream compaction and
+// for the DAG pipeline's run/level compactions. Counterpart of the reference's
+// decoupled_lookback_fused_kernel (cuda/scan/fused.cuh:102-159), but with
+// dynamic tile ids, a wave-wide look-back window, a per-element payload that
+// travels from Load to Store in registers, and an optional device-resident
+// element count (persistent tiles: blocks claim tiles until the count is
+// covered), so dependent stages need no host round trip.
 #pragma once
 
 #include <algorithm>
@@ -19,13 +22,24 @@ vx_status compact(const uint32_t* in, const uint32_t* flags, uint32_t n, uint32_
 constexpr uint32_t kFusedBlock = 256;
 constexpr uint32_t kFusedItems = 8;
 constexpr uint32_t kFusedTile = kFusedBlock * kFusedItems;
+constexpr uint32_t kFusedMaxBlocks = 2048;
 
-inline size_t fused_workspace_bytes(uint32_t n) {
+struct NoPayload {};
+
+inline size_t fused_workspace_bytes(uint64_t n) {
   const uint32_t tiles = ceil_div(n, kFusedTile);
   if (tiles <= 1) return 0;
   Carver c(nullptr);
   c.take<uint32_t>(64);
   c.take<uint64_t>(tiles);
+  return c.size();
+}
+
+// Workspace for the device-count form: always carved (the count is unknown).
+inline size_t fused_dev_workspace_bytes(uint64_t n_upper) {
+  Carver c(nullptr);
+  c.take<uint32_t>(64);
+  c.take<uint64_t>(std::max<uint32_t>(1, ceil_div(n_upper, kFusedTile)));
   return c.size();
 }
 
@@ -37,7 +51,9 @@ __device__ __forceinline__ uint32_t fcombine(uint32_t a, uint32_t b) {
 
 // Decoupled look-back executed by one full wave: lane i reads the record of
 // tile (base - i); the nearest inclusive prefix ends the walk, and any
-// unpublished record before it makes the wave re-read.
+// unpublished record before it makes the wave re-read. Records pack the
+// status (bits 32..33: 1 aggregate, 2 inclusive prefix) over a 32-bit value
+// so one 64-bit agent-scope load observes both.
 template <Op OP>
 __device__ uint32_t lookback_window(const uint64_t* rec, uint32_t tile) {
   const uint32_t lane = lane_id();
@@ -62,25 +78,22 @@ __device__ uint32_t lookback_window(const uint64_t* rec, uint32_t tile) {
   return excl;
 }
 
+// One tile: kFusedBlock threads x kFusedItems elements, wave-segment order
+// (lane-contiguous per round, so every round of loads is coalesced).
 template <Op OP, class Load, class Store>
-__global__ __launch_bounds__(kFusedBlock) void fused_kernel(uint32_t n, Load ld, Store st, uint32_t* counter,
-                                                            uint64_t* rec) {
+__device__ __forceinline__ void fused_tile(uint32_t tile, uint32_t n, bool multi, const Load& ld, const Store& st,
+                                           uint64_t* rec, uint32_t* s_wave, uint32_t* s_prefix) {
   constexpr uint32_t kWaves = kFusedBlock / kWave;
-  __shared__ uint32_t s_wave[kWaves];
-  __shared__ uint32_t s_tile, s_prefix;
-  const bool multi = gridDim.x > 1;
-  if (threadIdx.x == 0) s_tile = multi ? atomicAdd(counter, 1u) : 0u;
-  __syncthreads();
-  const uint32_t tile = s_tile;
+  using P = typename Load::Payload;
   const uint32_t lane = lane_id(), wave = wave_id();
   const uint32_t seg = tile * kFusedTile + wave * (kWave * kFusedItems);
-
   uint32_t v[kFusedItems], ex[kFusedItems];
+  P pl[kFusedItems];
   uint32_t carry = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kFusedItems; ++k) {
     const uint32_t i = seg + k * kWave + lane;
-    v[k] = i < n ? ld(i) : 0u;
+    v[k] = i < n ? ld(i, pl[k]) : 0u;
     uint32_t inc;
     if constexpr (OP == kSum) inc = wave_inclusive_sum(v[k]);
     else inc = wave_inclusive_max(v[k]);
@@ -109,17 +122,49 @@ __global__ __launch_bounds__(kFusedBlock) void fused_kernel(uint32_t n, Load ld,
         if (lane == 0) st_agent(&rec[tile], (uint64_t(2) << 32) | fcombine<OP>(prefix, agg));
       }
     }
-    if (lane == 0) s_prefix = prefix;
+    if (lane == 0) *s_prefix = prefix;
   }
   __syncthreads();
-  const uint32_t prefix = fcombine<OP>(s_prefix, before);
+  const uint32_t prefix = fcombine<OP>(*s_prefix, before);
 #pragma unroll
   for (uint32_t k = 0; k < kFusedItems; ++k) {
     const uint32_t i = seg + k * kWave + lane;
     if (i < n) {
       const uint32_t e = fcombine<OP>(prefix, ex[k]);
-      st(i, v[k], e, fcombine<OP>(e, v[k]));
+      st(i, pl[k], v[k], e, fcombine<OP>(e, v[k]));
     }
+  }
+}
+
+template <Op OP, class Load, class Store>
+__global__ __launch_bounds__(kFusedBlock) void fused_kernel(uint32_t n, Load ld, Store st, uint32_t* counter,
+                                                            uint64_t* rec) {
+  __shared__ uint32_t s_wave[kFusedBlock / kWave];
+  __shared__ uint32_t s_tile, s_prefix;
+  const bool multi = gridDim.x > 1;
+  if (threadIdx.x == 0) s_tile = multi ? atomicAdd(counter, 1u) : 0u;
+  __syncthreads();
+  fused_tile<OP>(s_tile, n, multi, ld, st, rec, s_wave, &s_prefix);
+}
+
+// Persistent form: the element count is read from device memory; blocks claim
+// tiles until it is covered. Every block exits once the claimed tile lies past
+// the count, so no block waits on a tile nobody will process.
+template <Op OP, class Load, class Store>
+__global__ __launch_bounds__(kFusedBlock) void fused_dev_kernel(const uint32_t* n_ptr, uint32_t n_cap, Load ld,
+                                                                Store st, uint32_t* counter, uint64_t* rec) {
+  __shared__ uint32_t s_wave[kFusedBlock / kWave];
+  __shared__ uint32_t s_tile, s_prefix;
+  const uint32_t n = min(*n_ptr, n_cap);
+  const uint32_t tiles = (n + kFusedTile - 1) / kFusedTile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    __syncthreads();
+    if (tile >= tiles) break;
+    fused_tile<OP>(tile, n, tiles > 1, ld, st, rec, s_wave, &s_prefix);
+    __syncthreads();
   }
 }
 
@@ -139,6 +184,22 @@ vx_status fused(uint32_t n, Load ld, Store st, void* ws, hipStream_t s) {
     if (r != VX_OK) return r;
   }
   fused_kernel<OP, Load, Store><<<tiles, kFusedBlock, 0, s>>>(n, ld, st, counter, rec);
+  return last_launch();
+}
+
+// Host launcher for a device-resident count bounded by n_upper. `ws` must hold
+// fused_dev_workspace_bytes(n_upper) bytes.
+template <Op OP, class Load, class Store>
+vx_status fused_dev(const uint32_t* n_ptr, uint64_t n_upper, Load ld, Store st, void* ws, hipStream_t s) {
+  const uint32_t tiles = std::max<uint32_t>(1, ceil_div(n_upper, kFusedTile));
+  Carver c(ws);
+  uint32_t* counter = c.take<uint32_t>(64);
+  uint64_t* rec = c.take<uint64_t>(tiles);
+  const vx_status r = fill_u32(counter, (reinterpret_cast<char*>(rec + tiles) - reinterpret_cast<char*>(counter)) / 4, 0, s);
+  if (r != VX_OK) return r;
+  const uint32_t blocks = std::min(tiles, kFusedMaxBlocks);
+  const uint32_t n_cap = uint32_t(std::min<uint64_t>(n_upper, 0xFFFFFFFFu));
+  fused_dev_kernel<OP, Load, Store><<<blocks, kFusedBlock, 0, s>>>(n_ptr, n_cap, ld, st, counter, rec);
   return last_launch();
 }
 

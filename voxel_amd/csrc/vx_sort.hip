@@ -1,22 +1,25 @@
-// Onesweep LSD radix sort for gfx950: stable, in place, u32/u64 keys with an
-// optional u32 payload.
+// Onesweep LSD radix sort for gfx950: stable, u32/u64 keys with an optional
+// u32 payload.
 //
 // Replaces algo::cuda::sort::sort_keys / sort_by_key (cuda/sort/sort.cuh:100-167)
 // and follows the onesweep structure of cuda/sort/radix/onesweep/host.cuh:12-205:
 //   1. one read of the keys builds the digit histograms of every pass
 //      (global_offsets.cuh:52-99 — here LDS atomics + one global flush per block);
 //   2. per pass, one partition kernel: dynamic tile id, wave64 multisplit ranks
-//      (8 ballots over 64-bit lane masks instead of 32-bit warp masks,
+//      (RB ballots over 64-bit lane masks instead of 32-bit warp masks,
 //      warp_rank/multi_split.cuh:8-48), per-digit decoupled look-back
 //      (lookback.cuh:7-79), and an LDS-staged scatter so consecutive output
 //      positions of one digit are written by consecutive lanes.
 // Look-back records are u32 {status:2, value:30}, hence n < 2^30
 // (onesweep/host.cuh:85-86). Record rows of the next pass are cleared by the
-// current pass, so no memset runs between passes.
+// current pass, so no memset runs between passes. Partition blocks are
+// persistent (they claim tiles until the key count is covered), which also
+// lets the DAG pipeline sort a device-resident count without a host sync.
 #include <algorithm>
 #include <mutex>
 
 #include "vx_scan.hpp"
+#include "vx_sort.hpp"
 
 namespace vx {
 namespace sort {
@@ -42,13 +45,19 @@ struct Cfg {
   static constexpr bool kFeasible = LDS <= kMaxLds && R <= BLOCK;
 };
 
+__device__ __forceinline__ uint32_t count_of(const uint32_t* n_ptr, uint32_t n_arg) {
+  return n_ptr ? min(*n_ptr, n_arg) : n_arg;
+}
+
 // ---- pass 0: histograms of every digit position in one read ----------------
 template <class K, uint32_t RB>
-__global__ __launch_bounds__(256) void histogram_kernel(const K* __restrict__ keys, uint32_t n, uint32_t key_bits,
+__global__ __launch_bounds__(256) void histogram_kernel(const K* __restrict__ keys, const uint32_t* n_ptr,
+                                                        uint32_t n_arg, uint32_t key_bits,
                                                         uint32_t* __restrict__ hist) {
   constexpr uint32_t R = 1u << RB;
   constexpr uint32_t MAXP = (sizeof(K) * 8 + RB - 1) / RB;
   __shared__ uint32_t s[MAXP * R];
+  const uint32_t n = count_of(n_ptr, n_arg);
   const uint32_t passes = (key_bits + RB - 1) / RB;
   for (uint32_t i = threadIdx.x; i < passes * R; i += blockDim.x) s[i] = 0;
   __syncthreads();
@@ -89,8 +98,8 @@ __global__ __launch_bounds__(256) void hist_scan_kernel(uint32_t* hist) {
 template <class K, bool VALS, uint32_t RB, uint32_t BLOCK, uint32_t KPT>
 __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ kin, K* __restrict__ kout,
                                                           const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
-                                                          uint32_t n, uint32_t shift, uint32_t dmask,
-                                                          const uint32_t* __restrict__ pass_offsets,
+                                                          const uint32_t* n_ptr, uint32_t n_arg, uint32_t shift,
+                                                          uint32_t dmask, const uint32_t* __restrict__ pass_offsets,
                                                           uint32_t* tile_counter, uint32_t* rec, uint32_t* rec_next) {
   using C = Cfg<K, VALS, RB, BLOCK, KPT>;
   constexpr uint32_t R = C::R, WAVES = C::WAVES, TILE = C::TILE;
@@ -103,112 +112,119 @@ __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ 
   uint32_t* s_global = s_start + R;          // global base - tile start per digit
   uint32_t* s_misc = s_global + R;
 
+  const uint32_t n = count_of(n_ptr, n_arg);
+  const uint32_t tiles = (n + TILE - 1) / TILE;
   const uint32_t t = threadIdx.x, lane = lane_id(), wave = wave_id();
-  for (uint32_t i = t; i < WAVES * R; i += BLOCK) s_hist[i] = 0;
-  if (t == 0) s_misc[0] = atomicAdd(tile_counter, 1u);
-  __syncthreads();
-  const uint32_t tile = s_misc[0];
-  const uint64_t tile_base = uint64_t(tile) * TILE;
-  const uint64_t base = tile_base + uint64_t(wave) * kWave * KPT;
-
-  K key[KPT];
-  uint32_t val[KPT], dig[KPT], rank[KPT];
   uint32_t* wh = s_hist + wave * R;
+
+  for (;;) {
+    for (uint32_t i = t; i < WAVES * R; i += BLOCK) s_hist[i] = 0;
+    if (t == 0) s_misc[0] = atomicAdd(tile_counter, 1u);
+    __syncthreads();
+    const uint32_t tile = s_misc[0];
+    if (tile >= tiles) break;
+    const uint64_t tile_base = uint64_t(tile) * TILE;
+    const uint64_t base = tile_base + uint64_t(wave) * kWave * KPT;
+
+    K key[KPT];
+    uint32_t val[KPT], dig[KPT], rank[KPT];
 #pragma unroll
-  for (uint32_t j = 0; j < KPT; ++j) {
-    const uint64_t i = base + j * kWave + lane;
-    const bool valid = i < n;
-    key[j] = valid ? kin[i] : K(0);
-    if constexpr (VALS) val[j] = valid ? vin[i] : 0u;
-    dig[j] = digit_of(key[j], shift, dmask);
-  }
-#pragma unroll
-  for (uint32_t j = 0; j < KPT; ++j) {
-    const uint64_t i = base + j * kWave + lane;
-    const bool valid = i < n;
-    uint64_t peers = ballot(valid);
-#pragma unroll
-    for (uint32_t b = 0; b < RB; ++b) {
-      const bool bit = (dig[j] >> b) & 1u;
-      const uint64_t bal = ballot(bit);
-      peers &= bit ? bal : ~bal;
+    for (uint32_t j = 0; j < KPT; ++j) {
+      const uint64_t i = base + j * kWave + lane;
+      const bool valid = i < n;
+      key[j] = valid ? kin[i] : K(0);
+      if constexpr (VALS) val[j] = valid ? vin[i] : 0u;
+      dig[j] = digit_of(key[j], shift, dmask);
     }
-    const uint32_t below = lanes_below(peers);
-    uint32_t cnt = 0;
-    if (valid) cnt = wh[dig[j]];
-    rank[j] = cnt + below;
-    if (valid && below == 0) wh[dig[j]] = cnt + uint32_t(__popcll(peers));
-  }
-  __syncthreads();
-  // Per digit: exclusive scan over waves (wave order = input order) and
-  // publication of the tile aggregate.
-  for (uint32_t d = t; d < R; d += BLOCK) {
-    uint32_t sum = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < WAVES; ++w) {
-      const uint32_t c = s_hist[w * R + d];
-      s_hist[w * R + d] = sum;
-      sum += c;
-    }
-    s_cnt[d] = sum;
-    st_agent(&rec[size_t(tile) * R + d], (tile == 0 ? kStatusPrefix : kStatusAgg) | sum);
-  }
-  __syncthreads();
-  // Exclusive scan over digits of the tile counts (wave 0, R <= 256).
-  if (wave == 0) {
-    constexpr uint32_t PER = (R + kWave - 1) / kWave;
-    uint32_t loc[PER];
-    uint32_t sum = 0;
+    for (uint32_t j = 0; j < KPT; ++j) {
+      const uint64_t i = base + j * kWave + lane;
+      const bool valid = i < n;
+      uint64_t peers = ballot(valid);
 #pragma unroll
-    for (uint32_t q = 0; q < PER; ++q) {
-      const uint32_t d = lane * PER + q;
-      loc[q] = d < R ? s_cnt[d] : 0u;
-      sum += loc[q];
-    }
-    const uint32_t inc = wave_inclusive_sum(sum);
-    uint32_t run = inc - sum;
-#pragma unroll
-    for (uint32_t q = 0; q < PER; ++q) {
-      const uint32_t d = lane * PER + q;
-      if (d < R) s_start[d] = run;
-      run += loc[q];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (uint32_t j = 0; j < KPT; ++j) {
-    const uint64_t i = base + j * kWave + lane;
-    if (i < n) {
-      const uint32_t p = s_start[dig[j]] + wh[dig[j]] + rank[j];
-      s_keys[p] = key[j];
-      if constexpr (VALS) s_vals[p] = val[j];
-    }
-  }
-  // Per-digit look-back over predecessor tiles.
-  for (uint32_t d = t; d < R; d += BLOCK) {
-    uint32_t excl = 0;
-    if (tile > 0) {
-      int64_t j = int64_t(tile) - 1;
-      while (j >= 0) {
-        const uint32_t r = ld_agent(&rec[size_t(j) * R + d]);
-        const uint32_t st = r & ~kValueMask;
-        if (st == 0) continue;
-        excl += r & kValueMask;
-        if (st == kStatusPrefix) break;
-        --j;
+      for (uint32_t b = 0; b < RB; ++b) {
+        const bool bit = (dig[j] >> b) & 1u;
+        const uint64_t bal = ballot(bit);
+        peers &= bit ? bal : ~bal;
       }
-      st_agent(&rec[size_t(tile) * R + d], kStatusPrefix | (excl + s_cnt[d]));
+      const uint32_t below = lanes_below(peers);
+      uint32_t cnt = 0;
+      if (valid) cnt = wh[dig[j]];
+      rank[j] = cnt + below;
+      if (valid && below == 0) wh[dig[j]] = cnt + uint32_t(__popcll(peers));
     }
-    s_global[d] = pass_offsets[d] + excl - s_start[d];
-    rec_next[size_t(tile) * R + d] = 0u;
-  }
-  __syncthreads();
-  const uint32_t tile_n = uint32_t(min<uint64_t>(TILE, n - tile_base));
-  for (uint32_t p = t; p < tile_n; p += BLOCK) {
-    const K k = s_keys[p];
-    const uint32_t g = s_global[digit_of(k, shift, dmask)] + p;
-    kout[g] = k;
-    if constexpr (VALS) vout[g] = s_vals[p];
+    __syncthreads();
+    // Per digit: exclusive scan over waves (wave order = input order) and
+    // publication of the tile aggregate.
+    for (uint32_t d = t; d < R; d += BLOCK) {
+      uint32_t sum = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < WAVES; ++w) {
+        const uint32_t c = s_hist[w * R + d];
+        s_hist[w * R + d] = sum;
+        sum += c;
+      }
+      s_cnt[d] = sum;
+      st_agent(&rec[size_t(tile) * R + d], (tile == 0 ? kStatusPrefix : kStatusAgg) | sum);
+    }
+    __syncthreads();
+    // Exclusive scan over digits of the tile counts (wave 0, R <= 256).
+    if (wave == 0) {
+      constexpr uint32_t PER = (R + kWave - 1) / kWave;
+      uint32_t loc[PER];
+      uint32_t sum = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t d = lane * PER + q;
+        loc[q] = d < R ? s_cnt[d] : 0u;
+        sum += loc[q];
+      }
+      const uint32_t inc = wave_inclusive_sum(sum);
+      uint32_t run = inc - sum;
+#pragma unroll
+      for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t d = lane * PER + q;
+        if (d < R) s_start[d] = run;
+        run += loc[q];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; ++j) {
+      const uint64_t i = base + j * kWave + lane;
+      if (i < n) {
+        const uint32_t p = s_start[dig[j]] + wh[dig[j]] + rank[j];
+        s_keys[p] = key[j];
+        if constexpr (VALS) s_vals[p] = val[j];
+      }
+    }
+    // Per-digit look-back over predecessor tiles.
+    for (uint32_t d = t; d < R; d += BLOCK) {
+      uint32_t excl = 0;
+      if (tile > 0) {
+        int64_t j = int64_t(tile) - 1;
+        while (j >= 0) {
+          const uint32_t r = ld_agent(&rec[size_t(j) * R + d]);
+          const uint32_t st = r & ~kValueMask;
+          if (st == 0) continue;
+          excl += r & kValueMask;
+          if (st == kStatusPrefix) break;
+          --j;
+        }
+        st_agent(&rec[size_t(tile) * R + d], kStatusPrefix | (excl + s_cnt[d]));
+      }
+      s_global[d] = pass_offsets[d] + excl - s_start[d];
+      rec_next[size_t(tile) * R + d] = 0u;
+    }
+    __syncthreads();
+    const uint32_t tile_n = uint32_t(min<uint64_t>(TILE, n - tile_base));
+    for (uint32_t p = t; p < tile_n; p += BLOCK) {
+      const K k = s_keys[p];
+      const uint32_t g = s_global[digit_of(k, shift, dmask)] + p;
+      kout[g] = k;
+      if constexpr (VALS) vout[g] = s_vals[p];
+    }
+    __syncthreads();
   }
 }
 
@@ -228,8 +244,8 @@ template <class K, bool VALS>
 struct Launcher {
   size_t lds = 0;
   uint32_t tile = 0;
-  void (*kernel)(const K*, K*, const uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t*,
-                 uint32_t*, uint32_t*) = nullptr;
+  void (*kernel)(const K*, K*, const uint32_t*, uint32_t*, const uint32_t*, uint32_t, uint32_t, uint32_t,
+                 const uint32_t*, uint32_t*, uint32_t*, uint32_t*) = nullptr;
   uint32_t block = 0;
 };
 
@@ -267,13 +283,23 @@ bool pick1(Launcher<K, VALS>& L, uint32_t block, uint32_t kpt) {
   return false;
 }
 
-size_t device_max_lds() {
-  static size_t cached = 0;
-  if (!cached) {
+struct DeviceLimits {
+  size_t max_lds = 64 * 1024;
+  uint32_t cus = 256;
+};
+
+const DeviceLimits& limits() {
+  static DeviceLimits cached;
+  static bool done = false;
+  if (!done) {
     int dev = 0, v = 0;
-    hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 64 * 1024;
-    cached = size_t(v);
+    if (hipGetDevice(&dev) == hipSuccess) {
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+        cached.max_lds = size_t(v);
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+        cached.cus = uint32_t(v);
+    }
+    done = true;
   }
   return cached;
 }
@@ -283,7 +309,7 @@ bool pick(Launcher<K, VALS>& L, const Tuning& tn) {
   bool ok = false;
   if (tn.rb == 8) ok = pick1<K, VALS, 8>(L, tn.block, tn.kpt);
   else if (tn.rb == 4) ok = pick1<K, VALS, 4>(L, tn.block, tn.kpt);
-  return ok && L.lds <= device_max_lds();
+  return ok && L.lds <= limits().max_lds;
 }
 
 Tuning tuning_for(size_t key_bytes) {
@@ -298,20 +324,65 @@ struct Layout {
   size_t bytes;
 };
 
-Layout layout(void* ws, uint32_t n, size_t key_bytes, uint32_t key_bits, bool vals, const Tuning& tn, uint32_t tile) {
+Layout layout(void* ws, uint64_t n, size_t key_bytes, uint32_t key_bits, bool vals, const Tuning& tn, uint32_t tile,
+              bool carve_alt) {
   Layout L{};
   const uint32_t R = 1u << tn.rb;
   L.passes = (key_bits + tn.rb - 1) / tn.rb;
-  L.tiles = ceil_div(n, tile);
+  L.tiles = std::max<uint32_t>(1, ceil_div(n, tile));
   Carver c(ws);
   L.hist = c.take<uint32_t>(size_t(L.passes) * R);
   L.counters = c.take<uint32_t>(64);
   L.rec0 = c.take<uint32_t>(size_t(L.tiles) * R);
   L.rec1 = c.take<uint32_t>(size_t(L.tiles) * R);
-  L.alt_keys = c.take<char>(size_t(n) * key_bytes);
-  L.alt_vals = vals ? c.take<uint32_t>(n) : nullptr;
+  if (carve_alt) {
+    L.alt_keys = c.take<char>(size_t(n) * key_bytes);
+    L.alt_vals = vals ? c.take<uint32_t>(n) : nullptr;
+  }
   L.bytes = c.size();
   return L;
+}
+
+// Core driver. Sorts (keys, vals) using (alt_keys, alt_vals) as the second
+// buffer; *in_alt reports where the result ended up.
+template <class K, bool VALS>
+vx_status drive(K* keys, uint32_t* vals, K* alt_keys, uint32_t* alt_vals, const uint32_t* n_ptr, uint32_t n,
+                uint32_t key_bits, const Layout& lay, const Launcher<K, VALS>& L, const Tuning& tn, bool* in_alt,
+                hipStream_t s) {
+  const uint32_t R = 1u << tn.rb;
+  // hist, counters and rec0 are contiguous: clear them with one fill.
+  const size_t clear_words = (reinterpret_cast<char*>(lay.rec0 + size_t(lay.tiles) * R) - reinterpret_cast<char*>(lay.hist)) / 4;
+  vx_status st = fill_u32(lay.hist, clear_words, 0, s);
+  if (st != VX_OK) return st;
+  const uint32_t hblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(n, 256 * 16), 2048));
+  if (tn.rb == 8) histogram_kernel<K, 8><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, key_bits, lay.hist);
+  else histogram_kernel<K, 4><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, key_bits, lay.hist);
+  if ((st = last_launch()) != VX_OK) return st;
+  if (tn.rb == 8) hist_scan_kernel<8><<<lay.passes, 256, 0, s>>>(lay.hist);
+  else hist_scan_kernel<4><<<lay.passes, 256, 0, s>>>(lay.hist);
+  if ((st = last_launch()) != VX_OK) return st;
+
+  hipFuncSetAttribute(reinterpret_cast<const void*>(L.kernel), hipFuncAttributeMaxDynamicSharedMemorySize, int(L.lds));
+  const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>(uint32_t(160 * 1024 / L.lds), 2048 / L.block));
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(lay.tiles, limits().cus * per_cu));
+
+  K* src = keys;
+  K* dst = alt_keys;
+  uint32_t* vsrc = vals;
+  uint32_t* vdst = alt_vals;
+  for (uint32_t p = 0; p < lay.passes; ++p) {
+    const uint32_t shift = p * tn.rb;
+    const uint32_t bits = std::min(tn.rb, key_bits - shift);
+    uint32_t* rec = (p & 1) ? lay.rec1 : lay.rec0;
+    uint32_t* rec_next = (p & 1) ? lay.rec0 : lay.rec1;
+    L.kernel<<<grid, L.block, L.lds, s>>>(src, dst, vsrc, vdst, n_ptr, n, shift, (1u << bits) - 1,
+                                          lay.hist + size_t(p) * R, lay.counters + p, rec, rec_next);
+    if ((st = last_launch()) != VX_OK) return st;
+    std::swap(src, dst);
+    std::swap(vsrc, vdst);
+  }
+  *in_alt = src != keys;
+  return VX_OK;
 }
 
 template <class K, bool VALS>
@@ -320,7 +391,7 @@ size_t ws_bytes(uint32_t n, uint32_t key_bits) {
   const Tuning tn = tuning_for(sizeof(K));
   Launcher<K, VALS> L;
   if (!pick(L, tn)) return 0;
-  return layout(nullptr, n, sizeof(K), key_bits, VALS, tn, L.tile).bytes;
+  return layout(nullptr, n, sizeof(K), key_bits, VALS, tn, L.tile, true).bytes;
 }
 
 template <class K, bool VALS>
@@ -329,44 +400,18 @@ vx_status run(K* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, void* ws, 
   const Tuning tn = tuning_for(sizeof(K));
   Launcher<K, VALS> L;
   if (!pick(L, tn)) return VX_UNSUPPORTED;
-  const Layout lay = layout(ws, n, sizeof(K), key_bits, VALS, tn, L.tile);
+  const Layout lay = layout(ws, n, sizeof(K), key_bits, VALS, tn, L.tile, true);
   if (!ws || ws_size < lay.bytes) return VX_INVALID_VALUE;
-  const uint32_t R = 1u << tn.rb;
-  // hist, counters and rec0 are contiguous: clear them with one fill.
-  const size_t clear_words = (reinterpret_cast<char*>(lay.rec0 + size_t(lay.tiles) * R) - reinterpret_cast<char*>(lay.hist)) / 4;
-  vx_status st = fill_u32(lay.hist, clear_words, 0, s);
+  bool in_alt = false;
+  vx_status st = drive<K, VALS>(keys, vals, static_cast<K*>(lay.alt_keys), lay.alt_vals, nullptr, n, key_bits, lay, L,
+                                tn, &in_alt, s);
   if (st != VX_OK) return st;
-  const uint32_t hblocks = std::min<uint32_t>(ceil_div(n, 256 * 16), 2048);
-  if (tn.rb == 8) histogram_kernel<K, 8><<<hblocks, 256, 0, s>>>(keys, n, key_bits, lay.hist);
-  else histogram_kernel<K, 4><<<hblocks, 256, 0, s>>>(keys, n, key_bits, lay.hist);
-  if ((st = last_launch()) != VX_OK) return st;
-  if (tn.rb == 8) hist_scan_kernel<8><<<lay.passes, 256, 0, s>>>(lay.hist);
-  else hist_scan_kernel<4><<<lay.passes, 256, 0, s>>>(lay.hist);
-  if ((st = last_launch()) != VX_OK) return st;
-
-  hipFuncSetAttribute(reinterpret_cast<const void*>(L.kernel), hipFuncAttributeMaxDynamicSharedMemorySize, int(L.lds));
-
-  K* src = keys;
-  K* dst = static_cast<K*>(lay.alt_keys);
-  uint32_t* vsrc = vals;
-  uint32_t* vdst = lay.alt_vals;
-  for (uint32_t p = 0; p < lay.passes; ++p) {
-    const uint32_t shift = p * tn.rb;
-    const uint32_t bits = std::min(tn.rb, key_bits - shift);
-    uint32_t* rec = (p & 1) ? lay.rec1 : lay.rec0;
-    uint32_t* rec_next = (p & 1) ? lay.rec0 : lay.rec1;
-    L.kernel<<<lay.tiles, L.block, L.lds, s>>>(src, dst, vsrc, vdst, n, shift, (1u << bits) - 1, lay.hist + size_t(p) * R,
-                                               lay.counters + p, rec, rec_next);
-    if ((st = last_launch()) != VX_OK) return st;
-    std::swap(src, dst);
-    std::swap(vsrc, vdst);
-  }
-  if (src != keys) {
+  if (in_alt) {
     const uint32_t blocks = std::min<uint32_t>(ceil_div(n, 256), 8192);
-    copy_kernel<K><<<blocks, 256, 0, s>>>(src, keys, n);
+    copy_kernel<K><<<blocks, 256, 0, s>>>(static_cast<K*>(lay.alt_keys), keys, n);
     if ((st = last_launch()) != VX_OK) return st;
     if constexpr (VALS) {
-      copy_kernel<uint32_t><<<blocks, 256, 0, s>>>(vsrc, vals, n);
+      copy_kernel<uint32_t><<<blocks, 256, 0, s>>>(lay.alt_vals, vals, n);
       if ((st = last_launch()) != VX_OK) return st;
     }
   }
@@ -388,6 +433,24 @@ vx_status sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits
 vx_status sort_u64(uint64_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, void* ws, size_t ws_size, hipStream_t s) {
   return vals ? run<uint64_t, true>(keys, vals, n, key_bits, ws, ws_size, s)
               : run<uint64_t, false>(keys, nullptr, n, key_bits, ws, ws_size, s);
+}
+
+size_t pairs_u64_dev_workspace_bytes(uint64_t n_upper, uint32_t key_bits) {
+  const Tuning tn = tuning_for(8);
+  Launcher<uint64_t, true> L;
+  if (!pick(L, tn)) return 0;
+  return layout(nullptr, n_upper, 8, key_bits, true, tn, L.tile, false).bytes;
+}
+
+vx_status pairs_u64_dev(uint64_t* keys, uint32_t* vals, uint64_t* alt_keys, uint32_t* alt_vals, const uint32_t* n_ptr,
+                        uint64_t n_upper, uint32_t key_bits, void* ws, bool* in_alt, hipStream_t s) {
+  const Tuning tn = tuning_for(8);
+  Launcher<uint64_t, true> L;
+  if (!pick(L, tn)) return VX_UNSUPPORTED;
+  if (n_upper >= (1u << 30)) return VX_INVALID_VALUE;
+  const Layout lay = layout(ws, n_upper, 8, key_bits, true, tn, L.tile, false);
+  return drive<uint64_t, true>(keys, vals, alt_keys, alt_vals, n_ptr, uint32_t(n_upper), key_bits, lay, L, tn, in_alt,
+                               s);
 }
 
 vx_status set_tuning(uint32_t key_bytes, uint32_t rb, uint32_t kpt, uint32_t block) {

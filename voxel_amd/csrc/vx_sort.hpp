@@ -1,0 +1,26 @@
+// Internal entry points of the onesweep sort used by other translation units.
+#pragma once
+
+#include "vx_common.hpp"
+
+namespace vx {
+namespace sort {
+
+size_t workspace_bytes(uint32_t n, uint32_t key_bytes, uint32_t key_bits, bool vals);
+vx_status sort_u32(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, void* ws, size_t ws_size,
+                   hipStream_t s);
+vx_status sort_u64(uint64_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, void* ws, size_t ws_size,
+                   hipStream_t s);
+
+// (u64 key, u32 value) pairs whose count lives in device memory (*n_ptr <=
+// n_upper). The caller provides the second buffer pair; *in_alt reports
+// whether the sorted result is in (alt_keys, alt_vals).
+size_t pairs_u64_dev_workspace_bytes(uint64_t n_upper, uint32_t key_bits);
+vx_status pairs_u64_dev(uint64_t* keys, uint32_t* vals, uint64_t* alt_keys, uint32_t* alt_vals, const uint32_t* n_ptr,
+                        uint64_t n_upper, uint32_t key_bits, void* ws, bool* in_alt, hipStream_t s);
+
+vx_status set_tuning(uint32_t key_bytes, uint32_t rb, uint32_t kpt, uint32_t block);
+vx_status get_tuning(uint32_t key_bytes, uint32_t* rb, uint32_t* kpt, uint32_t* block);
+
+}  // namespace sort
+}  // namespace vx

@@ -1,0 +1,437 @@
+// Heightfield -> Hash DAG builder for gfx950 (scene setup for every edit
+// workload). The reference builds scenes only through CPU box edits
+// (svt/hash_dag.hpp:343-443); at 16384^3+ that is ~10^12 voxels, so the scene
+// is built hierarchically instead:
+//   1. heights h(x, z) for every column (integer fixed point, so the CPU
+//      oracle computes identical values) and a min/max mip pyramid;
+//   2. top-down frontier: a node is EMPTY if y0 > max h over its columns,
+//      SOLID if y0 + extent - 1 <= min h, PARTIAL otherwise; only PARTIAL
+//      nodes are expanded (one fused-scan compaction per level);
+//   3. bottom-up interning of partial leaves and nodes through the same probe
+//      and commit kernels as the edit. SOLID children are FULL in an
+//      occupancy DAG; in a materials DAG they are the shared subtree for their
+//      (level, y0) (material 1 + y mod 255), built first, level by level.
+#include <vector>
+
+#include "vx_dag.hpp"
+#include "vx_scan.hpp"
+
+using namespace vx;
+using namespace vx::dag;
+
+namespace {
+
+constexpr uint64_t kStreamTerrain = 16;
+
+__device__ __forceinline__ int64_t smooth_q16(int64_t t) { return (t * t * (3 * 65536 - 2 * t)) >> 32; }
+
+__device__ __forceinline__ int64_t lattice(uint64_t base, uint32_t i, uint32_t j) {
+  return int64_t(stream_at(base, (uint64_t(i) << 32) | j) >> 47) - 65536;
+}
+
+// h(x,z) = base + floor(amp * sum_o 2^-o n_o(x,z)); n_o bilinear value noise
+// with smoothstep weights, everything in Q16 / Q20 integers.
+__global__ void height_kernel(vx_terrain t, uint32_t world, uint16_t* __restrict__ h) {
+  uint64_t bases[8];
+  const uint32_t oct = min(t.octaves, 8u);
+  for (uint32_t o = 0; o < oct; ++o) bases[o] = stream_base(t.seed, kStreamTerrain + o);
+  const uint64_t n = uint64_t(world) * world;
+  for (uint64_t idx = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n; idx += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t x = uint32_t(idx % world), z = uint32_t(idx / world);
+    int64_t total = 0;
+    for (uint32_t o = 0; o < oct; ++o) {
+      const uint32_t period = max(1u, t.period >> o);
+      const uint32_t ix = x / period, iz = z / period;
+      const int64_t sx = smooth_q16((int64_t(x % period) << 16) / period);
+      const int64_t sz = smooth_q16((int64_t(z % period) << 16) / period);
+      const int64_t a = lattice(bases[o], ix, iz), b = lattice(bases[o], ix + 1, iz);
+      const int64_t c = lattice(bases[o], ix, iz + 1), d = lattice(bases[o], ix + 1, iz + 1);
+      const int64_t u = a + (((b - a) * sx) >> 16);
+      const int64_t v = c + (((d - c) * sx) >> 16);
+      total += (u + (((v - u) * sz) >> 16)) << (4 - o);
+    }
+    const int64_t hv = int64_t(t.base) + ((int64_t(t.amp) * total) >> 20);
+    h[idx] = uint16_t(hv < 0 ? 0 : (hv > 65535 ? 65535 : hv));
+  }
+}
+
+__global__ void pyramid_kernel(const uint16_t* __restrict__ mn_in, const uint16_t* __restrict__ mx_in,
+                               uint16_t* __restrict__ mn, uint16_t* __restrict__ mx, uint32_t side) {
+  const uint64_t n = uint64_t(side) * side;
+  const uint32_t prev = side * 2;
+  for (uint64_t idx = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n; idx += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t x = uint32_t(idx % side), z = uint32_t(idx / side);
+    const uint64_t i0 = uint64_t(2 * z) * prev + 2 * x, i1 = i0 + prev;
+    mn[idx] = min(min(mn_in[i0], mn_in[i0 + 1]), min(mn_in[i1], mn_in[i1 + 1]));
+    mx[idx] = max(max(mx_in[i0], mx_in[i0 + 1]), max(mx_in[i1], mx_in[i1 + 1]));
+  }
+}
+
+struct Pyr {
+  const uint16_t* mn[32];
+  const uint16_t* mx[32];
+  uint32_t log_world;
+};
+
+enum : uint32_t { kClsEmpty = 0, kClsSolid = 1, kClsPartial = 2 };
+
+// Class of the cube at level `level` whose Morton key (over the level's
+// coordinates) is `key`.
+__device__ __forceinline__ uint32_t classify(const Pyr& p, uint32_t level, uint64_t key, uint32_t* y0_out) {
+  const uint32_t k = p.log_world - level;  // log2(extent)
+  const uint32_t cx = compact3(key), cy = compact3(key >> 1), cz = compact3(key >> 2);
+  const uint32_t side = 1u << level;
+  const uint32_t y0 = cy << k, e = 1u << k;
+  const uint64_t cell = uint64_t(cz) * side + cx;
+  *y0_out = y0;
+  if (y0 > p.mx[k][cell]) return kClsEmpty;
+  if (y0 + e - 1 <= p.mn[k][cell]) return kClsSolid;
+  return kClsPartial;
+}
+
+struct ExpandLoad {
+  using Payload = uint64_t;  // key | classes << 40 is too wide: keep key, recompute classes
+  const uint64_t* keys;
+  Pyr p;
+  uint32_t child_level;
+  __device__ uint32_t operator()(uint32_t j, uint64_t& key) const {
+    key = keys[j];
+    uint32_t cnt = 0, y0;
+    for (uint32_t c = 0; c < 8; ++c) cnt += classify(p, child_level, (key << 3) | c, &y0) == kClsPartial;
+    return cnt;
+  }
+};
+
+struct ExpandStore {
+  Pyr p;
+  uint32_t child_level;
+  uint32_t* cls;     // per node: 2 bits per child
+  uint32_t* first;   // per node: index of its first partial child
+  uint64_t* child_keys;
+  uint32_t* n_child;
+  const uint32_t* n_ptr;
+  __device__ void operator()(uint32_t j, const uint64_t& key, uint32_t, uint32_t excl, uint32_t incl) const {
+    uint32_t bits = 0, r = excl, y0;
+    for (uint32_t c = 0; c < 8; ++c) {
+      const uint64_t ck = (key << 3) | c;
+      const uint32_t k = classify(p, child_level, ck, &y0);
+      bits |= k << (2 * c);
+      if (k == kClsPartial) child_keys[r++] = ck;
+    }
+    cls[j] = bits;
+    first[j] = excl;
+    if (j == *n_ptr - 1) *n_child = incl;
+  }
+};
+
+template <uint32_t LW>
+__global__ void terrain_leaf_kernel(const uint32_t* n_ptr, const uint64_t* __restrict__ keys,
+                                    const uint16_t* __restrict__ h, uint32_t world, uint32_t* __restrict__ items,
+                                    uint64_t* __restrict__ hashes, uint64_t* __restrict__ res) {
+  const uint32_t n = *n_ptr;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint64_t key = keys[j];
+    const uint32_t x0 = compact3(key) * 4, y0 = compact3(key >> 1) * 4, z0 = compact3(key >> 2) * 4;
+    uint32_t w[LW];
+#pragma unroll
+    for (uint32_t k = 0; k < LW; ++k) w[k] = 0;
+#pragma unroll
+    for (uint32_t lz = 0; lz < 4; ++lz)
+#pragma unroll
+      for (uint32_t lx = 0; lx < 4; ++lx) {
+        const uint32_t hh = h[uint64_t(z0 + lz) * world + x0 + lx];
+#pragma unroll
+        for (uint32_t ly = 0; ly < 4; ++ly) {
+          const uint32_t y = y0 + ly;
+          if (y <= hh) {
+            const uint32_t bit = lx + 4 * ly + 16 * lz;
+            if constexpr (LW == 2) w[bit >> 5] |= 1u << (bit & 31);
+            else w[bit >> 2] |= (1u + y % 255u) << ((bit & 3) * 8);
+          }
+        }
+      }
+    uint32_t any = 0, all = 0xffffffffu;
+#pragma unroll
+    for (uint32_t k = 0; k < LW; ++k) {
+      any |= w[k];
+      all &= w[k];
+    }
+    if (!any) res[j] = res_make(kResFinal, kEmpty);
+    else if (LW == 2 && all == 0xffffffffu) res[j] = res_make(kResFinal, kFull);
+    else {
+      uint32_t* it = items + size_t(j) * LW;
+#pragma unroll
+      for (uint32_t k = 0; k < LW; ++k) it[k] = w[k];
+      hashes[j] = hash_words(w, LW);
+      res[j] = res_make(kResPending, 0);
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t solid_child(const uint64_t* solid_res, uint32_t occupancy) {
+  return occupancy ? kFull : 0u;
+}
+
+// Node of the terrain frontier: EMPTY / SOLID / PARTIAL children.
+__global__ void terrain_node_kernel(const uint32_t* n_ptr, const uint64_t* __restrict__ keys,
+                                    const uint32_t* __restrict__ cls, const uint32_t* __restrict__ first,
+                                    const uint64_t* __restrict__ child_res, const uint64_t* __restrict__ solid_res,
+                                    uint32_t child_level, uint32_t log_world, bool materials,
+                                    uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
+                                    uint64_t* __restrict__ res) {
+  const uint32_t n = *n_ptr;
+  const uint32_t k = log_world - child_level;  // log2(child extent)
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint64_t key = keys[j];
+    const uint32_t bits = cls[j];
+    uint32_t r = first[j];
+    uint32_t ch[8];
+    for (uint32_t c = 0; c < 8; ++c) {
+      const uint32_t cl = (bits >> (2 * c)) & 3u;
+      if (cl == kClsEmpty) ch[c] = kEmpty;
+      else if (cl == kClsPartial) ch[c] = resolve(child_res, r++);
+      else if (!materials) ch[c] = kFull;
+      else {
+        const uint32_t cy = compact3(((key << 3) | c) >> 1);  // child y index = y0 >> k
+        ch[c] = uint32_t(solid_res[cy]);
+      }
+    }
+    (void)k;
+    uint32_t* it = items + size_t(j) * kNodeStride;
+    uint32_t ref = 0;
+    const uint32_t w = make_node_item(ch, it, &ref);
+    if (!w) res[j] = res_make(kResFinal, ref);
+    else {
+      hashes[j] = hash_words(it, w);
+      res[j] = res_make(kResPending, 0);
+    }
+  }
+}
+
+// Solid subtrees of a materials DAG: at the leaf level one leaf per 4-voxel y
+// block; above, node y-index k has children 2k (lower) and 2k + 1 (upper).
+__global__ void solid_leaf_kernel(const uint32_t* n_ptr, uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
+                                  uint64_t* __restrict__ res) {
+  const uint32_t n = *n_ptr;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    uint32_t w[16];
+    for (uint32_t ly = 0; ly < 4; ++ly) {
+      const uint32_t m = 1u + (4 * k + ly) % 255u;
+      // bits with this ly: word index (lx + 4ly + 16lz) >> 2 = ly + 4lz
+      for (uint32_t lz = 0; lz < 4; ++lz) w[ly + 4 * lz] = m * 0x01010101u;
+    }
+    uint32_t* it = items + size_t(k) * 16;
+    for (uint32_t i = 0; i < 16; ++i) it[i] = w[i];
+    hashes[k] = hash_words(w, 16);
+    res[k] = res_make(kResPending, 0);
+  }
+}
+
+__global__ void solid_node_kernel(const uint32_t* n_ptr, const uint64_t* __restrict__ child_res,
+                                  uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
+                                  uint64_t* __restrict__ res) {
+  const uint32_t n = *n_ptr;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    uint32_t ch[8];
+    for (uint32_t c = 0; c < 8; ++c) ch[c] = resolve(child_res, 2 * k + ((c >> 1) & 1u));
+    uint32_t* it = items + size_t(k) * kNodeStride;
+    uint32_t ref = 0;
+    const uint32_t w = make_node_item(ch, it, &ref);
+    if (!w) res[k] = res_make(kResFinal, ref);
+    else {
+      hashes[k] = hash_words(it, w);
+      res[k] = res_make(kResPending, 0);
+    }
+  }
+}
+
+// Resolve DUP entries into plain refs so later levels can index them.
+__global__ void flatten_kernel(const uint32_t* n_ptr, const uint64_t* __restrict__ res, uint64_t* __restrict__ out) {
+  const uint32_t n = *n_ptr;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    out[k] = resolve(res, k);
+}
+
+__global__ void set_root_kernel(uint32_t* ctl, const uint64_t* res, uint32_t cls, const uint64_t* solid0) {
+  if (ctl[kCtlStatus] != kStatusOk) return;
+  if (cls == kClsPartial) ctl[kCtlRoot] = resolve(res, 0);
+  else if (cls == kClsEmpty) ctl[kCtlRoot] = kEmpty;
+  else ctl[kCtlRoot] = solid0 ? uint32_t(solid0[0]) : kFull;
+}
+
+__global__ void root_class_kernel(Pyr p, uint32_t* out) {
+  uint32_t y0;
+  *out = classify(p, 0, 0, &y0);
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+  template <class T>
+  T* get() const {
+    return static_cast<T*>(p);
+  }
+};
+
+vx_status dev_alloc(DevBuf& b, size_t bytes) {
+  return hipMalloc(&b.p, std::max<size_t>(bytes, 256)) == hipSuccess ? VX_OK : VX_OUT_OF_CAPACITY;
+}
+
+uint32_t read_u32(const uint32_t* p, hipStream_t s) {
+  uint32_t v = 0;
+  hipMemcpyAsync(&v, p, 4, hipMemcpyDeviceToHost, s);
+  hipStreamSynchronize(s);
+  return v;
+}
+
+}  // namespace
+
+extern "C" vx_status vx_dag_build_terrain(vx_dag* g, const vx_terrain* t, void* stream) {
+  if (!g || !t || t->octaves == 0 || t->octaves > 4) return VX_INVALID_VALUE;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  vx_status st = vx_dag_reset(g, stream);
+  if (st != VX_OK) return st;
+  const Dev& d = g->dev;
+  const uint32_t D = g->depth, log_world = D + 2, world = 1u << log_world;
+  const bool mats = g->materials;
+  const uint32_t LW = d.leaf_words;
+
+  // 1. heights + pyramid
+  DevBuf hbuf, pyr_buf;
+  const uint64_t cols = uint64_t(world) * world;
+  if ((st = dev_alloc(hbuf, cols * 2)) != VX_OK) return st;
+  height_kernel<<<4096, 256, 0, s>>>(*t, world, hbuf.get<uint16_t>());
+  if ((st = last_launch()) != VX_OK) return st;
+  uint64_t pyr_elems = 0;
+  for (uint32_t k = 1; k <= log_world; ++k) pyr_elems += uint64_t(world >> k) * (world >> k);
+  if ((st = dev_alloc(pyr_buf, pyr_elems * 4)) != VX_OK) return st;
+  Pyr p{};
+  p.log_world = log_world;
+  p.mn[0] = p.mx[0] = hbuf.get<uint16_t>();
+  uint16_t* cursor = pyr_buf.get<uint16_t>();
+  for (uint32_t k = 1; k <= log_world; ++k) {
+    const uint32_t side = world >> k;
+    uint16_t* mn = cursor;
+    uint16_t* mx = cursor + uint64_t(side) * side;
+    cursor = mx + uint64_t(side) * side;
+    pyramid_kernel<<<grid_for(uint64_t(side) * side), 256, 0, s>>>(p.mn[k - 1], p.mx[k - 1], mn, mx, side);
+    if ((st = last_launch()) != VX_OK) return st;
+    p.mn[k] = mn;
+    p.mx[k] = mx;
+  }
+
+  DevBuf ctl_buf;
+  if ((st = dev_alloc(ctl_buf, 64 * 4)) != VX_OK) return st;
+  uint32_t* cnt = ctl_buf.get<uint32_t>();  // [0..D] counts per level, [32] root class, [33..] scratch counts
+  if ((st = fill_u32(cnt, 64, 0, s)) != VX_OK) return st;
+
+  // 2. solid subtrees (materials): level D has 2^D entries, level l has 2^l.
+  std::vector<DevBuf> solid(D + 1);
+  const uint64_t solid_max = uint64_t(1) << D;
+  DevBuf items_s, hashes_s, fused_s;
+  if (mats) {
+    if ((st = dev_alloc(items_s, solid_max * 16 * 4)) != VX_OK) return st;
+    if ((st = dev_alloc(hashes_s, solid_max * 8)) != VX_OK) return st;
+    if ((st = dev_alloc(fused_s, scan::fused_dev_workspace_bytes(solid_max))) != VX_OK) return st;
+    std::vector<uint32_t> counts(D + 1);
+    for (uint32_t l = 0; l <= D; ++l) counts[l] = 1u << l;
+    hipMemcpyAsync(cnt + 33, counts.data(), (D + 1) * 4, hipMemcpyHostToDevice, s);
+    DevBuf raw;
+    if ((st = dev_alloc(raw, solid_max * 8)) != VX_OK) return st;
+    for (uint32_t l = D + 1; l-- > 0;) {
+      const uint64_t n = uint64_t(1) << l;
+      if ((st = dev_alloc(solid[l], n * 8)) != VX_OK) return st;
+      if (l == D) {
+        solid_leaf_kernel<<<grid_for(n), 256, 0, s>>>(cnt + 33 + l, items_s.get<uint32_t>(), hashes_s.get<uint64_t>(),
+                                                      raw.get<uint64_t>());
+        if ((st = last_launch()) != VX_OK) return st;
+        if ((st = intern(d, true, ItemSet{items_s.get<uint32_t>(), 16, 16}, hashes_s.get<uint64_t>(), cnt + 33 + l, n,
+                         raw.get<uint64_t>(), fused_s.p, s)) != VX_OK)
+          return st;
+      } else {
+        solid_node_kernel<<<grid_for(n), 256, 0, s>>>(cnt + 33 + l, solid[l + 1].get<uint64_t>(),
+                                                      items_s.get<uint32_t>(), hashes_s.get<uint64_t>(),
+                                                      raw.get<uint64_t>());
+        if ((st = last_launch()) != VX_OK) return st;
+        if ((st = intern(d, false, ItemSet{items_s.get<uint32_t>(), kNodeStride, 0}, hashes_s.get<uint64_t>(),
+                         cnt + 33 + l, n, raw.get<uint64_t>(), fused_s.p, s)) != VX_OK)
+          return st;
+      }
+      flatten_kernel<<<grid_for(n), 256, 0, s>>>(cnt + 33 + l, raw.get<uint64_t>(), solid[l].get<uint64_t>());
+      if ((st = last_launch()) != VX_OK) return st;
+    }
+  }
+
+  // 3. top-down frontier of partial nodes.
+  root_class_kernel<<<1, 1, 0, s>>>(p, cnt + 32);
+  if ((st = last_launch()) != VX_OK) return st;
+  const uint32_t root_cls = read_u32(cnt + 32, s);
+  std::vector<DevBuf> keys(D + 1), cls(D + 1), first(D + 1), res(D + 1);
+  std::vector<uint64_t> n_level(D + 1, 0);
+  if (root_cls == kClsPartial) {
+    n_level[0] = 1;
+    if ((st = dev_alloc(keys[0], 8)) != VX_OK) return st;
+    hipMemsetAsync(keys[0].p, 0, 8, s);
+    uint32_t one = 1;
+    hipMemcpyAsync(cnt + 0, &one, 4, hipMemcpyHostToDevice, s);
+    for (uint32_t l = 0; l < D; ++l) {
+      const uint64_t n = n_level[l];
+      if ((st = dev_alloc(cls[l], n * 4)) != VX_OK) return st;
+      if ((st = dev_alloc(first[l], n * 4)) != VX_OK) return st;
+      if ((st = dev_alloc(keys[l + 1], n * 8 * 8)) != VX_OK) return st;
+      DevBuf fws;
+      if ((st = dev_alloc(fws, scan::fused_dev_workspace_bytes(n))) != VX_OK) return st;
+      if ((st = scan::fused_dev<scan::kSum>(cnt + l, n, ExpandLoad{keys[l].get<uint64_t>(), p, l + 1},
+                                            ExpandStore{p, l + 1, cls[l].get<uint32_t>(), first[l].get<uint32_t>(),
+                                                        keys[l + 1].get<uint64_t>(), cnt + l + 1, cnt + l},
+                                            fws.p, s)) != VX_OK)
+        return st;
+      n_level[l + 1] = read_u32(cnt + l + 1, s);
+    }
+    // 4. bottom-up interning.
+    uint64_t max_n = 1;
+    for (uint32_t l = 0; l <= D; ++l) max_n = std::max(max_n, n_level[l]);
+    DevBuf items, hashes, fws;
+    if ((st = dev_alloc(items, max_n * std::max<uint32_t>(LW, kNodeStride) * 4)) != VX_OK) return st;
+    if ((st = dev_alloc(hashes, max_n * 8)) != VX_OK) return st;
+    if ((st = dev_alloc(fws, scan::fused_dev_workspace_bytes(max_n))) != VX_OK) return st;
+    for (uint32_t l = D + 1; l-- > 0;) {
+      const uint64_t n = n_level[l];
+      if ((st = dev_alloc(res[l], n * 8)) != VX_OK) return st;
+      if (n == 0) continue;
+      if (l == D) {
+        if (LW == 2)
+          terrain_leaf_kernel<2><<<grid_for(n), 256, 0, s>>>(cnt + l, keys[l].get<uint64_t>(), hbuf.get<uint16_t>(),
+                                                             world, items.get<uint32_t>(), hashes.get<uint64_t>(),
+                                                             res[l].get<uint64_t>());
+        else
+          terrain_leaf_kernel<16><<<grid_for(n), 256, 0, s>>>(cnt + l, keys[l].get<uint64_t>(), hbuf.get<uint16_t>(),
+                                                              world, items.get<uint32_t>(), hashes.get<uint64_t>(),
+                                                              res[l].get<uint64_t>());
+        if ((st = last_launch()) != VX_OK) return st;
+        if ((st = intern(d, true, ItemSet{items.get<uint32_t>(), LW, LW}, hashes.get<uint64_t>(), cnt + l, n,
+                         res[l].get<uint64_t>(), fws.p, s)) != VX_OK)
+          return st;
+      } else {
+        terrain_node_kernel<<<grid_for(n), 256, 0, s>>>(
+            cnt + l, keys[l].get<uint64_t>(), cls[l].get<uint32_t>(), first[l].get<uint32_t>(),
+            res[l + 1].get<uint64_t>(), mats ? solid[l + 1].get<uint64_t>() : nullptr, l + 1, log_world, mats,
+            items.get<uint32_t>(), hashes.get<uint64_t>(), res[l].get<uint64_t>());
+        if ((st = last_launch()) != VX_OK) return st;
+        if ((st = intern(d, false, ItemSet{items.get<uint32_t>(), kNodeStride, 0}, hashes.get<uint64_t>(), cnt + l, n,
+                         res[l].get<uint64_t>(), fws.p, s)) != VX_OK)
+          return st;
+      }
+    }
+    set_root_kernel<<<1, 1, 0, s>>>(d.ctl, res[0].get<uint64_t>(), root_cls, nullptr);
+  } else {
+    set_root_kernel<<<1, 1, 0, s>>>(d.ctl, nullptr, root_cls, mats ? solid[0].get<uint64_t>() : nullptr);
+  }
+  if ((st = last_launch()) != VX_OK) return st;
+  uint32_t status = 0;
+  hipMemcpyAsync(&status, d.ctl + kCtlStatus, 4, hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) return VX_DEVICE_ERROR;
+  return status == kStatusOk ? VX_OK : VX_OUT_OF_CAPACITY;
+}

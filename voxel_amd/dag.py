@@ -1,0 +1,161 @@
+"""Host mirror of the reference's ``algo::svt::hash_dag_gpu`` API over the C ABI.
+
+``HashDagGpu`` owns the device tables and pools (storage.cuh:16-105);
+``place_voxel_edits`` / ``destroy_voxel_edits`` / ``apply_voxel_edits`` and
+``apply_voxel_edits_workspace_size`` follow edit.cuh:16-61; ``get_voxel`` is a
+batched form of query.cuh:12-39. Brush batches, terrain builds and the
+canonical export extend the API for the synthetic workloads.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import check, lib, register
+from .primitives import _ptr, _stream
+
+PLACE = 0
+DESTROY = 1
+
+
+class Capacity(C.Structure):
+    _fields_ = [("node_words", C.c_uint64), ("leaf_words", C.c_uint64), ("node_slots", C.c_uint64),
+                ("leaf_slots", C.c_uint64)]
+
+
+class Terrain(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("base", C.c_uint32), ("amp", C.c_uint32), ("period", C.c_uint32),
+                ("octaves", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("node_words_used", C.c_uint64), ("leaf_words_used", C.c_uint64), ("node_slots", C.c_uint64),
+                ("leaf_slots", C.c_uint64), ("root", C.c_uint32), ("inner_depth", C.c_uint32),
+                ("leaf_words", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+BRUSH_DTYPE = np.dtype([("cx", "<i4"), ("cy", "<i4"), ("cz", "<i4"), ("hx", "<i4"), ("hy", "<i4"), ("hz", "<i4"),
+                        ("kind", "<u4"), ("material", "<u4")])
+
+_P = C.c_void_p
+_u32, _u64, _i32, _sz = C.c_uint32, C.c_uint64, C.c_int32, C.c_size_t
+register({
+    "vx_dag_create": (_i32, [_u32, _i32, _P, C.POINTER(_P)]),
+    "vx_dag_destroy": (_i32, [_P]),
+    "vx_dag_reset": (_i32, [_P, _P]),
+    "vx_dag_stats_get": (_i32, [_P, _P]),
+    "vx_dag_build_terrain": (_i32, [_P, _P, _P]),
+    "vx_dag_voxel_edits_workspace_size": (_i32, [_P, _u32, C.POINTER(_sz)]),
+    "vx_dag_apply_voxel_edits": (_i32, [_P, _P, _u32, _u32, _u32, _P, _sz, _P]),
+    "vx_dag_brush_workspace_size": (_i32, [_P, _P, _u32, C.POINTER(_sz)]),
+    "vx_dag_apply_brushes": (_i32, [_P, _P, _u32, _P, _sz, _P]),
+    "vx_dag_get_voxels": (_i32, [_P, _P, _u32, _P, _P]),
+    "vx_dag_export_canonical": (_i32, [_P, _P, _u64, C.POINTER(_u64)]),
+})
+
+
+def terrain_for_world(world: int, seed: int, octaves: int = 4) -> Terrain:
+    """The 512^3 heightfield (base 256, amplitude 64, period 64) scaled to `world`."""
+    return Terrain(seed, world // 2, world // 8, world // 8, octaves)
+
+
+class HashDagGpu:
+    """Device Hash DAG of side 4 << inner_depth (occupancy, or per-voxel materials)."""
+
+    def __init__(self, inner_depth: int, materials: bool = False, node_words: int = 1 << 22,
+                 leaf_words: int = 1 << 22, node_slots: int = 1 << 20, leaf_slots: int = 1 << 20):
+        self.h = C.c_void_p()
+        cap = Capacity(node_words, leaf_words, node_slots, leaf_slots)
+        check(lib().vx_dag_create(inner_depth, int(materials), C.byref(cap), C.byref(self.h)), "vx_dag_create")
+        self.inner_depth = inner_depth
+        self.world = 4 << inner_depth
+        self.materials = materials
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().vx_dag_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self, stream=None):
+        check(lib().vx_dag_reset(self.h, _stream(stream)), "vx_dag_reset")
+
+    def stats(self) -> Stats:
+        s = Stats()
+        check(lib().vx_dag_stats_get(self.h, C.byref(s)), "vx_dag_stats_get")
+        return s
+
+    def build_terrain(self, terrain: Terrain, stream=None):
+        check(lib().vx_dag_build_terrain(self.h, C.byref(terrain), _stream(stream)), "vx_dag_build_terrain")
+
+    # -- reference editing API (edit.cuh) -----------------------------------
+    def apply_voxel_edits_workspace_size(self, count: int) -> int:
+        n = C.c_size_t()
+        check(lib().vx_dag_voxel_edits_workspace_size(self.h, count, C.byref(n)), "workspace size")
+        return n.value
+
+    def apply_voxel_edits(self, edits: torch.Tensor | None, op: int, material: int = 1, workspace=None,
+                          stream=None, count: int | None = None):
+        """edits: int32 CUDA tensor [n, 3] of x, y, z (device memory)."""
+        n = (edits.shape[0] if edits is not None else 0) if count is None else count
+        need = self.apply_voxel_edits_workspace_size(n)
+        if workspace is None and need:
+            workspace = torch.empty(need, dtype=torch.uint8, device="cuda")
+        wsb = 0 if workspace is None else workspace.numel()
+        check(lib().vx_dag_apply_voxel_edits(self.h, _ptr(edits), n, op, material, _ptr(workspace), wsb,
+                                             _stream(stream)), "vx_dag_apply_voxel_edits")
+
+    def place_voxel_edits(self, edits, material: int = 1, **kw):
+        self.apply_voxel_edits(edits, PLACE, material, **kw)
+
+    def destroy_voxel_edits(self, edits, **kw):
+        self.apply_voxel_edits(edits, DESTROY, 0, **kw)
+
+    # -- brushes -------------------------------------------------------------
+    def brush_workspace_size(self, brushes: np.ndarray) -> int:
+        b = np.ascontiguousarray(brushes, BRUSH_DTYPE)
+        n = C.c_size_t()
+        check(lib().vx_dag_brush_workspace_size(self.h, b.ctypes.data_as(_P), len(b), C.byref(n)), "brush ws")
+        return n.value
+
+    def apply_brushes(self, brushes: np.ndarray, workspace=None, stream=None):
+        b = np.ascontiguousarray(brushes, BRUSH_DTYPE)
+        need = self.brush_workspace_size(b)
+        if workspace is None and need:
+            workspace = torch.empty(need, dtype=torch.uint8, device="cuda")
+        wsb = 0 if workspace is None else workspace.numel()
+        check(lib().vx_dag_apply_brushes(self.h, b.ctypes.data_as(_P), len(b), _ptr(workspace), wsb,
+                                         _stream(stream)), "vx_dag_apply_brushes")
+
+    # -- queries ------------------------------------------------------------------
+    def get_voxels(self, xyz: torch.Tensor, stream=None) -> torch.Tensor:
+        n = xyz.shape[0]
+        out = torch.empty(n, dtype=torch.int32, device=xyz.device)
+        check(lib().vx_dag_get_voxels(self.h, _ptr(xyz), n, _ptr(out), _stream(stream)), "vx_dag_get_voxels")
+        return out
+
+    def export(self) -> np.ndarray:
+        n = C.c_uint64()
+        check(lib().vx_dag_export_canonical(self.h, None, 0, C.byref(n)), "export size")
+        out = np.empty(n.value, np.uint32)
+        check(lib().vx_dag_export_canonical(self.h, out.ctypes.data_as(_P), n.value, C.byref(n)), "export")
+        return out
+
+
+def smoke_check(dev) -> None:
+    """Tiny end-to-end edit checked against the oracle (used by smoke())."""
+    from oracle import oracle as O
+    d = HashDagGpu(4, materials=True)
+    br = O.gen_brushes(3, O.STREAM_BRUSH, 12, d.world, 0)
+    d.apply_brushes(br)
+    ref = O.Dag(4, True)
+    ref.apply_brushes(br)
+    assert np.array_equal(d.export(), ref.export()), "GPU DAG differs from the oracle"
+    d.close()
