@@ -193,6 +193,33 @@ vx_status vx_dag_get_voxels(const vx_dag* dag, const vx_voxel* d_xyz, uint32_t n
  * *n_words receives the size; words are written only if cap suffices. */
 vx_status vx_dag_export_canonical(const vx_dag* dag, uint32_t* out, uint64_t cap, uint64_t* n_words);
 
+/* Seeded brush batch (host output). mix 0: spheres r in [4,16], 50% add;
+ * mix 1: 50% spheres r in [8,64] / 50% boxes half extents in [4,48], 60% add;
+ * added material uniform in [1,255]. Brush i uses draws 16i..16i+8 of the
+ * (seed, stream_id) splitmix64 stream. */
+vx_status vx_gen_brushes(uint64_t seed, uint64_t stream_id, uint32_t n, uint32_t world, uint32_t mix,
+                         vx_brush* out);
+
+/* Exact number of voxel operations of a brush batch: sum over brushes of the
+ * voxels each covers inside the world (overlaps count once per brush). */
+vx_status vx_brush_voxel_ops(const vx_brush* h_brushes, uint32_t n, uint32_t world, uint64_t* ops);
+
+/* Per-stage edit timing (hipEvents recorded on the edit's stream) and the
+ * sizes each stage worked on, accumulated over edits while enabled.
+ * Stages: 0 records, 1 sort, 2 level structure, 3 old-ref walk,
+ * 4 leaves (build + probe + commit), 5 nodes (all levels), 6 publish. */
+#define VX_DAG_STAGES 7
+typedef struct vx_dag_profile {
+  double ms[VX_DAG_STAGES];
+  uint64_t edits;        /* edit calls measured */
+  uint64_t records;      /* leaf-brick records (sum) */
+  uint64_t leaves;       /* distinct leaves touched (sum) */
+  uint64_t nodes;        /* distinct inner nodes touched (sum) */
+  uint64_t sort_passes;  /* radix passes per edit */
+} vx_dag_profile;
+vx_status vx_dag_profile_enable(vx_dag* dag, int32_t enable);
+vx_status vx_dag_profile_read(const vx_dag* dag, vx_dag_profile* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,7 @@
 // ever allocated for a duplicate, and no thread waits on another.
 // Status: first error wins (detail/edit_common.cuh:96-99); the root is
 // published only on success (detail/edit/root.cuh:13-25).
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <unordered_map>
@@ -464,6 +465,10 @@ EditWs carve(void* base, uint32_t depth, uint32_t leaf_words, uint64_t r_ub, uin
   return w;
 }
 
+inline void mark(vx_dag* g, uint32_t i, hipStream_t s) {
+  if (g->profile) hipEventRecord(g->ev[i], s);
+}
+
 // Everything after record generation: sort, levels, old refs, rebuild, publish.
 vx_status rebuild(vx_dag* g, EditWs& w, uint64_t r_ub, hipStream_t s) {
   const Dev& d = g->dev;
@@ -471,9 +476,11 @@ vx_status rebuild(vx_dag* g, EditWs& w, uint64_t r_ub, hipStream_t s) {
   uint32_t* cnt = w.counts;
   vx_status st;
   bool in_alt = false;
+  mark(g, 1, s);
   if ((st = sort::pairs_u64_dev(w.rkey, w.rval, w.rkey_alt, w.rval_alt, &cnt[0], std::max<uint64_t>(r_ub, 1), 3 * D,
                                 w.sort_ws, &in_alt, s)) != VX_OK)
     return st;
+  mark(g, 2, s);
   const uint64_t* skey = in_alt ? w.rkey_alt : w.rkey;
   const uint32_t* sval = in_alt ? w.rval_alt : w.rval;
   Levels& L = w.lv;
@@ -490,6 +497,7 @@ vx_status rebuild(vx_dag* g, EditWs& w, uint64_t r_ub, hipStream_t s) {
                                           w.fused_ws, s)) != VX_OK)
       return st;
   }
+  mark(g, 3, s);
   // Old refs, top-down.
   root_old_kernel<<<1, 1, 0, s>>>(d.ctl, &cnt[1], L.old[0]);
   if ((st = last_launch()) != VX_OK) return st;
@@ -498,6 +506,7 @@ vx_status rebuild(vx_dag* g, EditWs& w, uint64_t r_ub, hipStream_t s) {
                                                          L.old[dd - 1], L.old[dd]);
     if ((st = last_launch()) != VX_OK) return st;
   }
+  mark(g, 4, s);
   // Leaves.
   if (d.leaf_words == 2)
     leaf_build_kernel<2><<<grid_for(L.cap[D]), 256, 0, s>>>(d, &cnt[1 + D], L.old[D], L.first[D], sval, w.rmask,
@@ -509,6 +518,7 @@ vx_status rebuild(vx_dag* g, EditWs& w, uint64_t r_ub, hipStream_t s) {
   if ((st = intern(d, true, ItemSet{w.items, d.leaf_words, d.leaf_words}, w.hashes, &cnt[1 + D], L.cap[D], L.res[D],
                    w.fused_ws, s)) != VX_OK)
     return st;
+  mark(g, 5, s);
   // Nodes, bottom-up.
   for (uint32_t dd = D; dd-- > 0;) {
     node_build_kernel<<<grid_for(L.cap[dd]), 256, 0, s>>>(d, &cnt[1 + dd], L.old[dd], L.first[dd], L.keys[dd + 1],
@@ -518,14 +528,31 @@ vx_status rebuild(vx_dag* g, EditWs& w, uint64_t r_ub, hipStream_t s) {
                      w.fused_ws, s)) != VX_OK)
       return st;
   }
+  mark(g, 6, s);
   publish_kernel<<<1, 1, 0, s>>>(d.ctl, &cnt[1], L.res[0]);
-  return last_launch();
+  st = last_launch();
+  mark(g, 7, s);
+  return st;
 }
 
-vx_status finish(vx_dag* g, hipStream_t s) {
+vx_status finish(vx_dag* g, const EditWs& w, hipStream_t s) {
   uint32_t status = 0;
+  uint32_t counts[64];
   if (hipMemcpyAsync(&status, g->ctl + kCtlStatus, 4, hipMemcpyDeviceToHost, s) != hipSuccess) return VX_DEVICE_ERROR;
+  if (g->profile && hipMemcpyAsync(counts, w.counts, sizeof(counts), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return VX_DEVICE_ERROR;
   if (hipStreamSynchronize(s) != hipSuccess) return VX_DEVICE_ERROR;
+  if (g->profile) {
+    for (uint32_t i = 0; i < VX_DAG_STAGES; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, g->ev[i], g->ev[i + 1]) == hipSuccess) g->prof.ms[i] += ms;
+    }
+    g->prof.edits += 1;
+    g->prof.records += counts[0];
+    g->prof.leaves += counts[1 + g->depth];
+    for (uint32_t d = 0; d < g->depth; ++d) g->prof.nodes += counts[1 + d];
+    g->prof.sort_passes = (3 * g->depth + 7) / 8;
+  }
   return status == kStatusOk ? VX_OK : VX_OUT_OF_CAPACITY;
 }
 
@@ -706,6 +733,7 @@ extern "C" vx_status vx_dag_apply_voxel_edits(vx_dag* g, const vx_voxel* edits, 
   vx_status st;
   if ((st = fill_u32(w.counts, 64, 0, s)) != VX_OK) return st;
   clear_status_kernel<<<1, 1, 0, s>>>(g->ctl);
+  mark(g, 0, s);
   const uint32_t mat = op == 1 ? 0u : (g->materials ? (material & 0xffu) : 1u);
   if (op == 0 && g->materials && mat == 0) return VX_INVALID_VALUE;
   if ((st = scan::fused<scan::kSum>(count, VoxelLoad{edits, 4u << g->depth, mat},
@@ -713,7 +741,7 @@ extern "C" vx_status vx_dag_apply_voxel_edits(vx_dag* g, const vx_voxel* edits, 
                                     w.fused_ws, s)) != VX_OK)
     return st;
   if ((st = rebuild(g, w, count, s)) != VX_OK) return st;
-  return finish(g, s);
+  return finish(g, w, s);
 }
 
 extern "C" vx_status vx_dag_brush_workspace_size(const vx_dag* g, const vx_brush* b, uint32_t n, size_t* bytes) {
@@ -739,12 +767,13 @@ extern "C" vx_status vx_dag_apply_brushes(vx_dag* g, const vx_brush* b, uint32_t
     return VX_DEVICE_ERROR;
   if ((st = fill_u32(w.counts, 64, 0, s)) != VX_OK) return st;
   clear_status_kernel<<<1, 1, 0, s>>>(g->ctl);
+  mark(g, 0, s);
   if ((st = scan::fused<scan::kSum>(uint32_t(cand), BrushLoad{w.jobs, uint32_t(jobs.size()), g->materials},
                                     RecordStore{RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0]}, uint32_t(cand)},
                                     w.fused_ws, s)) != VX_OK)
     return st;
   if ((st = rebuild(g, w, cand, s)) != VX_OK) return st;
-  return finish(g, s);
+  return finish(g, w, s);
 }
 
 extern "C" vx_status vx_dag_get_voxels(const vx_dag* g, const vx_voxel* q, uint32_t n, uint32_t* out, void* stream) {
@@ -846,5 +875,60 @@ extern "C" vx_status vx_dag_export_canonical(const vx_dag* g, uint32_t* out, uin
     for (uint32_t v : r) out[k++] = v;
   for (uint32_t ref : leaf_refs)
     for (uint32_t i = 0; i < LW; ++i) out[k++] = lp[ref + i];
+  return VX_OK;
+}
+
+extern "C" vx_status vx_dag_profile_enable(vx_dag* g, int32_t enable) {
+  if (!g) return VX_INVALID_VALUE;
+  if (enable && !g->profile) {
+    for (auto& e : g->ev)
+      if (!e && hipEventCreate(&e) != hipSuccess) return VX_DEVICE_ERROR;
+  }
+  g->profile = enable != 0;
+  g->prof = vx_dag_profile{};
+  return VX_OK;
+}
+
+extern "C" vx_status vx_dag_profile_read(const vx_dag* g, vx_dag_profile* out) {
+  if (!g || !out) return VX_INVALID_VALUE;
+  *out = g->prof;
+  return VX_OK;
+}
+
+// Exact voxel count of each brush inside the world: spheres row by row
+// (integer square root of r^2 - dy^2 - dz^2), boxes as clipped extents.
+extern "C" vx_status vx_brush_voxel_ops(const vx_brush* b, uint32_t n, uint32_t world, uint64_t* ops) {
+  if (!ops || (!b && n)) return VX_INVALID_VALUE;
+  uint64_t total = 0;
+  auto clip_len = [&](int64_t lo, int64_t hi) -> int64_t {
+    lo = std::max<int64_t>(lo, 0);
+    hi = std::min<int64_t>(hi, int64_t(world) - 1);
+    return hi >= lo ? hi - lo + 1 : 0;
+  };
+  for (uint32_t i = 0; i < n; ++i) {
+    const vx_brush& br = b[i];
+    if (br.kind != 0) {
+      total += uint64_t(clip_len(br.cx - int64_t(br.hx), br.cx + int64_t(br.hx))) *
+               clip_len(br.cy - int64_t(br.hy), br.cy + int64_t(br.hy)) *
+               clip_len(br.cz - int64_t(br.hz), br.cz + int64_t(br.hz));
+      continue;
+    }
+    const int64_t r = br.hx;
+    if (r < 0) continue;
+    for (int64_t dz = -r; dz <= r; ++dz) {
+      const int64_t z = br.cz + dz;
+      if (z < 0 || z >= int64_t(world)) continue;
+      for (int64_t dy = -r; dy <= r; ++dy) {
+        const int64_t y = br.cy + dy;
+        const int64_t rem = r * r - dy * dy - dz * dz;
+        if (rem < 0 || y < 0 || y >= int64_t(world)) continue;
+        int64_t s = int64_t(std::sqrt(double(rem)));
+        while (s * s > rem) --s;
+        while ((s + 1) * (s + 1) <= rem) ++s;
+        total += uint64_t(clip_len(br.cx - s, br.cx + s));
+      }
+    }
+  }
+  *ops = total;
   return VX_OK;
 }

@@ -192,4 +192,8 @@ struct vx_dag {
   uint64_t* node_slots = nullptr;
   uint64_t* leaf_slots = nullptr;
   uint32_t* ctl = nullptr;
+  // Optional per-stage timing of edits (hipEvents on the edit's stream).
+  bool profile = false;
+  hipEvent_t ev[VX_DAG_STAGES + 1] = {};
+  vx_dag_profile prof{};
 };

@@ -96,3 +96,42 @@ extern "C" vx_status vx_device_info(int32_t* cu_count, int32_t* lds_bytes_per_bl
   if (hbm_bytes) *hbm_bytes = int64_t(p.totalGlobalMem);
   return VX_OK;
 }
+
+// Brush batches of the edit workloads (host side; the batch is small).
+// Brush i draws r_k = stream_at(base, 16 i + k): k=0 kind, 1..3 centre,
+// 4..6 extents, 7 add-vs-remove, 8 material. Mix 0: spheres r in [4,16], 50%
+// add. Mix 1: 50% spheres r in [8,64], 50% boxes with half extents in [4,48],
+// 60% add. Added material uniform in [1,255]; removal writes material 0.
+extern "C" vx_status vx_gen_brushes(uint64_t seed, uint64_t stream_id, uint32_t n, uint32_t world, uint32_t mix,
+                                    vx_brush* out) {
+  if ((!out && n) || world == 0 || mix > 1) return VX_INVALID_VALUE;
+  const uint64_t base = vx::stream_base(seed, stream_id);
+  for (uint32_t i = 0; i < n; ++i) {
+    uint64_t r[9];
+    for (uint32_t k = 0; k < 9; ++k) r[k] = vx::stream_at(base, uint64_t(i) * 16 + k);
+    vx_brush b{};
+    b.cx = int32_t(vx::uniform_u32(r[1], 0, world - 1));
+    b.cy = int32_t(vx::uniform_u32(r[2], 0, world - 1));
+    b.cz = int32_t(vx::uniform_u32(r[3], 0, world - 1));
+    uint32_t add_pct;
+    if (mix == 0) {
+      b.kind = 0;
+      b.hx = int32_t(vx::uniform_u32(r[4], 4, 16));
+      add_pct = 50;
+    } else {
+      b.kind = vx::uniform_u32(r[0], 0, 1);
+      if (b.kind == 0) {
+        b.hx = int32_t(vx::uniform_u32(r[4], 8, 64));
+      } else {
+        b.hx = int32_t(vx::uniform_u32(r[4], 4, 48));
+        b.hy = int32_t(vx::uniform_u32(r[5], 4, 48));
+        b.hz = int32_t(vx::uniform_u32(r[6], 4, 48));
+      }
+      add_pct = 60;
+    }
+    const bool add = vx::uniform_u32(r[7], 0, 99) < add_pct;
+    b.material = add ? vx::uniform_u32(r[8], 1, 255) : 0u;
+    out[i] = b;
+  }
+  return VX_OK;
+}

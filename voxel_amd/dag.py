@@ -53,7 +53,36 @@ register({
     "vx_dag_apply_brushes": (_i32, [_P, _P, _u32, _P, _sz, _P]),
     "vx_dag_get_voxels": (_i32, [_P, _P, _u32, _P, _P]),
     "vx_dag_export_canonical": (_i32, [_P, _P, _u64, C.POINTER(_u64)]),
+    "vx_brush_voxel_ops": (_i32, [_P, _u32, _u32, C.POINTER(_u64)]),
+    "vx_gen_brushes": (_i32, [_u64, _u64, _u32, _u32, _u32, _P]),
+    "vx_dag_profile_enable": (_i32, [_P, _i32]),
+    "vx_dag_profile_read": (_i32, [_P, _P]),
 })
+
+STAGES = ("records", "sort", "levels", "old_refs", "leaves", "nodes", "publish")
+
+
+class Profile(C.Structure):
+    _fields_ = [("ms", C.c_double * 7), ("edits", C.c_uint64), ("records", C.c_uint64), ("leaves", C.c_uint64),
+                ("nodes", C.c_uint64), ("sort_passes", C.c_uint64)]
+
+
+STREAM_BRUSH = 5
+
+
+def gen_brushes(seed: int, stream_id: int, n: int, world: int, mix: int) -> np.ndarray:
+    """Seeded brush batch (see vx_gen_brushes in include/vx.h)."""
+    out = np.zeros(n, BRUSH_DTYPE)
+    check(lib().vx_gen_brushes(seed, stream_id, n, world, mix, out.ctypes.data_as(_P)), "vx_gen_brushes")
+    return out
+
+
+def brush_voxel_ops(brushes: np.ndarray, world: int) -> int:
+    """Exact voxel operations of a brush batch (per-brush voxels inside the world)."""
+    b = np.ascontiguousarray(brushes, BRUSH_DTYPE)
+    n = C.c_uint64()
+    check(lib().vx_brush_voxel_ops(b.ctypes.data_as(_P), len(b), world, C.byref(n)), "vx_brush_voxel_ops")
+    return n.value
 
 
 def terrain_for_world(world: int, seed: int, octaves: int = 4) -> Terrain:
@@ -140,6 +169,14 @@ class HashDagGpu:
         out = torch.empty(n, dtype=torch.int32, device=xyz.device)
         check(lib().vx_dag_get_voxels(self.h, _ptr(xyz), n, _ptr(out), _stream(stream)), "vx_dag_get_voxels")
         return out
+
+    def profile(self, enable: bool = True):
+        check(lib().vx_dag_profile_enable(self.h, int(enable)), "vx_dag_profile_enable")
+
+    def profile_read(self) -> Profile:
+        p = Profile()
+        check(lib().vx_dag_profile_read(self.h, C.byref(p)), "vx_dag_profile_read")
+        return p
 
     def export(self) -> np.ndarray:
         n = C.c_uint64()
