@@ -104,6 +104,58 @@ vx_status vx_gen_unique_u64(uint64_t seed, uint64_t stream_id, uint64_t first,
 vx_status vx_iota_u32(uint32_t* out, uint32_t n, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Hash tables. Per-key status codes are the reference's
+ * (hash_table/slab_hash/common.cuh:18-25): 0 NotFound, 1 Inserted, 2 Replaced,
+ * 3 Erased, 4 Overflow, 5 InvalidKey, 6 Found, 7 AlreadyPresent. Status and
+ * result arrays are device memory and may be NULL where marked optional.
+ *
+ * Slab set of u64 keys: replaces SlabHashSetU32 (slab_hash/set/host.cuh:15-107,
+ * device.cuh:32-316) with 64-bit keys and 512-byte slabs (63 slots + link,
+ * one u64 per lane of a 64-lane wavefront). Keys ~0 and ~0-1 are reserved
+ * (InvalidKey). Overflow slabs come from a bump allocator.
+ * ------------------------------------------------------------------------- */
+typedef struct vx_slab_set vx_slab_set;
+vx_status vx_slab_set_create(uint32_t buckets, uint32_t overflow_slabs, vx_slab_set** out);
+vx_status vx_slab_set_destroy(vx_slab_set* set);
+vx_status vx_slab_set_clear(vx_slab_set* set, void* stream);
+vx_status vx_slab_set_insert(vx_slab_set* set, const uint64_t* d_keys, uint32_t n, uint32_t* d_status /*opt*/,
+                             void* stream);
+vx_status vx_slab_set_contains(const vx_slab_set* set, const uint64_t* d_keys, uint32_t n, uint8_t* d_found /*opt*/,
+                               uint32_t* d_status /*opt*/, void* stream);
+vx_status vx_slab_set_erase(vx_slab_set* set, const uint64_t* d_keys, uint32_t n, uint32_t* d_status /*opt*/,
+                            void* stream);
+vx_status vx_slab_set_slabs_used(const vx_slab_set* set, uint32_t* used);
+
+/* Slab map u32 -> u32: replaces SlabHashMapU32 (slab_hash/map/host.cuh:15-108,
+ * device.cuh:49-393); each slot packs key << 32 | value and changes by one
+ * 64-bit CAS. Keys 0xffffffff / 0xfffffffe are reserved. find writes
+ * 0xffffffff for missing keys. */
+typedef struct vx_slab_map vx_slab_map;
+vx_status vx_slab_map_create(uint32_t buckets, uint32_t overflow_slabs, vx_slab_map** out);
+vx_status vx_slab_map_destroy(vx_slab_map* map);
+vx_status vx_slab_map_clear(vx_slab_map* map, void* stream);
+vx_status vx_slab_map_insert_or_replace(vx_slab_map* map, const uint32_t* d_keys, const uint32_t* d_values,
+                                        uint32_t n, uint32_t* d_status /*opt*/, void* stream);
+vx_status vx_slab_map_find(const vx_slab_map* map, const uint32_t* d_keys, uint32_t* d_values, uint32_t n,
+                           uint32_t* d_status /*opt*/, void* stream);
+vx_status vx_slab_map_erase(vx_slab_map* map, const uint32_t* d_keys, uint32_t n, uint32_t* d_status /*opt*/,
+                            void* stream);
+
+/* Cuckoo set of u64 keys: replaces cuckoo_insert / cuckoo_find
+ * (cuda/cuckoo_hash.cuh:73-120) — four salted hash functions, eviction chains
+ * by 64-bit atomic exchange, the key itself stored in the slot. Keys of one
+ * insert call must be unique and not ~0; *d_failed (device u32) counts keys
+ * the table could not place within max_evictions (0 = default 512). */
+typedef struct vx_cuckoo_set vx_cuckoo_set;
+vx_status vx_cuckoo_set_create(uint64_t slots, uint32_t max_evictions, vx_cuckoo_set** out);
+vx_status vx_cuckoo_set_destroy(vx_cuckoo_set* set);
+vx_status vx_cuckoo_set_clear(vx_cuckoo_set* set, void* stream);
+vx_status vx_cuckoo_set_insert(vx_cuckoo_set* set, const uint64_t* d_keys, uint32_t n, uint32_t* d_failed,
+                               void* stream);
+vx_status vx_cuckoo_set_contains(const vx_cuckoo_set* set, const uint64_t* d_keys, uint32_t n, uint8_t* d_found,
+                                 void* stream);
+
+/* ---------------------------------------------------------------------------
  * Hash DAG. Replaces algo::svt::hash_dag_gpu (HashDagGpu storage.cuh:16-105,
  * apply/place/destroy_voxel_edits edit.cuh:16-61, get_voxel query.cuh:12-39)
  * with a runtime depth: world side 4 << inner_depth (inner_depth 1..13, i.e.

@@ -1,0 +1,524 @@
+// Slab hash set (u64 keys), slab hash map (u32 -> u32, packed 64-bit slots)
+// and cuckoo hash set (u64 keys) for gfx950.
+//
+// Slab tables follow algo::cuda slab_hash (slab_hash/set/device.cuh:32-316,
+// slab_hash/map/device.cuh:49-393): a bucket is a chain of slabs, one wave
+// works on one key at a time from a ballot work queue, every lane of the wave
+// reads one slot, and ballots find matches and empty slots. The slab is sized
+// to the 64-lane wavefront: 64 u64 words = 512 bytes, lanes 0..62 hold slots
+// and lane 63 holds the next-slab link (the reference's 32-lane slab holds 31
+// u32 slots + next). Status codes are the reference's
+// (slab_hash/common.cuh:18-25). Slabs beyond the bucket heads come from a
+// bump allocator (hash_table/bump_allocator.cuh:84-121).
+//
+// The cuckoo set follows cuckoo_hash.cuh:73-120 (four salted hash functions,
+// atomicExch eviction chains), storing the u64 key itself in the slot so one
+// 64-bit exchange moves a whole entry.
+#include "vx_common.hpp"
+
+namespace vx {
+namespace hash {
+
+constexpr uint64_t kEmpty64 = ~0ull;
+constexpr uint64_t kTomb64 = ~0ull - 1;
+constexpr uint32_t kNull = 0xffffffffu;
+constexpr uint32_t kSlotLanes = 63;
+
+enum : uint32_t {
+  kNotFound = 0,
+  kInserted = 1,
+  kReplaced = 2,
+  kErased = 3,
+  kOverflow = 4,
+  kInvalidKey = 5,
+  kFound = 6,
+  kAlreadyPresent = 7
+};
+
+struct SlabTable {
+  uint64_t* slabs;     // [slab][64]
+  uint32_t buckets;
+  uint32_t capacity;   // total slabs (buckets + overflow)
+  uint32_t* alloc;     // next free overflow slab
+};
+
+__host__ __device__ __forceinline__ uint32_t bucket_of(uint64_t key, uint32_t buckets) {
+  return uint32_t(mix64(key) % buckets);
+}
+
+// Bump allocation of an overflow slab (slabs are pre-cleared to ~0).
+__device__ __forceinline__ uint32_t alloc_slab(const SlabTable& t) {
+  const uint32_t id = atomicAdd(t.alloc, 1u);
+  return id < t.capacity ? id : kNull;
+}
+
+// ---- set --------------------------------------------------------------------
+enum class SetOp { kInsert, kContains, kErase };
+
+template <SetOp OP>
+__global__ void slab_set_kernel(SlabTable t, const uint64_t* __restrict__ keys, uint32_t n, uint8_t* found,
+                                uint32_t* status) {
+  const uint32_t lane = lane_id();
+  const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
+  const uint32_t wave_global = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  for (uint32_t base = wave_global * kWave; base < n; base += waves * kWave) {
+    const uint32_t i = base + lane;
+    uint64_t key = i < n ? keys[i] : 0;
+    bool active = i < n;
+    uint32_t st = kNotFound;
+    if (active && (key == kEmpty64 || key == kTomb64)) {
+      st = kInvalidKey;
+      active = false;
+    }
+    uint64_t queue = ballot(active);
+    while (queue) {
+      const uint32_t src = uint32_t(__builtin_ctzll(queue));
+      const uint64_t k = shfl(key, int(src));
+      uint32_t slab = bucket_of(k, t.buckets);
+      uint32_t result = kOverflow;
+      for (uint32_t guard = 0; guard < t.capacity * 4 + 64; ++guard) {
+        uint64_t* s = t.slabs + size_t(slab) * kWave;
+        const uint64_t v = ld_agent(&s[lane]);
+        const uint64_t match = ballot(lane < kSlotLanes && v == k);
+        if (match) {
+          if constexpr (OP == SetOp::kInsert) {
+            result = kAlreadyPresent;
+          } else if constexpr (OP == SetOp::kContains) {
+            result = kFound;
+          } else {
+            uint32_t ok = 0;
+            if (lane == uint32_t(__builtin_ctzll(match)))
+              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[lane]), (unsigned long long)k,
+                             (unsigned long long)kTomb64) == k;
+            if (!shfl(ok, int(__builtin_ctzll(match)))) continue;  // raced: re-read this slab
+            result = kErased;
+          }
+          break;
+        }
+        if constexpr (OP == SetOp::kInsert) {
+          const uint64_t empty = ballot(lane < kSlotLanes && v == kEmpty64);
+          if (empty) {
+            const uint32_t dst = uint32_t(__builtin_ctzll(empty));
+            uint32_t ok = 0;
+            if (lane == dst)
+              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[lane]), (unsigned long long)kEmpty64,
+                             (unsigned long long)k) == kEmpty64;
+            if (shfl(ok, int(dst))) {
+              result = kInserted;
+              break;
+            }
+            continue;  // lost the slot: re-read (the winner may hold the same key)
+          }
+        }
+        uint32_t next = uint32_t(shfl(v, 63));
+        if (next == kNull) {
+          if constexpr (OP != SetOp::kInsert) {
+            result = kNotFound;
+            break;
+          } else {
+            uint32_t fresh = kNull;
+            if (lane == 0) {
+              fresh = alloc_slab(t);
+              if (fresh != kNull) {
+                const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s[63]),
+                                                          (unsigned long long)kEmpty64, (unsigned long long)fresh);
+                if (prev != kEmpty64) fresh = uint32_t(prev);  // another wave linked first: follow it (ours leaks)
+              }
+            }
+            next = shfl(fresh, 0);
+            if (next == kNull) {
+              result = kOverflow;
+              break;
+            }
+          }
+        }
+        slab = next;
+      }
+      if (lane == src) {
+        st = result;
+        active = false;
+      }
+      queue = ballot(active);
+    }
+    if (i < n) {
+      if (status) status[i] = st;
+      if constexpr (OP == SetOp::kContains)
+        if (found) found[i] = st == kFound;
+    }
+  }
+}
+
+// ---- map (u32 -> u32 packed in one u64 slot) ----------------------------------
+enum class MapOp { kInsertOrReplace, kFind, kErase };
+
+__device__ __forceinline__ uint32_t slot_key(uint64_t v) { return uint32_t(v >> 32); }
+
+template <MapOp OP>
+__global__ void slab_map_kernel(SlabTable t, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals_in,
+                                uint32_t* vals_out, uint32_t n, uint32_t* status) {
+  const uint32_t lane = lane_id();
+  const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
+  const uint32_t wave_global = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  constexpr uint32_t kEmptyKey = 0xffffffffu, kTombKey = 0xfffffffeu;
+  for (uint32_t base = wave_global * kWave; base < n; base += waves * kWave) {
+    const uint32_t i = base + lane;
+    const uint32_t key = i < n ? keys[i] : 0u;
+    const uint32_t val = (OP == MapOp::kInsertOrReplace && i < n) ? vals_in[i] : 0u;
+    bool active = i < n;
+    uint32_t st = kNotFound, outv = 0xffffffffu;
+    if (active && (key == kEmptyKey || key == kTombKey)) {
+      st = kInvalidKey;
+      active = false;
+    }
+    uint64_t queue = ballot(active);
+    while (queue) {
+      const uint32_t src = uint32_t(__builtin_ctzll(queue));
+      const uint32_t k = shfl(key, int(src));
+      const uint32_t kv = shfl(val, int(src));
+      const uint64_t packed = (uint64_t(k) << 32) | kv;
+      uint32_t slab = bucket_of(k, t.buckets);
+      uint32_t result = kOverflow, found_val = 0xffffffffu;
+      for (uint32_t guard = 0; guard < t.capacity * 4 + 64; ++guard) {
+        uint64_t* s = t.slabs + size_t(slab) * kWave;
+        const uint64_t v = ld_agent(&s[lane]);
+        const uint64_t match = ballot(lane < kSlotLanes && slot_key(v) == k);
+        if (match) {
+          const uint32_t at = uint32_t(__builtin_ctzll(match));
+          if constexpr (OP == MapOp::kFind) {
+            found_val = uint32_t(shfl(v, int(at)));
+            result = kFound;
+            break;
+          } else {
+            const uint64_t repl = OP == MapOp::kErase ? ((uint64_t(kTombKey) << 32) | 0xffffffffu) : packed;
+            uint32_t ok = 0;
+            if (lane == at)
+              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[lane]), (unsigned long long)v,
+                             (unsigned long long)repl) == v;
+            if (!shfl(ok, int(at))) continue;
+            result = OP == MapOp::kErase ? kErased : kReplaced;
+            break;
+          }
+        }
+        if constexpr (OP == MapOp::kInsertOrReplace) {
+          const uint64_t empty = ballot(lane < kSlotLanes && v == kEmpty64);
+          if (empty) {
+            const uint32_t dst = uint32_t(__builtin_ctzll(empty));
+            uint32_t ok = 0;
+            if (lane == dst)
+              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[lane]), (unsigned long long)kEmpty64,
+                             (unsigned long long)packed) == kEmpty64;
+            if (shfl(ok, int(dst))) {
+              result = kInserted;
+              break;
+            }
+            continue;
+          }
+        }
+        uint32_t next = uint32_t(shfl(v, 63));
+        if (next == kNull) {
+          if constexpr (OP != MapOp::kInsertOrReplace) {
+            result = kNotFound;
+            break;
+          } else {
+            uint32_t fresh = kNull;
+            if (lane == 0) {
+              fresh = alloc_slab(t);
+              if (fresh != kNull) {
+                const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s[63]),
+                                                          (unsigned long long)kEmpty64, (unsigned long long)fresh);
+                if (prev != kEmpty64) fresh = uint32_t(prev);
+              }
+            }
+            next = shfl(fresh, 0);
+            if (next == kNull) {
+              result = kOverflow;
+              break;
+            }
+          }
+        }
+        slab = next;
+      }
+      if (lane == src) {
+        st = result;
+        outv = found_val;
+        active = false;
+      }
+      queue = ballot(active);
+    }
+    if (i < n) {
+      if (status) status[i] = st;
+      if constexpr (OP == MapOp::kFind) vals_out[i] = outv;
+    }
+  }
+}
+
+// ---- cuckoo -------------------------------------------------------------------
+constexpr uint32_t kCuckooHashes = 4;
+
+__device__ __forceinline__ uint64_t cuckoo_slot(uint64_t key, uint32_t h, uint64_t slots) {
+  constexpr uint64_t salts[kCuckooHashes] = {0x9e3779b97f4a7c15ull, 0x85ebca6bc2b2ae35ull, 0x27d4eb2f165667b1ull,
+                                             0xd6e8feb86659fd93ull};
+  return __umul64hi(mix64(key ^ salts[h]), slots);
+}
+
+__global__ void cuckoo_insert_kernel(uint64_t* table, uint64_t slots, const uint64_t* __restrict__ keys, uint32_t n,
+                                     uint32_t max_evictions, uint32_t* failed) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint64_t cur = keys[i];
+    if (cur == kEmpty64) {
+      atomicAdd(failed, 1u);
+      continue;
+    }
+    uint32_t h = 0;
+    bool placed = false;
+    for (uint32_t a = 0; a < max_evictions; ++a) {
+      const uint64_t slot = cuckoo_slot(cur, h, slots);
+      const uint64_t prev = atomicExch(reinterpret_cast<unsigned long long*>(&table[slot]), (unsigned long long)cur);
+      if (prev == kEmpty64) {
+        placed = true;
+        break;
+      }
+      // The evicted key sat at one of its own hash slots: continue it with
+      // the next function after that one (cuckoo_hash.cuh:104-110).
+      cur = prev;
+      uint32_t j = 0;
+      while (j < kCuckooHashes && cuckoo_slot(cur, j, slots) != slot) ++j;
+      h = (j + 1) % kCuckooHashes;
+    }
+    if (!placed) atomicAdd(failed, 1u);
+  }
+}
+
+__global__ void cuckoo_find_kernel(const uint64_t* table, uint64_t slots, const uint64_t* __restrict__ keys,
+                                   uint32_t n, uint8_t* found) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t k = keys[i];
+    uint8_t f = 0;
+#pragma unroll
+    for (uint32_t h = 0; h < kCuckooHashes; ++h)
+      if (table[cuckoo_slot(k, h, slots)] == k) f = 1;
+    found[i] = k == kEmpty64 ? 0 : f;
+  }
+}
+
+inline uint32_t grid_for(uint64_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(n, 256), 8192)); }
+
+}  // namespace hash
+}  // namespace vx
+
+using namespace vx;
+using namespace vx::hash;
+
+struct vx_slab_table {
+  SlabTable t{};
+  uint64_t* mem = nullptr;
+  uint32_t* alloc = nullptr;
+};
+struct vx_slab_set : vx_slab_table {};
+struct vx_slab_map : vx_slab_table {};
+struct vx_cuckoo_set {
+  uint64_t* table = nullptr;
+  uint64_t slots = 0;
+  uint32_t max_evictions = 0;
+};
+
+namespace {
+
+vx_status slab_create(vx_slab_table* h, uint32_t buckets, uint32_t overflow) {
+  if (buckets == 0) return VX_INVALID_VALUE;
+  const uint64_t cap = uint64_t(buckets) + overflow;
+  if (cap >= kNull) return VX_INVALID_VALUE;
+  if (hipMalloc(&h->mem, cap * kWave * 8) != hipSuccess) return VX_OUT_OF_CAPACITY;
+  if (hipMalloc(&h->alloc, 256) != hipSuccess) return VX_OUT_OF_CAPACITY;
+  h->t = SlabTable{h->mem, buckets, uint32_t(cap), h->alloc};
+  return VX_OK;
+}
+
+__global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
+
+vx_status slab_clear(vx_slab_table* h, hipStream_t s) {
+  vx_status st = fill_u32(reinterpret_cast<uint32_t*>(h->mem), uint64_t(h->t.capacity) * kWave * 2, 0xffffffffu, s);
+  if (st != VX_OK) return st;
+  set_u32_kernel<<<1, 1, 0, s>>>(h->alloc, h->t.buckets);
+  return last_launch();
+}
+
+void slab_free(vx_slab_table* h) {
+  if (h->mem) hipFree(h->mem);
+  if (h->alloc) hipFree(h->alloc);
+}
+
+uint32_t slab_grid(uint32_t n) {
+  return std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(n, 256), 4096));
+}
+
+}  // namespace
+
+// ---- C ABI: slab set ---------------------------------------------------------------
+extern "C" vx_status vx_slab_set_create(uint32_t buckets, uint32_t overflow_slabs, vx_slab_set** out) {
+  if (!out) return VX_INVALID_VALUE;
+  auto* h = new vx_slab_set();
+  vx_status st = slab_create(h, buckets, overflow_slabs);
+  if (st == VX_OK) st = slab_clear(h, nullptr);
+  if (st == VX_OK) st = from_hip(hipDeviceSynchronize());
+  if (st != VX_OK) {
+    slab_free(h);
+    delete h;
+    return st;
+  }
+  *out = h;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_slab_set_destroy(vx_slab_set* h) {
+  if (!h) return VX_INVALID_VALUE;
+  slab_free(h);
+  delete h;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_slab_set_clear(vx_slab_set* h, void* stream) {
+  if (!h) return VX_INVALID_VALUE;
+  return slab_clear(h, static_cast<hipStream_t>(stream));
+}
+
+extern "C" vx_status vx_slab_set_insert(vx_slab_set* h, const uint64_t* keys, uint32_t n, uint32_t* status,
+                                        void* stream) {
+  if (!h || (!keys && n)) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_set_kernel<SetOp::kInsert><<<slab_grid(n), 256, 0, static_cast<hipStream_t>(stream)>>>(h->t, keys, n, nullptr,
+                                                                                              status);
+  return last_launch();
+}
+
+extern "C" vx_status vx_slab_set_contains(const vx_slab_set* h, const uint64_t* keys, uint32_t n, uint8_t* found,
+                                          uint32_t* status, void* stream) {
+  if (!h || (!keys && n) || (!found && !status && n)) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_set_kernel<SetOp::kContains><<<slab_grid(n), 256, 0, static_cast<hipStream_t>(stream)>>>(h->t, keys, n, found,
+                                                                                                status);
+  return last_launch();
+}
+
+extern "C" vx_status vx_slab_set_erase(vx_slab_set* h, const uint64_t* keys, uint32_t n, uint32_t* status,
+                                       void* stream) {
+  if (!h || (!keys && n)) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_set_kernel<SetOp::kErase><<<slab_grid(n), 256, 0, static_cast<hipStream_t>(stream)>>>(h->t, keys, n, nullptr,
+                                                                                             status);
+  return last_launch();
+}
+
+extern "C" vx_status vx_slab_set_slabs_used(const vx_slab_set* h, uint32_t* used) {
+  if (!h || !used) return VX_INVALID_VALUE;
+  if (hipDeviceSynchronize() != hipSuccess) return VX_DEVICE_ERROR;
+  if (hipMemcpy(used, h->alloc, 4, hipMemcpyDeviceToHost) != hipSuccess) return VX_DEVICE_ERROR;
+  *used = std::min(*used, h->t.capacity);
+  return VX_OK;
+}
+
+// ---- C ABI: slab map -----------------------------------------------------------------
+extern "C" vx_status vx_slab_map_create(uint32_t buckets, uint32_t overflow_slabs, vx_slab_map** out) {
+  if (!out) return VX_INVALID_VALUE;
+  auto* h = new vx_slab_map();
+  vx_status st = slab_create(h, buckets, overflow_slabs);
+  if (st == VX_OK) st = slab_clear(h, nullptr);
+  if (st == VX_OK) st = from_hip(hipDeviceSynchronize());
+  if (st != VX_OK) {
+    slab_free(h);
+    delete h;
+    return st;
+  }
+  *out = h;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_slab_map_destroy(vx_slab_map* h) {
+  if (!h) return VX_INVALID_VALUE;
+  slab_free(h);
+  delete h;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_slab_map_clear(vx_slab_map* h, void* stream) {
+  if (!h) return VX_INVALID_VALUE;
+  return slab_clear(h, static_cast<hipStream_t>(stream));
+}
+
+extern "C" vx_status vx_slab_map_insert_or_replace(vx_slab_map* h, const uint32_t* keys, const uint32_t* values,
+                                                   uint32_t n, uint32_t* status, void* stream) {
+  if (!h || (n && (!keys || !values))) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_map_kernel<MapOp::kInsertOrReplace><<<slab_grid(n), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      h->t, keys, values, nullptr, n, status);
+  return last_launch();
+}
+
+extern "C" vx_status vx_slab_map_find(const vx_slab_map* h, const uint32_t* keys, uint32_t* values, uint32_t n,
+                                      uint32_t* status, void* stream) {
+  if (!h || (n && (!keys || !values))) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_map_kernel<MapOp::kFind><<<slab_grid(n), 256, 0, static_cast<hipStream_t>(stream)>>>(h->t, keys, nullptr,
+                                                                                            values, n, status);
+  return last_launch();
+}
+
+extern "C" vx_status vx_slab_map_erase(vx_slab_map* h, const uint32_t* keys, uint32_t n, uint32_t* status,
+                                       void* stream) {
+  if (!h || (!keys && n)) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_map_kernel<MapOp::kErase><<<slab_grid(n), 256, 0, static_cast<hipStream_t>(stream)>>>(h->t, keys, nullptr,
+                                                                                             nullptr, n, status);
+  return last_launch();
+}
+
+// ---- C ABI: cuckoo set ------------------------------------------------------------------
+extern "C" vx_status vx_cuckoo_set_create(uint64_t slots, uint32_t max_evictions, vx_cuckoo_set** out) {
+  if (!out || slots < 8) return VX_INVALID_VALUE;
+  auto* h = new vx_cuckoo_set();
+  h->slots = slots;
+  h->max_evictions = max_evictions ? max_evictions : 512;
+  if (hipMalloc(&h->table, slots * 8) != hipSuccess) {
+    delete h;
+    return VX_OUT_OF_CAPACITY;
+  }
+  vx_status st = fill_u32(reinterpret_cast<uint32_t*>(h->table), slots * 2, 0xffffffffu, nullptr);
+  if (st == VX_OK) st = from_hip(hipDeviceSynchronize());
+  if (st != VX_OK) {
+    hipFree(h->table);
+    delete h;
+    return st;
+  }
+  *out = h;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_cuckoo_set_destroy(vx_cuckoo_set* h) {
+  if (!h) return VX_INVALID_VALUE;
+  if (h->table) hipFree(h->table);
+  delete h;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_cuckoo_set_clear(vx_cuckoo_set* h, void* stream) {
+  if (!h) return VX_INVALID_VALUE;
+  return fill_u32(reinterpret_cast<uint32_t*>(h->table), h->slots * 2, 0xffffffffu, static_cast<hipStream_t>(stream));
+}
+
+extern "C" vx_status vx_cuckoo_set_insert(vx_cuckoo_set* h, const uint64_t* keys, uint32_t n, uint32_t* d_failed,
+                                          void* stream) {
+  if (!h || !d_failed || (!keys && n)) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  cuckoo_insert_kernel<<<hash::grid_for(n), 256, 0, static_cast<hipStream_t>(stream)>>>(h->table, h->slots, keys, n,
+                                                                                        h->max_evictions, d_failed);
+  return last_launch();
+}
+
+extern "C" vx_status vx_cuckoo_set_contains(const vx_cuckoo_set* h, const uint64_t* keys, uint32_t n, uint8_t* found,
+                                            void* stream) {
+  if (!h || (n && (!keys || !found))) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  cuckoo_find_kernel<<<hash::grid_for(n), 256, 0, static_cast<hipStream_t>(stream)>>>(h->table, h->slots, keys, n,
+                                                                                      found);
+  return last_launch();
+}
