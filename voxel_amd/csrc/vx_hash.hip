@@ -52,6 +52,33 @@ __device__ __forceinline__ uint32_t alloc_slab(const SlabTable& t) {
   return id < t.capacity ? id : kNull;
 }
 
+// Link word values (low 32 bits of lane 63's word) besides slab ids.
+constexpr uint32_t kLinkLock = 0xfffffffeu;  // a wave is allocating the next slab
+constexpr uint32_t kLinkFull = 0xfffffffdu;  // the allocator was exhausted here
+
+// Called by the whole wave at a slab whose link is not a slab id. Lane 0 locks
+// the link, allocates, then publishes the id, so no slab is ever allocated
+// and lost to a race; a wave that finds the link locked waits for the locking
+// wave (another wave, which needs nothing from it to finish). Returns the next
+// slab or kNull when the allocator is exhausted.
+__device__ uint32_t link_next(const SlabTable& t, uint64_t* s) {
+  uint32_t link = kNull;
+  if (lane_id() == 0) {
+    uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s[63]), (unsigned long long)kEmpty64,
+                              (unsigned long long)(0xffffffff00000000ull | kLinkLock));
+    if (prev == kEmpty64) {
+      const uint32_t id = alloc_slab(t);
+      link = id == kNull ? kLinkFull : id;
+      st_agent(&s[63], id == kNull ? (0xffffffff00000000ull | kLinkFull) : uint64_t(id));
+    } else {
+      while (uint32_t(prev) == kLinkLock) prev = ld_agent(&s[63]);
+      link = uint32_t(prev);
+    }
+  }
+  link = shfl(link, 0);
+  return link >= kLinkFull ? kNull : link;
+}
+
 // ---- set --------------------------------------------------------------------
 enum class SetOp { kInsert, kContains, kErase };
 
@@ -111,21 +138,12 @@ __global__ void slab_set_kernel(SlabTable t, const uint64_t* __restrict__ keys, 
           }
         }
         uint32_t next = uint32_t(shfl(v, 63));
-        if (next == kNull) {
+        if (next >= kLinkFull) {  // null, being linked, or exhausted
           if constexpr (OP != SetOp::kInsert) {
             result = kNotFound;
             break;
           } else {
-            uint32_t fresh = kNull;
-            if (lane == 0) {
-              fresh = alloc_slab(t);
-              if (fresh != kNull) {
-                const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s[63]),
-                                                          (unsigned long long)kEmpty64, (unsigned long long)fresh);
-                if (prev != kEmpty64) fresh = uint32_t(prev);  // another wave linked first: follow it (ours leaks)
-              }
-            }
-            next = shfl(fresh, 0);
+            next = next == kLinkFull ? kNull : link_next(t, s);
             if (next == kNull) {
               result = kOverflow;
               break;
@@ -215,21 +233,12 @@ __global__ void slab_map_kernel(SlabTable t, const uint32_t* __restrict__ keys, 
           }
         }
         uint32_t next = uint32_t(shfl(v, 63));
-        if (next == kNull) {
+        if (next >= kLinkFull) {  // null, being linked, or exhausted
           if constexpr (OP != MapOp::kInsertOrReplace) {
             result = kNotFound;
             break;
           } else {
-            uint32_t fresh = kNull;
-            if (lane == 0) {
-              fresh = alloc_slab(t);
-              if (fresh != kNull) {
-                const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s[63]),
-                                                          (unsigned long long)kEmpty64, (unsigned long long)fresh);
-                if (prev != kEmpty64) fresh = uint32_t(prev);
-              }
-            }
-            next = shfl(fresh, 0);
+            next = next == kLinkFull ? kNull : link_next(t, s);
             if (next == kNull) {
               result = kOverflow;
               break;

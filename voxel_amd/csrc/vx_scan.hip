@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void scan_vec_kernel(uint32_t* __restrict__
         if (lane == 0) st_agent(&rec[0], (uint64_t(2) << 32) | agg);
       } else {
         if (lane == 0) st_agent(&rec[tile], (uint64_t(1) << 32) | agg);
-        prefix = lookback_window<OP>(rec, tile);
+        prefix = lookback_window<typename OpOf<OP>::type>(rec, tile);
         if (lane == 0) st_agent(&rec[tile], (uint64_t(2) << 32) | combine<OP>(prefix, agg));
       }
     }

@@ -3,9 +3,9 @@
 // for the DAG pipeline's run/level compactions. Counterpart of the reference's
 // decoupled_lookback_fused_kernel (cuda/scan/fused.cuh:102-159), but with
 // dynamic tile ids, a wave-wide look-back window, a per-element payload that
-// travels from Load to Store in registers, and an optional device-resident
-// element count (persistent tiles: blocks claim tiles until the count is
-// covered), so dependent stages need no host round trip.
+// travels from Load to Store in registers, an optional device-resident element
+// count (persistent tiles: blocks claim tiles until the count is covered, so
+// dependent stages need no host round trip), and a segmented monoid.
 #pragma once
 
 #include <algorithm>
@@ -26,6 +26,44 @@ constexpr uint32_t kFusedMaxBlocks = 2048;
 
 struct NoPayload {};
 
+// Monoids. A look-back record packs {status:2 at kStatusShift, value below}.
+struct SumOp {
+  using T = uint32_t;
+  static constexpr uint32_t kStatusShift = 32;
+  __host__ __device__ static T identity() { return 0u; }
+  __device__ static T combine(T a, T b) { return a + b; }
+};
+struct MaxOp {
+  using T = uint32_t;
+  static constexpr uint32_t kStatusShift = 32;
+  __host__ __device__ static T identity() { return 0u; }
+  __device__ static T combine(T a, T b) { return a > b ? a : b; }
+};
+// Segmented sum: bit 32 marks "a segment head is inside", bits 0..31 the sum
+// since the last head. combine((f1,v1),(f2,v2)) = (f1|f2, f2 ? v2 : v1+v2).
+struct SegSumOp {
+  using T = uint64_t;
+  static constexpr uint32_t kStatusShift = 40;
+  static constexpr uint64_t kHead = 1ull << 32;
+  __host__ __device__ static T identity() { return 0ull; }
+  __device__ static T combine(T a, T b) {
+    const uint64_t f = (a | b) & kHead;
+    return (b & kHead) ? (f | uint32_t(b)) : (f | uint32_t(uint32_t(a) + uint32_t(b)));
+  }
+  __host__ __device__ static T make(uint32_t v, bool head) { return (head ? kHead : 0ull) | v; }
+};
+
+template <Op OP>
+struct OpOf;
+template <>
+struct OpOf<kSum> {
+  using type = SumOp;
+};
+template <>
+struct OpOf<kMax> {
+  using type = MaxOp;
+};
+
 inline size_t fused_workspace_bytes(uint64_t n) {
   const uint32_t tiles = ceil_div(n, kFusedTile);
   if (tiles <= 1) return 0;
@@ -43,35 +81,43 @@ inline size_t fused_dev_workspace_bytes(uint64_t n_upper) {
   return c.size();
 }
 
-template <Op OP>
-__device__ __forceinline__ uint32_t fcombine(uint32_t a, uint32_t b) {
-  if constexpr (OP == kSum) return a + b;
-  else return a > b ? a : b;
+template <class M>
+__device__ __forceinline__ typename M::T wave_inclusive(typename M::T v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const typename M::T o = __shfl_up(v, d, kWave);
+    if (lane >= uint32_t(d)) v = M::combine(o, v);
+  }
+  return v;
 }
 
 // Decoupled look-back executed by one full wave: lane i reads the record of
 // tile (base - i); the nearest inclusive prefix ends the walk, and any
-// unpublished record before it makes the wave re-read. Records pack the
-// status (bits 32..33: 1 aggregate, 2 inclusive prefix) over a 32-bit value
-// so one 64-bit agent-scope load observes both.
-template <Op OP>
-__device__ uint32_t lookback_window(const uint64_t* rec, uint32_t tile) {
+// unpublished record before it makes the wave re-read. The window is folded
+// oldest-first, so non-commutative (segmented) monoids are exact.
+template <class M>
+__device__ typename M::T lookback_window(const uint64_t* rec, uint32_t tile) {
+  using T = typename M::T;
+  constexpr uint64_t kValMask = (uint64_t(1) << M::kStatusShift) - 1;
   const uint32_t lane = lane_id();
-  uint32_t excl = 0;
+  T excl = M::identity();
   int64_t base = int64_t(tile) - 1;
   while (base >= 0) {
     const int64_t idx = base - lane;
-    const uint64_t r = idx >= 0 ? ld_agent(&rec[idx]) : (uint64_t(2) << 32);
-    const uint32_t st = uint32_t(r >> 32);
+    const uint64_t r = idx >= 0 ? ld_agent(&rec[idx]) : (uint64_t(2) << M::kStatusShift);
+    const uint32_t st = uint32_t(r >> M::kStatusShift);
     const uint64_t pref = ballot(st == 2);
     const uint64_t inval = ballot(st == 0);
     const uint32_t first = pref ? uint32_t(__builtin_ctzll(pref)) : 64u;
     const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1);
     if (inval & need) continue;
-    uint32_t contrib = lane <= first ? uint32_t(r) : 0u;
-    if constexpr (OP == kSum) contrib = wave_inclusive_sum(contrib);
-    else contrib = wave_inclusive_max(contrib);
-    excl = fcombine<OP>(excl, shfl(contrib, 63));
+    const T mine = lane <= first ? T(r & kValMask) : M::identity();
+    // Reverse lane order (oldest first), then an inclusive scan: lane 63 holds
+    // rec[base-63] (+) ... (+) rec[base].
+    const T rev = __shfl(mine, int(63 - lane), kWave);
+    const T win = shfl(wave_inclusive<M>(rev), 63);
+    excl = M::combine(win, excl);
     if (first < 64) break;
     base -= 64;
   }
@@ -80,81 +126,82 @@ __device__ uint32_t lookback_window(const uint64_t* rec, uint32_t tile) {
 
 // One tile: kFusedBlock threads x kFusedItems elements, wave-segment order
 // (lane-contiguous per round, so every round of loads is coalesced).
-template <Op OP, class Load, class Store>
+template <class M, class Load, class Store>
 __device__ __forceinline__ void fused_tile(uint32_t tile, uint32_t n, bool multi, const Load& ld, const Store& st,
-                                           uint64_t* rec, uint32_t* s_wave, uint32_t* s_prefix) {
+                                           uint64_t* rec, typename M::T* s_wave, typename M::T* s_prefix) {
+  using T = typename M::T;
   constexpr uint32_t kWaves = kFusedBlock / kWave;
   using P = typename Load::Payload;
   const uint32_t lane = lane_id(), wave = wave_id();
   const uint32_t seg = tile * kFusedTile + wave * (kWave * kFusedItems);
-  uint32_t v[kFusedItems], ex[kFusedItems];
+  T v[kFusedItems], ex[kFusedItems];
   P pl[kFusedItems];
-  uint32_t carry = 0;
+  T carry = M::identity();
 #pragma unroll
   for (uint32_t k = 0; k < kFusedItems; ++k) {
     const uint32_t i = seg + k * kWave + lane;
-    v[k] = i < n ? ld(i, pl[k]) : 0u;
-    uint32_t inc;
-    if constexpr (OP == kSum) inc = wave_inclusive_sum(v[k]);
-    else inc = wave_inclusive_max(v[k]);
-    uint32_t e = __shfl_up(inc, 1, kWave);
-    if (lane == 0) e = 0;
-    ex[k] = fcombine<OP>(carry, e);
-    carry = fcombine<OP>(carry, shfl(inc, 63));
+    v[k] = i < n ? T(ld(i, pl[k])) : M::identity();
+    const T inc = wave_inclusive<M>(v[k]);
+    T e = __shfl_up(inc, 1, kWave);
+    if (lane == 0) e = M::identity();
+    ex[k] = M::combine(carry, e);
+    carry = M::combine(carry, shfl(inc, 63));
   }
   if (lane == 0) s_wave[wave] = carry;
   __syncthreads();
-  uint32_t before = 0, agg = 0;
+  T before = M::identity(), agg = M::identity();
 #pragma unroll
   for (uint32_t w = 0; w < kWaves; ++w) {
-    const uint32_t t = s_wave[w];
-    if (w < wave) before = fcombine<OP>(before, t);
-    agg = fcombine<OP>(agg, t);
+    const T t = s_wave[w];
+    if (w < wave) before = M::combine(before, t);
+    agg = M::combine(agg, t);
   }
   if (wave == 0) {
-    uint32_t prefix = 0;
+    T prefix = M::identity();
     if (multi) {
       if (tile == 0) {
-        if (lane == 0) st_agent(&rec[0], (uint64_t(2) << 32) | agg);
+        if (lane == 0) st_agent(&rec[0], (uint64_t(2) << M::kStatusShift) | uint64_t(agg));
       } else {
-        if (lane == 0) st_agent(&rec[tile], (uint64_t(1) << 32) | agg);
-        prefix = lookback_window<OP>(rec, tile);
-        if (lane == 0) st_agent(&rec[tile], (uint64_t(2) << 32) | fcombine<OP>(prefix, agg));
+        if (lane == 0) st_agent(&rec[tile], (uint64_t(1) << M::kStatusShift) | uint64_t(agg));
+        prefix = lookback_window<M>(rec, tile);
+        if (lane == 0) st_agent(&rec[tile], (uint64_t(2) << M::kStatusShift) | uint64_t(M::combine(prefix, agg)));
       }
     }
     if (lane == 0) *s_prefix = prefix;
   }
   __syncthreads();
-  const uint32_t prefix = fcombine<OP>(*s_prefix, before);
+  const T prefix = M::combine(*s_prefix, before);
 #pragma unroll
   for (uint32_t k = 0; k < kFusedItems; ++k) {
     const uint32_t i = seg + k * kWave + lane;
     if (i < n) {
-      const uint32_t e = fcombine<OP>(prefix, ex[k]);
-      st(i, pl[k], v[k], e, fcombine<OP>(e, v[k]));
+      const T e = M::combine(prefix, ex[k]);
+      st(i, pl[k], v[k], e, M::combine(e, v[k]));
     }
   }
 }
 
-template <Op OP, class Load, class Store>
+template <class M, class Load, class Store>
 __global__ __launch_bounds__(kFusedBlock) void fused_kernel(uint32_t n, Load ld, Store st, uint32_t* counter,
                                                             uint64_t* rec) {
-  __shared__ uint32_t s_wave[kFusedBlock / kWave];
-  __shared__ uint32_t s_tile, s_prefix;
+  __shared__ typename M::T s_wave[kFusedBlock / kWave];
+  __shared__ typename M::T s_prefix;
+  __shared__ uint32_t s_tile;
   const bool multi = gridDim.x > 1;
   if (threadIdx.x == 0) s_tile = multi ? atomicAdd(counter, 1u) : 0u;
   __syncthreads();
-  fused_tile<OP>(s_tile, n, multi, ld, st, rec, s_wave, &s_prefix);
+  fused_tile<M>(s_tile, n, multi, ld, st, rec, s_wave, &s_prefix);
 }
 
 // Persistent form: the element count is read from device memory; blocks claim
 // tiles until it is covered. Every block exits once the claimed tile lies past
 // the count, so no block waits on a tile nobody will process.
-template <Op OP, class Load, class Store>
+template <class M, class Load, class Store>
 __global__ __launch_bounds__(kFusedBlock) void fused_dev_kernel(const uint32_t* n_ptr, uint32_t n_cap, Load ld,
                                                                 Store st, uint32_t* counter, uint64_t* rec) {
-  __shared__ uint32_t s_wave[kFusedBlock / kWave];
-  __shared__ uint32_t s_tile, s_prefix;
+  __shared__ typename M::T s_wave[kFusedBlock / kWave];
+  __shared__ typename M::T s_prefix;
+  __shared__ uint32_t s_tile;
   const uint32_t n = min(*n_ptr, n_cap);
   const uint32_t tiles = (n + kFusedTile - 1) / kFusedTile;
   for (;;) {
@@ -163,15 +210,15 @@ __global__ __launch_bounds__(kFusedBlock) void fused_dev_kernel(const uint32_t* 
     const uint32_t tile = s_tile;
     __syncthreads();
     if (tile >= tiles) break;
-    fused_tile<OP>(tile, n, tiles > 1, ld, st, rec, s_wave, &s_prefix);
+    fused_tile<M>(tile, n, tiles > 1, ld, st, rec, s_wave, &s_prefix);
     __syncthreads();
   }
 }
 
 // Host launcher. `ws` must hold fused_workspace_bytes(n) bytes when n spans
 // more than one tile.
-template <Op OP, class Load, class Store>
-vx_status fused(uint32_t n, Load ld, Store st, void* ws, hipStream_t s) {
+template <class M, class Load, class Store>
+vx_status fused_m(uint32_t n, Load ld, Store st, void* ws, hipStream_t s) {
   if (n == 0) return VX_OK;
   const uint32_t tiles = ceil_div(n, kFusedTile);
   uint32_t* counter = nullptr;
@@ -183,14 +230,19 @@ vx_status fused(uint32_t n, Load ld, Store st, void* ws, hipStream_t s) {
     const vx_status r = fill_u32(counter, (reinterpret_cast<char*>(rec + tiles) - reinterpret_cast<char*>(counter)) / 4, 0, s);
     if (r != VX_OK) return r;
   }
-  fused_kernel<OP, Load, Store><<<tiles, kFusedBlock, 0, s>>>(n, ld, st, counter, rec);
+  fused_kernel<M, Load, Store><<<tiles, kFusedBlock, 0, s>>>(n, ld, st, counter, rec);
   return last_launch();
+}
+
+template <Op OP, class Load, class Store>
+vx_status fused(uint32_t n, Load ld, Store st, void* ws, hipStream_t s) {
+  return fused_m<typename OpOf<OP>::type>(n, ld, st, ws, s);
 }
 
 // Host launcher for a device-resident count bounded by n_upper. `ws` must hold
 // fused_dev_workspace_bytes(n_upper) bytes.
-template <Op OP, class Load, class Store>
-vx_status fused_dev(const uint32_t* n_ptr, uint64_t n_upper, Load ld, Store st, void* ws, hipStream_t s) {
+template <class M, class Load, class Store>
+vx_status fused_dev_m(const uint32_t* n_ptr, uint64_t n_upper, Load ld, Store st, void* ws, hipStream_t s) {
   const uint32_t tiles = std::max<uint32_t>(1, ceil_div(n_upper, kFusedTile));
   Carver c(ws);
   uint32_t* counter = c.take<uint32_t>(64);
@@ -199,8 +251,13 @@ vx_status fused_dev(const uint32_t* n_ptr, uint64_t n_upper, Load ld, Store st, 
   if (r != VX_OK) return r;
   const uint32_t blocks = std::min(tiles, kFusedMaxBlocks);
   const uint32_t n_cap = uint32_t(std::min<uint64_t>(n_upper, 0xFFFFFFFFu));
-  fused_dev_kernel<OP, Load, Store><<<blocks, kFusedBlock, 0, s>>>(n_ptr, n_cap, ld, st, counter, rec);
+  fused_dev_kernel<M, Load, Store><<<blocks, kFusedBlock, 0, s>>>(n_ptr, n_cap, ld, st, counter, rec);
   return last_launch();
+}
+
+template <Op OP, class Load, class Store>
+vx_status fused_dev(const uint32_t* n_ptr, uint64_t n_upper, Load ld, Store st, void* ws, hipStream_t s) {
+  return fused_dev_m<typename OpOf<OP>::type>(n_ptr, n_upper, ld, st, ws, s);
 }
 
 }  // namespace scan
