@@ -69,7 +69,7 @@ __device__ uint32_t link_next(const SlabTable& t, uint64_t* s) {
     if (prev == kEmpty64) {
       const uint32_t id = alloc_slab(t);
       link = id == kNull ? kLinkFull : id;
-      st_agent(&s[63], id == kNull ? (0xffffffff00000000ull | kLinkFull) : uint64_t(id));
+      st_agent(&s[63], id == kNull ? uint64_t(0xffffffff00000000ull | kLinkFull) : uint64_t(id));
     } else {
       while (uint32_t(prev) == kLinkLock) prev = ld_agent(&s[63]);
       link = uint32_t(prev);
