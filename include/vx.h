@@ -234,6 +234,18 @@ vx_status vx_dag_brush_workspace_size(const vx_dag* dag, const vx_brush* h_brush
 vx_status vx_dag_apply_brushes(vx_dag* dag, const vx_brush* h_brushes, uint32_t n, void* ws, size_t ws_bytes,
                                void* stream);
 
+/* Batched edit over several resident scenes in one pipeline (one segmented
+ * sort / scan over all batches; no counterpart in the reference, which edits
+ * one DAG per call). Batch i (h_brushes[i][0..counts[i]), host memory) applies
+ * to dags[i]; batches of one scene apply in array order (the result equals
+ * applying them one by one), different scenes are independent. All scenes must
+ * be of one kind (occupancy or materials). A scene whose tables overflow keeps
+ * its old root (VX_OUT_OF_CAPACITY is returned); the others publish. */
+vx_status vx_dag_batch_workspace_size(vx_dag* const* dags, const vx_brush* const* h_brushes, const uint32_t* counts,
+                                      uint32_t n_batches, size_t* bytes);
+vx_status vx_dag_apply_batches(vx_dag* const* dags, const vx_brush* const* h_brushes, const uint32_t* counts,
+                               uint32_t n_batches, void* ws, size_t ws_bytes, void* stream);
+
 /* get_voxel for n device coordinates: d_out[i] = material (1 = solid in an
  * occupancy DAG), 0 if empty or out of the world (query.cuh:12-39). */
 vx_status vx_dag_get_voxels(const vx_dag* dag, const vx_voxel* d_xyz, uint32_t n, uint32_t* d_out, void* stream);

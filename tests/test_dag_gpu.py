@@ -172,3 +172,38 @@ def test_config0_terrain_plus_spheres_matches_oracle(gpu, materials):
     ref.build_terrain(O.terrain_for_world(world, 1))
     ref.apply_brushes(br)
     assert np.array_equal(d.export(), ref.export())
+
+
+@pytest.mark.parametrize("materials", [False, True])
+def test_batched_scenes_equal_sequential_and_oracle(gpu, materials):
+    """Batches over scenes of different depths in one pipeline == each batch
+    applied in order (and == the CPU oracle)."""
+    from voxel_amd.dag import apply_batches, terrain_for_world
+    depths = [5, 7, 6]
+    dags = [_dag(dd, materials, node_words=1 << 22, leaf_words=1 << 23) for dd in depths]
+    refs = []
+    for k, (dd, d) in enumerate(zip(depths, dags)):
+        d.build_terrain(terrain_for_world(4 << dd, 10 + k))
+        r = O.Dag(dd, materials)
+        r.build_terrain(O.terrain_for_world(4 << dd, 10 + k))
+        refs.append(r)
+    order = [0, 1, 0, 2, 1, 1, 0, 2]
+    batches, targets = [], []
+    for i, s in enumerate(order):
+        br = O.gen_brushes(100 + i, O.STREAM_BRUSH, 5 + 3 * i, 4 << depths[s], 0)
+        batches.append(br)
+        targets.append(dags[s])
+        refs[s].apply_brushes(br)
+    apply_batches(targets, batches)
+    for s in range(3):
+        assert np.array_equal(dags[s].export(), refs[s].export()), s
+
+
+def test_batch_of_one_scene_matches_plain(gpu):
+    from voxel_amd.dag import apply_batches
+    a, b = _dag(6, True), _dag(6, True)
+    bs = [O.gen_brushes(7 + i, O.STREAM_BRUSH, 20, 256, 0) for i in range(3)]
+    apply_batches([a, a, a], bs)
+    for br in bs:
+        b.apply_brushes(br)
+    assert np.array_equal(a.export(), b.export())

@@ -1,33 +1,37 @@
-// Hash DAG edit for gfx950.
+// Hash DAG edit for gfx950 — one pipeline for one scene or many.
 //
 // Replaces the reference's hash_dag_gpu edit pipeline (detail/edit_pipeline.cuh:31-97)
-// with a level-compacted design:
+// with a level-compacted design that also batches many scenes into one launch
+// sequence (the reference edits one 1024^3 DAG per call):
 //   records   brushes (or voxels) -> one record per touched 4^3 leaf brick
-//             {leaf Morton key, voxel mask, material}, compacted in brush order
-//             by the fused scan (replaces pack_leaf_edit_records_kernel,
-//             build_leaf_masks/pipeline.cuh:16-36);
-//   sort      onesweep sort of (key, record index), 3*D key bits (stable, so
-//             records of one leaf stay in brush order: last writer wins);
-//   levels    run compaction -> distinct leaves; then, bottom-up, distinct
-//             parents per level with child ranges (replaces the per-depth
-//             request emission + binary-search parent link,
-//             detail/edit/requests.cuh:51-173) — every later kernel runs over
-//             exactly one level's entries, not over 9N threads;
-//   old refs  one top-down pass per level: each entry reads one parent item
+//             {(scene, leaf Morton) key, voxel mask, material}, compacted in
+//             brush order by the fused scan (replaces
+//             pack_leaf_edit_records_kernel, build_leaf_masks/pipeline.cuh:16-36);
+//   sort      onesweep sort of (key, record index): a segmented sort, the scene
+//             id sits above the Morton bits; stable, so records of one leaf
+//             stay in brush order (last writer wins);
+//   heights   run compaction -> distinct leaves (height 0); then per height the
+//             distinct parents with child ranges (replaces the per-depth request
+//             emission + binary-search parent link, detail/edit/requests.cuh:51-173);
+//             a scene whose root is at this height contributes its root entry
+//             and stops — every later kernel runs over exactly one height's
+//             entries, not over 9N threads;
+//   old refs  one top-down pass per height: each entry reads one parent item
 //             (replaces the per-request root walk, detail/edit/snapshots.cuh:14-53);
-//   rebuild   leaves then levels D-1..0: build item -> probe -> commit
+//   rebuild   leaves then heights 1..H: build item -> probe -> commit
 //             (replaces apply_leaf_masks / process_depth,
 //             detail/edit/leaf_update.cuh:52-109, detail/edit/rebuild.cuh:30-89).
 // Interning is a true insert-if-absent on a u64-slot table: the probe kernel
 // claims an empty slot by CAS with {fingerprint, kTag | item index}; an equal
 // item that meets the tagged slot becomes a duplicate of that representative
 // (the in-batch dedup the reference does with a temporary map,
-// detail/edit/temp_dedup.cuh:12-64); the commit kernel (a fused scan over the
-// representatives' widths) copies each representative into the pool at its
-// scanned offset and rewrites its slot with the pool offset. No pool space is
-// ever allocated for a duplicate, and no thread waits on another.
-// Status: first error wins (detail/edit_common.cuh:96-99); the root is
-// published only on success (detail/edit/root.cuh:13-25).
+// detail/edit/temp_dedup.cuh:12-64); the commit kernel — a segmented scan over
+// the representatives' widths, one segment per scene — copies each
+// representative into its scene's pool at the scanned offset and rewrites its
+// slot with the pool offset. No pool space is ever allocated for a duplicate,
+// and no thread waits on another.
+// Status: first error wins per scene (detail/edit_common.cuh:96-99); a scene's
+// root is published only if that scene saw no error (detail/edit/root.cuh:13-25).
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -44,6 +48,31 @@ using namespace vx::dag;
 namespace vx {
 namespace dag {
 
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr uint32_t kMaxHeights = 16;
+
+// Composite keys: scene id above `shift` bits of (leaf Morton >> 3h).
+struct Codec {
+  uint32_t shift;
+  __host__ __device__ __forceinline__ uint32_t scene(uint64_t k) const { return uint32_t(k >> shift); }
+  __host__ __device__ __forceinline__ uint64_t parent(uint64_t k) const {
+    const uint64_t m = (uint64_t(1) << shift) - 1;
+    return (k & ~m) | ((k & m) >> 3);
+  }
+};
+
+// The scene an entry belongs to: a device table indexed by the key's scene id,
+// or a single scene passed by value.
+struct Scenes {
+  const Dev* table;
+  const uint64_t* keys;
+  Codec codec;
+  Dev single;
+  __device__ __forceinline__ uint32_t id(uint32_t i) const { return table ? codec.scene(keys[i]) : 0u; }
+  __device__ __forceinline__ const Dev& of(uint32_t i) const { return table ? table[codec.scene(keys[i])] : single; }
+  __device__ __forceinline__ const Dev& at(uint32_t s) const { return table ? table[s] : single; }
+};
+
 // ---- interning --------------------------------------------------------------
 struct Table {
   uint64_t* slots;
@@ -51,15 +80,26 @@ struct Table {
   uint32_t* pool;
   uint64_t cap;
   uint32_t* used;
-  uint32_t* base_snapshot;
+  uint32_t* base;
 };
 
+__device__ __forceinline__ Table table_of(const Dev& d, bool leaves) {
+  return leaves ? Table{d.leaf_slots, d.leaf_mask, d.leaf_pool, d.leaf_cap, d.ctl + kCtlLeafUsed, d.ctl + kCtlLeafBase}
+                : Table{d.node_slots, d.node_mask, d.node_pool, d.node_cap, d.ctl + kCtlNodeUsed, d.ctl + kCtlNodeBase};
+}
+
 __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, const uint32_t* n_ptr, uint64_t* res,
-                             Table t, uint32_t* ctl) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *t.base_snapshot = *t.used;
+                             Scenes sc, uint32_t n_scenes, bool leaves) {
+  if (blockIdx.x == 0)
+    for (uint32_t s = threadIdx.x; s < n_scenes; s += blockDim.x) {
+      const Table t = table_of(sc.at(s), leaves);
+      *t.base = *t.used;
+    }
   const uint32_t n = *n_ptr;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (res_kind(res[i]) != kResPending) continue;
+    const Dev& d = sc.of(i);
+    const Table t = table_of(d, leaves);
     const uint32_t* it = set.items + size_t(i) * set.stride;
     const uint32_t w = set.width(it);
     const uint64_t h = hashes[i];
@@ -100,35 +140,42 @@ __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, c
       }
       idx = (idx + 1) & t.mask;
     }
-    if (!done) set_status(ctl, kStatusOverflow);
+    if (!done) set_status(d.ctl, kStatusOverflow);
     res[i] = out;
   }
 }
 
+// Commit: segmented scan (one segment per scene) of representative widths.
 struct CommitLoad {
   using Payload = uint64_t;
   const uint64_t* res;
   ItemSet set;
-  __device__ uint32_t operator()(uint32_t i, uint64_t& r) const {
+  Scenes sc;
+  __device__ uint64_t operator()(uint32_t i, uint64_t& r) const {
     r = res[i];
-    return res_kind(r) == kResRep ? set.width(set.items + size_t(i) * set.stride) : 0u;
+    const uint32_t w = res_kind(r) == kResRep ? set.width(set.items + size_t(i) * set.stride) : 0u;
+    const bool head = i == 0 || sc.id(i - 1) != sc.id(i);
+    return scan::SegSumOp::make(w, head);
   }
 };
 
 struct CommitStore {
   ItemSet set;
   uint64_t* res;
-  Table t;
-  uint32_t* ctl;
+  Scenes sc;
+  bool leaves;
   const uint32_t* n_ptr;
-  __device__ void operator()(uint32_t i, const uint64_t& r, uint32_t w, uint32_t excl, uint32_t incl) const {
-    const uint32_t base = *t.base_snapshot;
+  __device__ void operator()(uint32_t i, const uint64_t& r, uint64_t v, uint64_t, uint64_t incl) const {
+    const Dev& d = sc.of(i);
+    const Table t = table_of(d, leaves);
+    const uint32_t base = *t.base;
+    const uint32_t w = uint32_t(v), seg_incl = uint32_t(incl);
     if (w) {
       uint64_t* slot = &t.slots[uint32_t(r)];
       const uint64_t fp_hi = ld_agent(slot) & 0xFFFFFFFF00000000ull;
-      const uint64_t off = uint64_t(base) + excl;
+      const uint64_t off = uint64_t(base) + seg_incl - w;
       if (off + w > t.cap || off >= kTag) {
-        set_status(ctl, kStatusOverflow);
+        set_status(d.ctl, kStatusOverflow);
         st_agent(slot, fp_hi | kTomb);
         res[i] = res_make(kResFinal, kEmpty);
       } else {
@@ -138,28 +185,27 @@ struct CommitStore {
         res[i] = res_make(kResFinal, uint32_t(off));
       }
     }
-    if (i == *n_ptr - 1) {
-      const uint64_t used = uint64_t(base) + incl;
+    const uint32_t n = *n_ptr;
+    if (i == n - 1 || sc.id(i + 1) != sc.id(i)) {
+      const uint64_t used = uint64_t(base) + seg_incl;
       *t.used = used > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(used);
     }
   }
 };
 
-Table node_table(const Dev& d) {
-  return Table{d.node_slots, d.node_mask, d.node_pool, d.node_cap, d.ctl + kCtlNodeUsed, d.ctl + kCtlNodeBase};
-}
-Table leaf_table(const Dev& d) {
-  return Table{d.leaf_slots, d.leaf_mask, d.leaf_pool, d.leaf_cap, d.ctl + kCtlLeafUsed, d.ctl + kCtlLeafBase};
+vx_status intern_scenes(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet set, const uint64_t* hashes,
+                        const uint32_t* n_ptr, uint64_t n_ub, uint64_t* res, void* fused_ws, hipStream_t s) {
+  probe_kernel<<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, n_scenes, leaves);
+  vx_status st = last_launch();
+  if (st != VX_OK) return st;
+  return scan::fused_dev_m<scan::SegSumOp>(n_ptr, n_ub, CommitLoad{res, set, sc},
+                                           CommitStore{set, res, sc, leaves, n_ptr}, fused_ws, s);
 }
 
 vx_status intern(const Dev& d, bool leaves, ItemSet set, const uint64_t* hashes, const uint32_t* n_ptr, uint64_t n_ub,
                  uint64_t* res, void* fused_ws, hipStream_t s) {
-  const Table t = leaves ? leaf_table(d) : node_table(d);
-  probe_kernel<<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, t, d.ctl);
-  vx_status st = last_launch();
-  if (st != VX_OK) return st;
-  return scan::fused_dev<scan::kSum>(n_ptr, n_ub, CommitLoad{res, set}, CommitStore{set, res, t, d.ctl, n_ptr},
-                                     fused_ws, s);
+  const Scenes sc{nullptr, nullptr, Codec{63}, d};
+  return intern_scenes(sc, 1, leaves, set, hashes, n_ptr, n_ub, res, fused_ws, s);
 }
 
 // ---- records ----------------------------------------------------------------
@@ -168,7 +214,7 @@ struct BrushJob {
   int32_t bx0, by0, bz0;
   uint32_t nx, ny, nz;
   uint32_t start;
-  uint32_t pad;
+  uint32_t scene;
 };
 
 struct Record {
@@ -225,6 +271,7 @@ struct BrushLoad {
   const BrushJob* jobs;
   uint32_t n_jobs;
   bool materials;
+  Codec codec;
   __device__ uint32_t operator()(uint32_t i, Record& rec) const {
     uint32_t lo = 0, hi = n_jobs;  // last job with start <= i
     while (hi - lo > 1) {
@@ -237,7 +284,7 @@ struct BrushLoad {
     const uint32_t bx = local % j.nx, by = (local / j.nx) % j.ny, bz = local / (j.nx * j.ny);
     const uint32_t gx = j.bx0 + bx, gy = j.by0 + by, gz = j.bz0 + bz;
     rec.mask = brick_mask(j.b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4));
-    rec.key = morton3(gx, gy, gz);
+    rec.key = (uint64_t(j.scene) << codec.shift) | morton3(gx, gy, gz);
     rec.mat = materials ? j.b.material : (j.b.material ? 1u : 0u);
     return rec.mask != 0;
   }
@@ -272,28 +319,69 @@ struct RecordStore {
   }
 };
 
-// ---- level structure --------------------------------------------------------
-struct KeyLoad {
+// ---- height structure -------------------------------------------------------
+// Height 0: distinct leaf keys of the sorted records, with record ranges.
+struct RunLoad {
   using Payload = uint64_t;
   const uint64_t* keys;
-  uint32_t shift;  // 0: distinct keys; 3: distinct parents
   __device__ uint32_t operator()(uint32_t i, uint64_t& k) const {
     k = keys[i];
-    return i == 0 || (keys[i - 1] >> shift) != (k >> shift);
+    return i == 0 || keys[i - 1] != k;
   }
 };
 
-struct LevelStore {
+struct RunStore {
+  uint64_t* keys0;
+  uint32_t* first0;
+  uint32_t* n0;
+  const uint32_t* n_ptr;
+  __device__ void operator()(uint32_t i, const uint64_t& k, uint32_t f, uint32_t excl, uint32_t incl) const {
+    if (f) {
+      keys0[excl] = k;
+      first0[excl] = i;
+    }
+    if (i == *n_ptr - 1) {
+      *n0 = incl;
+      first0[incl] = i + 1;
+    }
+  }
+};
+
+// Height h -> h + 1: distinct parents; entries of a scene whose root sits at
+// height h are its root and get no parent.
+struct UpPayload {
+  uint64_t key;
+  uint32_t scene;
+  bool inc;
+};
+
+struct UpLoad {
+  using Payload = UpPayload;
+  const uint64_t* keys;
+  Scenes sc;
+  uint32_t h;
+  __device__ uint32_t operator()(uint32_t i, UpPayload& p) const {
+    p.key = keys[i];
+    p.scene = sc.codec.scene(p.key);
+    p.inc = sc.at(p.scene).depth > h;
+    if (!p.inc) return 0u;
+    return i == 0 || sc.codec.parent(keys[i - 1]) != sc.codec.parent(p.key);
+  }
+};
+
+struct UpStore {
+  Codec codec;
   uint64_t* up_keys;
   uint32_t* up_first;
-  uint32_t* parent;  // may be null
+  uint32_t* parent;
   uint32_t* up_n;
+  uint32_t* root_pos;  // per scene: index of its root entry at its root height
   const uint32_t* n_ptr;
-  uint32_t shift;
-  __device__ void operator()(uint32_t i, const uint64_t& k, uint32_t f, uint32_t excl, uint32_t incl) const {
-    if (parent) parent[i] = incl - 1;
+  __device__ void operator()(uint32_t i, const UpPayload& p, uint32_t f, uint32_t excl, uint32_t incl) const {
+    parent[i] = p.inc ? incl - 1 : kNone;
+    if (!p.inc) root_pos[p.scene] = i;
     if (f) {
-      up_keys[excl] = k >> shift;
+      up_keys[excl] = codec.parent(p.key);
       up_first[excl] = i;
     }
     if (i == *n_ptr - 1) {
@@ -303,21 +391,19 @@ struct LevelStore {
   }
 };
 
-__global__ void root_old_kernel(const uint32_t* ctl, const uint32_t* n0, uint32_t* old0) {
-  if (*n0 > 0) old0[0] = ctl[kCtlRoot];
-}
-
-__global__ void topdown_kernel(const uint32_t* __restrict__ node_pool, const uint32_t* n_ptr,
-                               const uint64_t* __restrict__ keys, const uint32_t* __restrict__ parent,
-                               const uint32_t* __restrict__ old_prev, uint32_t* __restrict__ old) {
+__global__ void topdown_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ parent,
+                               const uint32_t* __restrict__ old_up, uint32_t* __restrict__ old, uint32_t h) {
   const uint32_t n = *n_ptr;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-    old[j] = child_of(node_pool, old_prev[parent[j]], uint32_t(keys[j] & 7u));
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const Dev& d = sc.of(j);
+    old[j] = d.depth == h ? d.ctl[kCtlRoot]
+                          : child_of(d.node_pool, old_up[parent[j]], uint32_t(sc.keys[j] & 7u));
+  }
 }
 
 // ---- rebuild ---------------------------------------------------------------
 template <uint32_t LW>
-__global__ void leaf_build_kernel(Dev d, const uint32_t* n_ptr, const uint32_t* __restrict__ old,
+__global__ void leaf_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ old,
                                   const uint32_t* __restrict__ first, const uint32_t* __restrict__ sval,
                                   const uint64_t* __restrict__ rmask, const uint32_t* __restrict__ rmat,
                                   uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
@@ -333,8 +419,9 @@ __global__ void leaf_build_kernel(Dev d, const uint32_t* n_ptr, const uint32_t* 
 #pragma unroll
       for (uint32_t k = 0; k < LW; ++k) w[k] = 0xffffffffu;
     } else {
+      const uint32_t* lp = sc.of(j).leaf_pool + ref;
 #pragma unroll
-      for (uint32_t k = 0; k < LW; ++k) w[k] = d.leaf_pool[ref + k];
+      for (uint32_t k = 0; k < LW; ++k) w[k] = lp[k];
     }
     const uint32_t r1 = first[j + 1];
     for (uint32_t r = first[j]; r < r1; ++r) {
@@ -379,16 +466,20 @@ __global__ void leaf_build_kernel(Dev d, const uint32_t* n_ptr, const uint32_t* 
   }
 }
 
-__global__ void node_build_kernel(Dev d, const uint32_t* n_ptr, const uint32_t* __restrict__ old,
+// Node at height h: old item plus the changed children (entries of height
+// h - 1 whose parent is this entry).
+__global__ void node_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ old,
                                   const uint32_t* __restrict__ first, const uint64_t* __restrict__ ckeys,
-                                  const uint64_t* __restrict__ cres, uint32_t* __restrict__ items,
-                                  uint64_t* __restrict__ hashes, uint64_t* __restrict__ res) {
+                                  const uint32_t* __restrict__ cparent, const uint64_t* __restrict__ cres,
+                                  uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
+                                  uint64_t* __restrict__ res) {
   const uint32_t n = *n_ptr;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     uint32_t ch[8];
-    children_of(d.node_pool, old[j], ch);
+    children_of(sc.of(j).node_pool, old[j], ch);
     const uint32_t k1 = first[j + 1];
-    for (uint32_t k = first[j]; k < k1; ++k) ch[uint32_t(ckeys[k] & 7u)] = resolve(cres, k);
+    for (uint32_t k = first[j]; k < k1; ++k)
+      if (cparent[k] == j) ch[uint32_t(ckeys[k] & 7u)] = resolve(cres, k);
     uint32_t* it = items + size_t(j) * kNodeStride;
     uint32_t ref = 0;
     const uint32_t w = make_node_item(ch, it, &ref);
@@ -401,38 +492,74 @@ __global__ void node_build_kernel(Dev d, const uint32_t* n_ptr, const uint32_t* 
   }
 }
 
-__global__ void publish_kernel(uint32_t* ctl, const uint32_t* n0, const uint64_t* res0) {
-  if (*n0 == 0 || ctl[kCtlStatus] != kStatusOk) return;
-  ctl[kCtlRoot] = resolve(res0, 0);
+struct ResByHeight {
+  const uint64_t* p[kMaxHeights];
+};
+
+__global__ void publish_kernel(Scenes sc, uint32_t n_scenes, const uint32_t* __restrict__ root_pos, ResByHeight rh) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n_scenes; s += gridDim.x * blockDim.x) {
+    const uint32_t pos = root_pos[s];
+    const Dev& d = sc.at(s);
+    if (pos == kNone || d.ctl[kCtlStatus] != kStatusOk) continue;
+    d.ctl[kCtlRoot] = resolve(rh.p[d.depth], pos);
+  }
+}
+
+__global__ void begin_kernel(Scenes sc, uint32_t n_scenes) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n_scenes; s += gridDim.x * blockDim.x)
+    sc.at(s).ctl[kCtlStatus] = kStatusOk;
 }
 
 // ---- workspace ---------------------------------------------------------------
-struct Levels {
-  uint64_t* keys[16];
-  uint32_t* first[16];
-  uint32_t* parent[16];
-  uint32_t* old[16];
-  uint64_t* res[16];
-  uint64_t cap[16];
+struct Heights {
+  uint64_t* keys[kMaxHeights + 1];
+  uint32_t* first[kMaxHeights + 1];
+  uint32_t* parent[kMaxHeights + 1];
+  uint32_t* old[kMaxHeights + 1];
+  uint64_t* res[kMaxHeights + 1];
+  uint64_t cap[kMaxHeights + 1];
 };
 
 struct EditWs {
-  uint32_t* counts;  // [0] records, [1 + d] entries at level d
+  uint32_t* counts;  // [0] records, [1 + h] entries at height h
+  uint32_t* root_pos;
   uint64_t *rkey, *rkey_alt, *rmask;
   uint32_t *rval, *rval_alt, *rmat;
   void* sort_ws;
-  Levels lv;
+  Heights hs;
   uint32_t* items;
   uint64_t* hashes;
   void* fused_ws;
   BrushJob* jobs;
+  Dev* scenes;
   size_t bytes;
 };
 
-EditWs carve(void* base, uint32_t depth, uint32_t leaf_words, uint64_t r_ub, uint32_t n_jobs, uint64_t candidates) {
+struct Plan {
+  uint32_t n_scenes = 1;
+  uint32_t max_depth = 0;
+  uint32_t leaf_words = 2;
+  uint32_t key_bits = 0;
+  Codec codec{0};
+};
+
+Plan make_plan(const std::vector<const vx_dag*>& dags) {
+  Plan p;
+  p.n_scenes = uint32_t(dags.size());
+  for (const vx_dag* g : dags) p.max_depth = std::max(p.max_depth, g->depth);
+  p.leaf_words = dags[0]->dev.leaf_words;
+  p.codec.shift = 3 * p.max_depth;
+  uint32_t sbits = 0;
+  while ((1u << sbits) < p.n_scenes) ++sbits;
+  p.key_bits = p.codec.shift + sbits;
+  return p;
+}
+
+EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t candidates) {
   EditWs w{};
   Carver c(base);
   w.counts = c.take<uint32_t>(64);
+  w.root_pos = c.take<uint32_t>(p.n_scenes);
   const uint64_t R = std::max<uint64_t>(1, r_ub);
   w.rkey = c.take<uint64_t>(R);
   w.rkey_alt = c.take<uint64_t>(R);
@@ -440,127 +567,111 @@ EditWs carve(void* base, uint32_t depth, uint32_t leaf_words, uint64_t r_ub, uin
   w.rval = c.take<uint32_t>(R);
   w.rval_alt = c.take<uint32_t>(R);
   w.rmat = c.take<uint32_t>(R);
-  const uint32_t key_bits = 3 * depth;
-  const size_t sws = sort::pairs_u64_dev_workspace_bytes(R, key_bits);
-  w.sort_ws = c.take<char>(sws);
+  w.sort_ws = c.take<char>(sort::pairs_u64_dev_workspace_bytes(R, p.key_bits));
   uint64_t max_cap = 1, max_items = 1;
-  for (uint32_t d = 0; d <= depth; ++d) {
+  const uint32_t H = p.max_depth;
+  for (uint32_t h = 0; h <= H + 1; ++h) {
     uint64_t cap = R;
-    if (d < depth && 3 * d < 60) cap = std::min<uint64_t>(R, uint64_t(1) << (3 * d));
-    w.lv.cap[d] = cap;
-    w.lv.keys[d] = c.take<uint64_t>(cap);
-    w.lv.first[d] = c.take<uint32_t>(cap + 1);
-    w.lv.parent[d] = c.take<uint32_t>(cap);
-    w.lv.old[d] = c.take<uint32_t>(cap);
-    w.lv.res[d] = c.take<uint64_t>(cap);
+    if (h > 0) {
+      const uint32_t lv = h <= H ? H - h : 0;
+      if (3 * lv < 50) cap = std::min<uint64_t>(R, uint64_t(p.n_scenes) << (3 * lv));
+    }
+    w.hs.cap[h] = cap;
+    w.hs.keys[h] = c.take<uint64_t>(cap);
+    w.hs.first[h] = c.take<uint32_t>(cap + 1);
+    w.hs.parent[h] = c.take<uint32_t>(cap);
+    w.hs.old[h] = c.take<uint32_t>(cap);
+    w.hs.res[h] = c.take<uint64_t>(cap);
     max_cap = std::max(max_cap, cap);
-    max_items = std::max<uint64_t>(max_items, cap * (d == depth ? leaf_words : kNodeStride));
+    max_items = std::max<uint64_t>(max_items, cap * (h == 0 ? p.leaf_words : kNodeStride));
   }
   w.items = c.take<uint32_t>(max_items);
   w.hashes = c.take<uint64_t>(max_cap);
   w.fused_ws = c.take<char>(std::max(scan::fused_dev_workspace_bytes(std::max(max_cap, R)),
                                      scan::fused_workspace_bytes(std::max<uint64_t>(candidates, 1))));
   w.jobs = c.take<BrushJob>(std::max<uint32_t>(1, n_jobs));
+  w.scenes = c.take<Dev>(p.n_scenes);
   w.bytes = c.size();
   return w;
 }
 
 inline void mark(vx_dag* g, uint32_t i, hipStream_t s) {
-  if (g->profile) hipEventRecord(g->ev[i], s);
+  if (g && g->profile) hipEventRecord(g->ev[i], s);
 }
 
-// Everything after record generation: sort, levels, old refs, rebuild, publish.
-vx_status rebuild(vx_dag* g, EditWs& w, uint64_t r_ub, hipStream_t s) {
-  const Dev& d = g->dev;
-  const uint32_t D = g->depth;
+// Everything after record generation: sort, heights, old refs, rebuild,
+// publish. `g` (may be null) receives the stage timings.
+vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_t r_ub, hipStream_t s) {
+  const uint32_t H = p.max_depth;
   uint32_t* cnt = w.counts;
+  Heights& L = w.hs;
   vx_status st;
   bool in_alt = false;
   mark(g, 1, s);
-  if ((st = sort::pairs_u64_dev(w.rkey, w.rval, w.rkey_alt, w.rval_alt, &cnt[0], std::max<uint64_t>(r_ub, 1), 3 * D,
-                                w.sort_ws, &in_alt, s)) != VX_OK)
+  if ((st = sort::pairs_u64_dev(w.rkey, w.rval, w.rkey_alt, w.rval_alt, &cnt[0], std::max<uint64_t>(r_ub, 1),
+                                std::max(p.key_bits, 1u), w.sort_ws, &in_alt, s)) != VX_OK)
     return st;
   mark(g, 2, s);
   const uint64_t* skey = in_alt ? w.rkey_alt : w.rkey;
   const uint32_t* sval = in_alt ? w.rval_alt : w.rval;
-  Levels& L = w.lv;
-  // Distinct leaves (level D) and their record ranges.
-  if ((st = scan::fused_dev<scan::kSum>(&cnt[0], r_ub, KeyLoad{skey, 0},
-                                        LevelStore{L.keys[D], L.first[D], nullptr, &cnt[1 + D], &cnt[0], 0},
+  if ((st = scan::fused_dev<scan::kSum>(&cnt[0], r_ub, RunLoad{skey}, RunStore{L.keys[0], L.first[0], &cnt[1], &cnt[0]},
                                         w.fused_ws, s)) != VX_OK)
     return st;
-  // Distinct ancestors, bottom-up.
-  for (uint32_t dd = D; dd-- > 0;) {
-    if ((st = scan::fused_dev<scan::kSum>(&cnt[1 + dd + 1], L.cap[dd + 1], KeyLoad{L.keys[dd + 1], 3},
-                                          LevelStore{L.keys[dd], L.first[dd], L.parent[dd + 1], &cnt[1 + dd],
-                                                     &cnt[1 + dd + 1], 3},
+  for (uint32_t h = 0; h <= H; ++h) {
+    Scenes sh = sc;
+    sh.keys = L.keys[h];
+    if ((st = scan::fused_dev<scan::kSum>(&cnt[1 + h], L.cap[h], UpLoad{L.keys[h], sh, h},
+                                          UpStore{p.codec, L.keys[h + 1], L.first[h + 1], L.parent[h], &cnt[2 + h],
+                                                  w.root_pos, &cnt[1 + h]},
                                           w.fused_ws, s)) != VX_OK)
       return st;
   }
   mark(g, 3, s);
-  // Old refs, top-down.
-  root_old_kernel<<<1, 1, 0, s>>>(d.ctl, &cnt[1], L.old[0]);
-  if ((st = last_launch()) != VX_OK) return st;
-  for (uint32_t dd = 1; dd <= D; ++dd) {
-    topdown_kernel<<<grid_for(L.cap[dd]), 256, 0, s>>>(d.node_pool, &cnt[1 + dd], L.keys[dd], L.parent[dd],
-                                                         L.old[dd - 1], L.old[dd]);
+  for (uint32_t h = H + 1; h-- > 0;) {
+    Scenes sh = sc;
+    sh.keys = L.keys[h];
+    topdown_kernel<<<grid_for(L.cap[h]), 256, 0, s>>>(sh, &cnt[1 + h], L.parent[h], L.old[h + 1], L.old[h], h);
     if ((st = last_launch()) != VX_OK) return st;
   }
   mark(g, 4, s);
-  // Leaves.
-  if (d.leaf_words == 2)
-    leaf_build_kernel<2><<<grid_for(L.cap[D]), 256, 0, s>>>(d, &cnt[1 + D], L.old[D], L.first[D], sval, w.rmask,
-                                                              w.rmat, w.items, w.hashes, L.res[D]);
-  else
-    leaf_build_kernel<16><<<grid_for(L.cap[D]), 256, 0, s>>>(d, &cnt[1 + D], L.old[D], L.first[D], sval, w.rmask,
-                                                               w.rmat, w.items, w.hashes, L.res[D]);
-  if ((st = last_launch()) != VX_OK) return st;
-  if ((st = intern(d, true, ItemSet{w.items, d.leaf_words, d.leaf_words}, w.hashes, &cnt[1 + D], L.cap[D], L.res[D],
-                   w.fused_ws, s)) != VX_OK)
-    return st;
-  mark(g, 5, s);
-  // Nodes, bottom-up.
-  for (uint32_t dd = D; dd-- > 0;) {
-    node_build_kernel<<<grid_for(L.cap[dd]), 256, 0, s>>>(d, &cnt[1 + dd], L.old[dd], L.first[dd], L.keys[dd + 1],
-                                                           L.res[dd + 1], w.items, w.hashes, L.res[dd]);
+  {
+    Scenes sh = sc;
+    sh.keys = L.keys[0];
+    if (p.leaf_words == 2)
+      leaf_build_kernel<2><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.old[0], L.first[0], sval, w.rmask, w.rmat,
+                                                              w.items, w.hashes, L.res[0]);
+    else
+      leaf_build_kernel<16><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.old[0], L.first[0], sval, w.rmask,
+                                                               w.rmat, w.items, w.hashes, L.res[0]);
     if ((st = last_launch()) != VX_OK) return st;
-    if ((st = intern(d, false, ItemSet{w.items, kNodeStride, 0}, w.hashes, &cnt[1 + dd], L.cap[dd], L.res[dd],
-                     w.fused_ws, s)) != VX_OK)
+    if ((st = intern_scenes(sh, p.n_scenes, true, ItemSet{w.items, p.leaf_words, p.leaf_words}, w.hashes, &cnt[1],
+                            L.cap[0], L.res[0], w.fused_ws, s)) != VX_OK)
+      return st;
+  }
+  mark(g, 5, s);
+  for (uint32_t h = 1; h <= H; ++h) {
+    Scenes sh = sc;
+    sh.keys = L.keys[h];
+    node_build_kernel<<<grid_for(L.cap[h]), 256, 0, s>>>(sh, &cnt[1 + h], L.old[h], L.first[h], L.keys[h - 1],
+                                                         L.parent[h - 1], L.res[h - 1], w.items, w.hashes, L.res[h]);
+    if ((st = last_launch()) != VX_OK) return st;
+    if ((st = intern_scenes(sh, p.n_scenes, false, ItemSet{w.items, kNodeStride, 0}, w.hashes, &cnt[1 + h], L.cap[h],
+                            L.res[h], w.fused_ws, s)) != VX_OK)
       return st;
   }
   mark(g, 6, s);
-  publish_kernel<<<1, 1, 0, s>>>(d.ctl, &cnt[1], L.res[0]);
+  ResByHeight rh{};
+  for (uint32_t h = 0; h <= H && h < kMaxHeights; ++h) rh.p[h] = L.res[h];
+  publish_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes, w.root_pos, rh);
   st = last_launch();
   mark(g, 7, s);
   return st;
 }
 
-vx_status finish(vx_dag* g, const EditWs& w, hipStream_t s) {
-  uint32_t status = 0;
-  uint32_t counts[64];
-  if (hipMemcpyAsync(&status, g->ctl + kCtlStatus, 4, hipMemcpyDeviceToHost, s) != hipSuccess) return VX_DEVICE_ERROR;
-  if (g->profile && hipMemcpyAsync(counts, w.counts, sizeof(counts), hipMemcpyDeviceToHost, s) != hipSuccess)
-    return VX_DEVICE_ERROR;
-  if (hipStreamSynchronize(s) != hipSuccess) return VX_DEVICE_ERROR;
-  if (g->profile) {
-    for (uint32_t i = 0; i < VX_DAG_STAGES; ++i) {
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, g->ev[i], g->ev[i + 1]) == hipSuccess) g->prof.ms[i] += ms;
-    }
-    g->prof.edits += 1;
-    g->prof.records += counts[0];
-    g->prof.leaves += counts[1 + g->depth];
-    for (uint32_t d = 0; d < g->depth; ++d) g->prof.nodes += counts[1 + d];
-    g->prof.sort_passes = (3 * g->depth + 7) / 8;
-  }
-  return status == kStatusOk ? VX_OK : VX_OUT_OF_CAPACITY;
-}
-
-__global__ void clear_status_kernel(uint32_t* ctl) { ctl[kCtlStatus] = kStatusOk; }
-
 // Brush jobs: brick bounding boxes clipped to the world; empty ones dropped.
-uint64_t make_jobs(const vx_brush* b, uint32_t n, uint32_t world, std::vector<BrushJob>* jobs) {
-  uint64_t total = 0;
+uint64_t make_jobs(const vx_brush* b, uint32_t n, uint32_t world, uint32_t scene, uint64_t start,
+                   std::vector<BrushJob>* jobs) {
+  uint64_t total = start;
   for (uint32_t i = 0; i < n; ++i) {
     const vx_brush& br = b[i];
     const int64_t c[3] = {br.cx, br.cy, br.cz};
@@ -584,10 +695,63 @@ uint64_t make_jobs(const vx_brush* b, uint32_t n, uint32_t world, std::vector<Br
     j.ny = uint32_t(cnt[1]);
     j.nz = uint32_t(cnt[2]);
     j.start = uint32_t(total);
+    j.scene = scene;
     total += uint64_t(cnt[0]) * cnt[1] * cnt[2];
     if (jobs) jobs->push_back(j);
   }
-  return total;
+  return total - start;
+}
+
+vx_status finish(vx_dag* g, const std::vector<vx_dag*>& dags, const EditWs& w, const Plan& p, hipStream_t s) {
+  std::vector<uint32_t> status(dags.size(), 0);
+  uint32_t counts[64];
+  for (size_t i = 0; i < dags.size(); ++i)
+    if (hipMemcpyAsync(&status[i], dags[i]->ctl + kCtlStatus, 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+      return VX_DEVICE_ERROR;
+  const bool prof = g && g->profile;
+  if (prof && hipMemcpyAsync(counts, w.counts, sizeof(counts), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return VX_DEVICE_ERROR;
+  if (hipStreamSynchronize(s) != hipSuccess) return VX_DEVICE_ERROR;
+  if (prof) {
+    for (uint32_t i = 0; i < VX_DAG_STAGES; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, g->ev[i], g->ev[i + 1]) == hipSuccess) g->prof.ms[i] += ms;
+    }
+    g->prof.edits += 1;
+    g->prof.records += counts[0];
+    g->prof.leaves += counts[1];
+    for (uint32_t h = 1; h <= p.max_depth; ++h) g->prof.nodes += counts[1 + h];
+    g->prof.sort_passes = (p.key_bits + 7) / 8;
+  }
+  for (uint32_t v : status)
+    if (v != kStatusOk) return VX_OUT_OF_CAPACITY;
+  return VX_OK;
+}
+
+// Shared driver: records from `load` (host count n_cand), then rebuild.
+template <class Load>
+vx_status run_edit(const std::vector<vx_dag*>& dags, vx_dag* prof_dag, const Plan& p, EditWs& w,
+                   const std::vector<BrushJob>* jobs, uint32_t n_cand, Load load, hipStream_t s) {
+  vx_status st;
+  std::vector<Dev> devs;
+  for (vx_dag* g : dags) devs.push_back(g->dev);
+  if (hipMemcpyAsync(w.scenes, devs.data(), devs.size() * sizeof(Dev), hipMemcpyHostToDevice, s) != hipSuccess)
+    return VX_DEVICE_ERROR;
+  if (jobs && !jobs->empty() &&
+      hipMemcpyAsync(w.jobs, jobs->data(), jobs->size() * sizeof(BrushJob), hipMemcpyHostToDevice, s) != hipSuccess)
+    return VX_DEVICE_ERROR;
+  if ((st = fill_u32(w.counts, 64, 0, s)) != VX_OK) return st;
+  if ((st = fill_u32(w.root_pos, p.n_scenes, kNone, s)) != VX_OK) return st;
+  const Scenes sc{p.n_scenes > 1 ? w.scenes : nullptr, nullptr, p.codec, devs[0]};
+  begin_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes);
+  if ((st = last_launch()) != VX_OK) return st;
+  mark(prof_dag, 0, s);
+  if ((st = scan::fused<scan::kSum>(n_cand, load,
+                                    RecordStore{RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0]}, n_cand},
+                                    w.fused_ws, s)) != VX_OK)
+    return st;
+  if ((st = rebuild(prof_dag, p, sc, w, n_cand, s)) != VX_OK) return st;
+  return finish(prof_dag, dags, w, p, s);
 }
 
 // ---- queries -------------------------------------------------------------------
@@ -696,6 +860,8 @@ extern "C" vx_status vx_dag_destroy(vx_dag* g) {
   if (g->node_slots) hipFree(g->node_slots);
   if (g->leaf_slots) hipFree(g->leaf_slots);
   if (g->ctl) hipFree(g->ctl);
+  for (auto& e : g->ev)
+    if (e) hipEventDestroy(e);
   delete g;
   return VX_OK;
 }
@@ -718,7 +884,7 @@ extern "C" vx_status vx_dag_stats_get(const vx_dag* g, vx_dag_stats* out) {
 
 extern "C" vx_status vx_dag_voxel_edits_workspace_size(const vx_dag* g, uint32_t count, size_t* bytes) {
   if (!g || !bytes) return VX_INVALID_VALUE;
-  *bytes = count ? carve(nullptr, g->depth, g->dev.leaf_words, count, 0, count).bytes : 0;
+  *bytes = count ? carve(nullptr, make_plan({g}), count, 0, count).bytes : 0;
   return VX_OK;
 }
 
@@ -726,29 +892,21 @@ extern "C" vx_status vx_dag_apply_voxel_edits(vx_dag* g, const vx_voxel* edits, 
                                               uint32_t material, void* ws, size_t ws_bytes, void* stream) {
   if (!g || (!edits && count) || op > 1) return VX_INVALID_VALUE;  // edit.cuh:28-37
   if (count == 0) return VX_OK;
-  EditWs w = carve(ws, g->depth, g->dev.leaf_words, count, 0, count);
-  if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
   if (count >= (1u << 30)) return VX_INVALID_VALUE;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  vx_status st;
-  if ((st = fill_u32(w.counts, 64, 0, s)) != VX_OK) return st;
-  clear_status_kernel<<<1, 1, 0, s>>>(g->ctl);
-  mark(g, 0, s);
+  const Plan p = make_plan({g});
+  EditWs w = carve(ws, p, count, 0, count);
+  if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
   const uint32_t mat = op == 1 ? 0u : (g->materials ? (material & 0xffu) : 1u);
   if (op == 0 && g->materials && mat == 0) return VX_INVALID_VALUE;
-  if ((st = scan::fused<scan::kSum>(count, VoxelLoad{edits, 4u << g->depth, mat},
-                                    RecordStore{RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0]}, count},
-                                    w.fused_ws, s)) != VX_OK)
-    return st;
-  if ((st = rebuild(g, w, count, s)) != VX_OK) return st;
-  return finish(g, w, s);
+  return run_edit({g}, g, p, w, nullptr, count, VoxelLoad{edits, 4u << g->depth, mat},
+                  static_cast<hipStream_t>(stream));
 }
 
 extern "C" vx_status vx_dag_brush_workspace_size(const vx_dag* g, const vx_brush* b, uint32_t n, size_t* bytes) {
   if (!g || !bytes || (!b && n)) return VX_INVALID_VALUE;
   std::vector<BrushJob> jobs;
-  const uint64_t cand = make_jobs(b, n, 4u << g->depth, &jobs);
-  *bytes = cand ? carve(nullptr, g->depth, g->dev.leaf_words, cand, uint32_t(jobs.size()), cand).bytes : 0;
+  const uint64_t cand = make_jobs(b, n, 4u << g->depth, 0, 0, &jobs);
+  *bytes = cand ? carve(nullptr, make_plan({g}), cand, uint32_t(jobs.size()), cand).bytes : 0;
   return VX_OK;
 }
 
@@ -756,24 +914,68 @@ extern "C" vx_status vx_dag_apply_brushes(vx_dag* g, const vx_brush* b, uint32_t
                                           void* stream) {
   if (!g || (!b && n)) return VX_INVALID_VALUE;
   std::vector<BrushJob> jobs;
-  const uint64_t cand = make_jobs(b, n, 4u << g->depth, &jobs);
+  const uint64_t cand = make_jobs(b, n, 4u << g->depth, 0, 0, &jobs);
   if (cand == 0) return VX_OK;
   if (cand >= (1u << 30)) return VX_INVALID_VALUE;
-  EditWs w = carve(ws, g->depth, g->dev.leaf_words, cand, uint32_t(jobs.size()), cand);
+  const Plan p = make_plan({g});
+  EditWs w = carve(ws, p, cand, uint32_t(jobs.size()), cand);
   if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  vx_status st;
-  if (hipMemcpyAsync(w.jobs, jobs.data(), jobs.size() * sizeof(BrushJob), hipMemcpyHostToDevice, s) != hipSuccess)
-    return VX_DEVICE_ERROR;
-  if ((st = fill_u32(w.counts, 64, 0, s)) != VX_OK) return st;
-  clear_status_kernel<<<1, 1, 0, s>>>(g->ctl);
-  mark(g, 0, s);
-  if ((st = scan::fused<scan::kSum>(uint32_t(cand), BrushLoad{w.jobs, uint32_t(jobs.size()), g->materials},
-                                    RecordStore{RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0]}, uint32_t(cand)},
-                                    w.fused_ws, s)) != VX_OK)
-    return st;
-  if ((st = rebuild(g, w, cand, s)) != VX_OK) return st;
-  return finish(g, w, s);
+  return run_edit({g}, g, p, w, &jobs, uint32_t(cand), BrushLoad{w.jobs, uint32_t(jobs.size()), g->materials, p.codec},
+                  static_cast<hipStream_t>(stream));
+}
+
+// Batched: batches[i] applies to dags[i]; batches of one scene apply in array
+// order, scenes are independent; all of it runs as one pipeline.
+static vx_status batch_setup(vx_dag* const* dags, const vx_brush* const* brushes, const uint32_t* counts,
+                             uint32_t n_batches, std::vector<vx_dag*>* scenes, std::vector<BrushJob>* jobs,
+                             uint64_t* cand) {
+  if (!dags || !brushes || !counts || n_batches == 0) return VX_INVALID_VALUE;
+  std::unordered_map<const vx_dag*, uint32_t> index;
+  for (uint32_t i = 0; i < n_batches; ++i) {
+    if (!dags[i] || (!brushes[i] && counts[i])) return VX_INVALID_VALUE;
+    if (index.emplace(dags[i], uint32_t(scenes->size())).second) scenes->push_back(dags[i]);
+  }
+  const vx_dag* first = (*scenes)[0];
+  for (vx_dag* g : *scenes)
+    if (g->materials != first->materials) return VX_INVALID_VALUE;
+  *cand = 0;
+  // Jobs grouped by scene (scene order = first appearance), batch order kept.
+  for (uint32_t s = 0; s < scenes->size(); ++s)
+    for (uint32_t i = 0; i < n_batches; ++i)
+      if (index[dags[i]] == s)
+        *cand += make_jobs(brushes[i], counts[i], 4u << dags[i]->depth, s, *cand, jobs);
+  return VX_OK;
+}
+
+extern "C" vx_status vx_dag_batch_workspace_size(vx_dag* const* dags, const vx_brush* const* brushes,
+                                                 const uint32_t* counts, uint32_t n_batches, size_t* bytes) {
+  if (!bytes) return VX_INVALID_VALUE;
+  std::vector<vx_dag*> scenes;
+  std::vector<BrushJob> jobs;
+  uint64_t cand = 0;
+  const vx_status st = batch_setup(dags, brushes, counts, n_batches, &scenes, &jobs, &cand);
+  if (st != VX_OK) return st;
+  const Plan p = make_plan(std::vector<const vx_dag*>(scenes.begin(), scenes.end()));
+  *bytes = cand ? carve(nullptr, p, cand, uint32_t(jobs.size()), cand).bytes : 0;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_dag_apply_batches(vx_dag* const* dags, const vx_brush* const* brushes, const uint32_t* counts,
+                                          uint32_t n_batches, void* ws, size_t ws_bytes, void* stream) {
+  std::vector<vx_dag*> scenes;
+  std::vector<BrushJob> jobs;
+  uint64_t cand = 0;
+  vx_status st = batch_setup(dags, brushes, counts, n_batches, &scenes, &jobs, &cand);
+  if (st != VX_OK) return st;
+  if (cand == 0) return VX_OK;
+  if (cand >= (1u << 30) || scenes.size() > (1u << 20)) return VX_INVALID_VALUE;
+  const Plan p = make_plan(std::vector<const vx_dag*>(scenes.begin(), scenes.end()));
+  if (p.key_bits > 64) return VX_INVALID_VALUE;
+  EditWs w = carve(ws, p, cand, uint32_t(jobs.size()), cand);
+  if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
+  return run_edit(scenes, scenes[0], p, w, &jobs, uint32_t(cand),
+                  BrushLoad{w.jobs, uint32_t(jobs.size()), scenes[0]->materials, p.codec},
+                  static_cast<hipStream_t>(stream));
 }
 
 extern "C" vx_status vx_dag_get_voxels(const vx_dag* g, const vx_voxel* q, uint32_t n, uint32_t* out, void* stream) {

@@ -55,6 +55,8 @@ register({
     "vx_dag_export_canonical": (_i32, [_P, _P, _u64, C.POINTER(_u64)]),
     "vx_brush_voxel_ops": (_i32, [_P, _u32, _u32, C.POINTER(_u64)]),
     "vx_gen_brushes": (_i32, [_u64, _u64, _u32, _u32, _u32, _P]),
+    "vx_dag_batch_workspace_size": (_i32, [_P, _P, _P, _u32, C.POINTER(_sz)]),
+    "vx_dag_apply_batches": (_i32, [_P, _P, _P, _u32, _P, _sz, _P]),
     "vx_dag_profile_enable": (_i32, [_P, _i32]),
     "vx_dag_profile_read": (_i32, [_P, _P]),
 })
@@ -184,6 +186,39 @@ class HashDagGpu:
         out = np.empty(n.value, np.uint32)
         check(lib().vx_dag_export_canonical(self.h, out.ctypes.data_as(_P), n.value, C.byref(n)), "export")
         return out
+
+
+class _BatchArgs:
+    """Keeps the ctypes arrays of one batched call alive."""
+
+    def __init__(self, dags, batches):
+        if len(dags) != len(batches) or not dags:
+            raise ValueError("one brush array per batch")
+        self.arrays = [np.ascontiguousarray(b, BRUSH_DTYPE) for b in batches]
+        n = len(dags)
+        self.dags = (_P * n)(*[d.h.value for d in dags])
+        self.ptrs = (_P * n)(*[a.ctypes.data for a in self.arrays])
+        self.counts = (_u32 * n)(*[len(a) for a in self.arrays])
+        self.n = n
+
+
+def batch_workspace_size(dags, batches) -> int:
+    a = _BatchArgs(dags, batches)
+    n = C.c_size_t()
+    check(lib().vx_dag_batch_workspace_size(a.dags, a.ptrs, a.counts, a.n, C.byref(n)), "batch workspace")
+    return n.value
+
+
+def apply_batches(dags, batches, workspace=None, stream=None):
+    """Apply batches[i] to dags[i], all in one pipeline (same result as in order)."""
+    a = _BatchArgs(dags, batches)
+    need = C.c_size_t()
+    check(lib().vx_dag_batch_workspace_size(a.dags, a.ptrs, a.counts, a.n, C.byref(need)), "batch workspace")
+    if workspace is None and need.value:
+        workspace = torch.empty(need.value, dtype=torch.uint8, device="cuda")
+    wsb = 0 if workspace is None else workspace.numel()
+    check(lib().vx_dag_apply_batches(a.dags, a.ptrs, a.counts, a.n, _ptr(workspace), wsb, _stream(stream)),
+          "vx_dag_apply_batches")
 
 
 def smoke_check(dev) -> None:
