@@ -264,6 +264,14 @@ vx_status vx_dag_export_canonical(const vx_dag* dag, uint32_t* out, uint64_t cap
 vx_status vx_gen_brushes(uint64_t seed, uint64_t stream_id, uint32_t n, uint32_t world, uint32_t mix,
                          vx_brush* out);
 
+/* Brush batch sized to a voxel-op target (batched and serving workloads):
+ * target = 2^t, t uniform in [lo_log2, hi_log2]; n uniform in [1, max_brushes]
+ * brushes of volume target/n (50% spheres, 50% cubes), 60% add / 40% remove;
+ * *scene_out = uniform scene in [0, n_scenes). `out` holds max_brushes. */
+vx_status vx_gen_sized_batch(uint64_t seed, uint64_t stream_id, uint64_t index, uint32_t lo_log2, uint32_t hi_log2,
+                             uint32_t max_brushes, uint32_t world, uint32_t n_scenes, vx_brush* out, uint32_t* n_out,
+                             uint32_t* scene_out);
+
 /* Exact number of voxel operations of a brush batch: sum over brushes of the
  * voxels each covers inside the world (overlaps count once per brush). */
 vx_status vx_brush_voxel_ops(const vx_brush* h_brushes, uint32_t n, uint32_t world, uint64_t* ops);

@@ -1,0 +1,65 @@
+"""Continuous-batching server: final DAGs equal the CPU oracle replaying the
+requests in arrival order; a scene's result is the same when its requests
+are served alone and when they are mixed with other scenes' requests."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes(depths, materials=True):
+    from voxel_amd.dag import HashDagGpu, terrain_for_world
+    out = []
+    for k, d in enumerate(depths):
+        g = HashDagGpu(d, materials, node_words=1 << 22, leaf_words=1 << 23)
+        g.build_terrain(terrain_for_world(4 << d, 20 + k))
+        out.append(g)
+    return out
+
+
+def test_served_scenes_equal_oracle_replay(gpu):
+    from voxel_amd.serving import Server, make_requests, poisson_arrivals, serve
+    depths = [5, 6, 4, 6]
+    scenes = _scenes(depths)
+    reqs = make_requests(48, len(depths), [4 << d for d in depths], seed=20, lo_log2=6, hi_log2=12, max_brushes=8)
+    res = serve(Server(scenes), reqs, poisson_arrivals(len(reqs), 4000.0, 1))
+    assert res["completed"] == len(reqs) and res["rejected"] == 0
+    assert res["ticks"] < len(reqs)  # requests were batched
+    for s, d in enumerate(depths):
+        ref = O.Dag(d, True)
+        ref.build_terrain(O.terrain_for_world(4 << d, 20 + s))
+        for r in reqs:  # arrival order
+            if r.scene == s:
+                ref.apply_brushes(r.brushes)
+        assert np.array_equal(scenes[s].export(), ref.export()), s
+
+
+def test_result_same_alone_and_in_mix(gpu):
+    from voxel_amd.serving import Server, make_requests, poisson_arrivals, serve
+    depths = [5, 5, 5]
+    reqs = make_requests(36, len(depths), [4 << d for d in depths], seed=21, lo_log2=6, hi_log2=11, max_brushes=6)
+    mixed = _scenes(depths)
+    serve(Server(mixed), reqs, poisson_arrivals(len(reqs), 5000.0, 2))
+    for s in range(len(depths)):
+        alone = _scenes([depths[s]])
+        mine = [r for r in reqs if r.scene == s]
+        for r in mine:
+            r.scene = 0
+        serve(Server(alone), mine, poisson_arrivals(len(mine), 5000.0, 3))
+        for r in mine:
+            r.scene = s
+        assert np.array_equal(alone[0].export(), mixed[s].export()), s
+
+
+def test_admission_defers_and_rejects_by_free_pool_space(gpu):
+    from voxel_amd.dag import HashDagGpu
+    from voxel_amd.serving import Request, Server
+    d = HashDagGpu(5, True, node_words=4096, leaf_words=4096, node_slots=1024, leaf_slots=1024)
+    big = O.gen_brushes(1, O.STREAM_BRUSH, 4, 128, 0)
+    small = np.zeros(1, big.dtype)
+    small[0] = (3, 3, 3, 1, 0, 0, 0, 9)
+    srv = Server([d])
+    admitted = srv.admit([Request(0, 0, big), Request(1, 0, small)])
+    assert [r.rid for r in admitted] == [1]

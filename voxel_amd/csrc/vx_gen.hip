@@ -1,6 +1,8 @@
 // Device-side generators of the synthetic workloads. Each element is a pure
 // function of (seed, stream, index) through a counter-based splitmix64 stream,
 // so the host oracle can regenerate any element independently.
+#include <cmath>
+
 #include "vx_common.hpp"
 
 namespace vx {
@@ -133,5 +135,45 @@ extern "C" vx_status vx_gen_brushes(uint64_t seed, uint64_t stream_id, uint32_t 
     b.material = add ? vx::uniform_u32(r[8], 1, 255) : 0u;
     out[i] = b;
   }
+  return VX_OK;
+}
+
+// Brush batch sized to a voxel-op target (the batched and serving workloads).
+// Batch `index` of (seed, stream_id) draws t = uniform [lo_log2, hi_log2] in
+// 1/1024 steps, target = 2^t voxel ops, n = uniform [1, max_brushes] brushes of
+// volume target / n each: 50% spheres of radius round(cbrt(3V / 4 pi)), 50%
+// cubes of half extent round((cbrt(V) - 1) / 2) (both at least 1), centres
+// uniform in the world, 60% add with material uniform [1,255], 40% remove.
+// Brush i of the batch uses draws 64 + 8 i .. 64 + 8 i + 5; *scene_out gets
+// uniform [0, n_scenes - 1] from draw 2.
+extern "C" vx_status vx_gen_sized_batch(uint64_t seed, uint64_t stream_id, uint64_t index, uint32_t lo_log2,
+                                        uint32_t hi_log2, uint32_t max_brushes, uint32_t world, uint32_t n_scenes,
+                                        vx_brush* out, uint32_t* n_out, uint32_t* scene_out) {
+  if (!out || !n_out || !max_brushes || world == 0 || lo_log2 > hi_log2 || hi_log2 > 40) return VX_INVALID_VALUE;
+  const uint64_t base = vx::stream_base(seed ^ (index * 0x9E3779B97F4A7C15ull), stream_id);
+  const uint32_t tq = vx::uniform_u32(vx::stream_at(base, 0), lo_log2 * 1024, hi_log2 * 1024);
+  const double target = std::exp2(double(tq) / 1024.0);
+  const uint32_t n = vx::uniform_u32(vx::stream_at(base, 1), 1, max_brushes);
+  if (scene_out) *scene_out = n_scenes ? vx::uniform_u32(vx::stream_at(base, 2), 0, n_scenes - 1) : 0;
+  const double vol = target / n;
+  const int32_t r = std::max<int32_t>(1, int32_t(std::lround(std::cbrt(3.0 * vol / (4.0 * 3.141592653589793)))));
+  const int32_t h = std::max<int32_t>(1, int32_t(std::lround((std::cbrt(vol) - 1.0) / 2.0)));
+  for (uint32_t i = 0; i < n; ++i) {
+    uint64_t d[6];
+    for (uint32_t k = 0; k < 6; ++k) d[k] = vx::stream_at(base, 64 + uint64_t(i) * 8 + k);
+    vx_brush b{};
+    b.kind = vx::uniform_u32(d[0], 0, 1);
+    b.cx = int32_t(vx::uniform_u32(d[1], 0, world - 1));
+    b.cy = int32_t(vx::uniform_u32(d[2], 0, world - 1));
+    b.cz = int32_t(vx::uniform_u32(d[3], 0, world - 1));
+    if (b.kind == 0) {
+      b.hx = r;
+    } else {
+      b.hx = b.hy = b.hz = h;
+    }
+    b.material = vx::uniform_u32(d[4], 0, 99) < 60 ? vx::uniform_u32(d[5], 1, 255) : 0u;
+    out[i] = b;
+  }
+  *n_out = n;
   return VX_OK;
 }

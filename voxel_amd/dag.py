@@ -55,6 +55,8 @@ register({
     "vx_dag_export_canonical": (_i32, [_P, _P, _u64, C.POINTER(_u64)]),
     "vx_brush_voxel_ops": (_i32, [_P, _u32, _u32, C.POINTER(_u64)]),
     "vx_gen_brushes": (_i32, [_u64, _u64, _u32, _u32, _u32, _P]),
+    "vx_gen_sized_batch": (_i32, [_u64, _u64, _u64, _u32, _u32, _u32, _u32, _u32, _P, C.POINTER(_u32),
+                                  C.POINTER(_u32)]),
     "vx_dag_batch_workspace_size": (_i32, [_P, _P, _P, _u32, C.POINTER(_sz)]),
     "vx_dag_apply_batches": (_i32, [_P, _P, _P, _u32, _P, _sz, _P]),
     "vx_dag_profile_enable": (_i32, [_P, _i32]),
@@ -79,6 +81,19 @@ def gen_brushes(seed: int, stream_id: int, n: int, world: int, mix: int) -> np.n
     return out
 
 
+STREAM_BATCH = 6
+
+
+def gen_sized_batch(seed: int, stream_id: int, index: int, lo_log2: int, hi_log2: int, max_brushes: int,
+                    world: int, n_scenes: int = 1):
+    """(brushes, scene) of a batch sized to a log-uniform voxel-op target."""
+    out = np.zeros(max_brushes, BRUSH_DTYPE)
+    n, s = C.c_uint32(), C.c_uint32()
+    check(lib().vx_gen_sized_batch(seed, stream_id, index, lo_log2, hi_log2, max_brushes, world, n_scenes,
+                                   out.ctypes.data_as(_P), C.byref(n), C.byref(s)), "vx_gen_sized_batch")
+    return out[: n.value].copy(), s.value
+
+
 def brush_voxel_ops(brushes: np.ndarray, world: int) -> int:
     """Exact voxel operations of a brush batch (per-brush voxels inside the world)."""
     b = np.ascontiguousarray(brushes, BRUSH_DTYPE)
@@ -100,6 +115,7 @@ class HashDagGpu:
         self.h = C.c_void_p()
         cap = Capacity(node_words, leaf_words, node_slots, leaf_slots)
         check(lib().vx_dag_create(inner_depth, int(materials), C.byref(cap), C.byref(self.h)), "vx_dag_create")
+        self._cap = cap
         self.inner_depth = inner_depth
         self.world = 4 << inner_depth
         self.materials = materials
