@@ -246,6 +246,22 @@ vx_status vx_dag_batch_workspace_size(vx_dag* const* dags, const vx_brush* const
 vx_status vx_dag_apply_batches(vx_dag* const* dags, const vx_brush* const* h_brushes, const uint32_t* counts,
                                uint32_t n_batches, void* ws, size_t ws_bytes, void* stream);
 
+/* Persistent batched executor over a fixed set of resident scenes: owns a
+ * workspace sized for max_candidates brick candidates / max_jobs brushes and
+ * its own stream. With use_graph != 0 the whole device pipeline is captured
+ * into a hipGraph on the first run and replayed afterwards (every count it
+ * works on is device-resident). run: batch i applies h_brushes[i] (host) to
+ * scenes[scene_of_batch[i]], in array order per scene; *candidates receives
+ * the batch set's candidate count (VX_INVALID_VALUE if above the capacity).
+ * No counterpart in the reference. */
+typedef struct vx_batcher vx_batcher;
+vx_status vx_batcher_create(vx_dag* const* scenes, uint32_t n_scenes, uint64_t max_candidates, uint32_t max_jobs,
+                            int32_t use_graph, vx_batcher** out);
+vx_status vx_batcher_destroy(vx_batcher* b);
+vx_status vx_batcher_info(const vx_batcher* b, uint64_t* workspace_bytes, int32_t* graph_ready);
+vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batch, const vx_brush* const* h_brushes,
+                         const uint32_t* counts, uint32_t n_batches, uint64_t* candidates);
+
 /* get_voxel for n device coordinates: d_out[i] = material (1 = solid in an
  * occupancy DAG), 0 if empty or out of the world (query.cuh:12-39). */
 vx_status vx_dag_get_voxels(const vx_dag* dag, const vx_voxel* d_xyz, uint32_t n, uint32_t* d_out, void* stream);

@@ -207,3 +207,29 @@ def test_batch_of_one_scene_matches_plain(gpu):
     for br in bs:
         b.apply_brushes(br)
     assert np.array_equal(a.export(), b.export())
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batcher_replays_match_sequential(gpu, use_graph):
+    """The persistent executor (graph captured once, replayed for new batch
+    sets) gives the same scenes as plain per-batch calls."""
+    from voxel_amd.dag import Batcher, batch_candidates, terrain_for_world
+    depths = [4, 6, 5]
+    a = [_dag(d, True, node_words=1 << 22, leaf_words=1 << 23) for d in depths]
+    b = [_dag(d, True, node_words=1 << 22, leaf_words=1 << 23) for d in depths]
+    for k, d in enumerate(depths):
+        a[k].build_terrain(terrain_for_world(4 << d, 30 + k))
+        b[k].build_terrain(terrain_for_world(4 << d, 30 + k))
+    rounds = []
+    for r in range(3):
+        rounds.append([(s, O.gen_brushes(200 + 10 * r + i, O.STREAM_BRUSH, 4 + i, 4 << depths[s], 0))
+                       for i, s in enumerate([0, 1, 2, 1, 0])])
+    cap = max(batch_candidates(a, rb) for rb in rounds)
+    ex = Batcher(a, cap, 64, use_graph=use_graph)
+    for rb in rounds:
+        ex.run(rb)
+        for s, br in rb:
+            b[s].apply_brushes(br)
+        for k in range(3):
+            assert np.array_equal(a[k].export(), b[k].export()), k
+    assert ex.info()[1] == use_graph

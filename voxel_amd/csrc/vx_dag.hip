@@ -269,11 +269,11 @@ struct RecordOut {
 struct BrushLoad {
   using Payload = Record;
   const BrushJob* jobs;
-  uint32_t n_jobs;
+  const uint32_t* n_jobs;  // device count
   bool materials;
   Codec codec;
   __device__ uint32_t operator()(uint32_t i, Record& rec) const {
-    uint32_t lo = 0, hi = n_jobs;  // last job with start <= i
+    uint32_t lo = 0, hi = *n_jobs;  // last job with start <= i
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
       if (jobs[mid].start <= i) lo = mid;
@@ -307,7 +307,7 @@ struct VoxelLoad {
 
 struct RecordStore {
   RecordOut out;
-  uint32_t n;
+  const uint32_t* n_ptr;  // candidates (device count)
   __device__ void operator()(uint32_t i, const Record& rec, uint32_t v, uint32_t excl, uint32_t incl) const {
     if (v) {
       out.key[excl] = rec.key;
@@ -315,7 +315,7 @@ struct RecordStore {
       out.mask[excl] = rec.mask;
       out.mat[excl] = rec.mat;
     }
-    if (i == n - 1) *out.n_rec = incl;
+    if (i == *n_ptr - 1) *out.n_rec = incl;
   }
 };
 
@@ -728,29 +728,57 @@ vx_status finish(vx_dag* g, const std::vector<vx_dag*>& dags, const EditWs& w, c
   return VX_OK;
 }
 
-// Shared driver: records from `load` (host count n_cand), then rebuild.
+// Counts words the host uploads before the device pipeline runs.
+constexpr uint32_t kCntCandidates = 62;
+constexpr uint32_t kCntJobs = 63;
+
+// Host -> device inputs of one edit: scene table, brush jobs, candidate and
+// job counts. `staging` must stay alive until the stream synchronises.
+vx_status upload(EditWs& w, const std::vector<Dev>& devs, const std::vector<BrushJob>* jobs, uint32_t n_cand,
+                 uint32_t* staging, hipStream_t s) {
+  if (hipMemcpyAsync(w.scenes, devs.data(), devs.size() * sizeof(Dev), hipMemcpyHostToDevice, s) != hipSuccess)
+    return VX_DEVICE_ERROR;
+  if (jobs && !jobs->empty() &&
+      hipMemcpyAsync(w.jobs, jobs->data(), jobs->size() * sizeof(BrushJob), hipMemcpyHostToDevice, s) != hipSuccess)
+    return VX_DEVICE_ERROR;
+  staging[0] = n_cand;
+  staging[1] = jobs ? uint32_t(jobs->size()) : 0u;
+  if (hipMemcpyAsync(&w.counts[kCntCandidates], staging, 8, hipMemcpyHostToDevice, s) != hipSuccess)
+    return VX_DEVICE_ERROR;
+  return VX_OK;
+}
+
+// All device work of an edit (no host synchronisation, no memset nodes), so a
+// stream capture of it replays as a hipGraph. Grids are sized by the
+// capacities in `w`; the actual counts are read on the device.
+template <class Load>
+vx_status device_pipeline(vx_dag* prof_dag, const Plan& p, const Scenes& sc, EditWs& w, uint64_t cand_cap, Load load,
+                          hipStream_t s) {
+  vx_status st;
+  if ((st = fill_u32(w.counts, kCntCandidates, 0, s)) != VX_OK) return st;
+  if ((st = fill_u32(w.root_pos, p.n_scenes, kNone, s)) != VX_OK) return st;
+  begin_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes);
+  if ((st = last_launch()) != VX_OK) return st;
+  mark(prof_dag, 0, s);
+  if ((st = scan::fused_dev<scan::kSum>(&w.counts[kCntCandidates], cand_cap, load,
+                                        RecordStore{RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0]},
+                                                    &w.counts[kCntCandidates]},
+                                        w.fused_ws, s)) != VX_OK)
+    return st;
+  return rebuild(prof_dag, p, sc, w, cand_cap, s);
+}
+
+// Shared driver of the direct (uncaptured) calls.
 template <class Load>
 vx_status run_edit(const std::vector<vx_dag*>& dags, vx_dag* prof_dag, const Plan& p, EditWs& w,
                    const std::vector<BrushJob>* jobs, uint32_t n_cand, Load load, hipStream_t s) {
   vx_status st;
   std::vector<Dev> devs;
   for (vx_dag* g : dags) devs.push_back(g->dev);
-  if (hipMemcpyAsync(w.scenes, devs.data(), devs.size() * sizeof(Dev), hipMemcpyHostToDevice, s) != hipSuccess)
-    return VX_DEVICE_ERROR;
-  if (jobs && !jobs->empty() &&
-      hipMemcpyAsync(w.jobs, jobs->data(), jobs->size() * sizeof(BrushJob), hipMemcpyHostToDevice, s) != hipSuccess)
-    return VX_DEVICE_ERROR;
-  if ((st = fill_u32(w.counts, 64, 0, s)) != VX_OK) return st;
-  if ((st = fill_u32(w.root_pos, p.n_scenes, kNone, s)) != VX_OK) return st;
+  uint32_t staging[2];
+  if ((st = upload(w, devs, jobs, n_cand, staging, s)) != VX_OK) return st;
   const Scenes sc{p.n_scenes > 1 ? w.scenes : nullptr, nullptr, p.codec, devs[0]};
-  begin_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes);
-  if ((st = last_launch()) != VX_OK) return st;
-  mark(prof_dag, 0, s);
-  if ((st = scan::fused<scan::kSum>(n_cand, load,
-                                    RecordStore{RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0]}, n_cand},
-                                    w.fused_ws, s)) != VX_OK)
-    return st;
-  if ((st = rebuild(prof_dag, p, sc, w, n_cand, s)) != VX_OK) return st;
+  if ((st = device_pipeline(prof_dag, p, sc, w, n_cand, load, s)) != VX_OK) return st;
   return finish(prof_dag, dags, w, p, s);
 }
 
@@ -920,7 +948,7 @@ extern "C" vx_status vx_dag_apply_brushes(vx_dag* g, const vx_brush* b, uint32_t
   const Plan p = make_plan({g});
   EditWs w = carve(ws, p, cand, uint32_t(jobs.size()), cand);
   if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
-  return run_edit({g}, g, p, w, &jobs, uint32_t(cand), BrushLoad{w.jobs, uint32_t(jobs.size()), g->materials, p.codec},
+  return run_edit({g}, g, p, w, &jobs, uint32_t(cand), BrushLoad{w.jobs, &w.counts[kCntJobs], g->materials, p.codec},
                   static_cast<hipStream_t>(stream));
 }
 
@@ -974,7 +1002,7 @@ extern "C" vx_status vx_dag_apply_batches(vx_dag* const* dags, const vx_brush* c
   EditWs w = carve(ws, p, cand, uint32_t(jobs.size()), cand);
   if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
   return run_edit(scenes, scenes[0], p, w, &jobs, uint32_t(cand),
-                  BrushLoad{w.jobs, uint32_t(jobs.size()), scenes[0]->materials, p.codec},
+                  BrushLoad{w.jobs, &w.counts[kCntJobs], scenes[0]->materials, p.codec},
                   static_cast<hipStream_t>(stream));
 }
 
@@ -1133,4 +1161,125 @@ extern "C" vx_status vx_brush_voxel_ops(const vx_brush* b, uint32_t n, uint32_t 
   }
   *ops = total;
   return VX_OK;
+}
+
+// ---- persistent batched executor ------------------------------------------------
+// Owns a workspace sized for a candidate capacity and its own stream; the whole
+// device pipeline of a batched edit is captured into a hipGraph on the first
+// run and replayed for every later batch set within the capacity (the counts
+// it works on are uploaded to device memory before each replay).
+struct vx_batcher {
+  std::vector<vx_dag*> scenes;
+  std::unordered_map<const vx_dag*, uint32_t> index;
+  Plan plan;
+  uint64_t cap_cand = 0;
+  uint32_t cap_jobs = 0;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  EditWs w{};
+  hipStream_t stream = nullptr;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<Dev> devs;
+  std::vector<BrushJob> jobs;
+  uint32_t staging[2] = {0, 0};
+};
+
+extern "C" vx_status vx_batcher_destroy(vx_batcher* b) {
+  if (!b) return VX_INVALID_VALUE;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->ws) hipFree(b->ws);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_batcher_create(vx_dag* const* scenes, uint32_t n_scenes, uint64_t max_candidates,
+                                       uint32_t max_jobs, int32_t use_graph, vx_batcher** out) {
+  if (!scenes || !n_scenes || !out || !max_candidates || !max_jobs || max_candidates >= (1u << 30))
+    return VX_INVALID_VALUE;
+  auto* b = new (std::nothrow) vx_batcher();
+  if (!b) return VX_DEVICE_ERROR;
+  for (uint32_t i = 0; i < n_scenes; ++i) {
+    if (!scenes[i] || scenes[i]->materials != scenes[0]->materials || !b->index.emplace(scenes[i], i).second) {
+      delete b;
+      return VX_INVALID_VALUE;
+    }
+    b->scenes.push_back(scenes[i]);
+    b->devs.push_back(scenes[i]->dev);
+  }
+  b->plan = make_plan(std::vector<const vx_dag*>(b->scenes.begin(), b->scenes.end()));
+  if (b->plan.key_bits > 64) {
+    delete b;
+    return VX_INVALID_VALUE;
+  }
+  b->cap_cand = max_candidates;
+  b->cap_jobs = max_jobs;
+  b->use_graph = use_graph != 0;
+  b->ws_bytes = carve(nullptr, b->plan, max_candidates, max_jobs, max_candidates).bytes;
+  if (hipMalloc(&b->ws, b->ws_bytes) != hipSuccess) {
+    b->ws = nullptr;
+    vx_batcher_destroy(b);
+    return VX_OUT_OF_CAPACITY;
+  }
+  b->w = carve(b->ws, b->plan, max_candidates, max_jobs, max_candidates);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    b->stream = nullptr;
+    vx_batcher_destroy(b);
+    return VX_DEVICE_ERROR;
+  }
+  *out = b;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_batcher_info(const vx_batcher* b, uint64_t* workspace_bytes, int32_t* graph_ready) {
+  if (!b) return VX_INVALID_VALUE;
+  if (workspace_bytes) *workspace_bytes = b->ws_bytes;
+  if (graph_ready) *graph_ready = b->exec != nullptr;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batch, const vx_brush* const* brushes,
+                                    const uint32_t* counts, uint32_t n_batches, uint64_t* candidates) {
+  if (!b || (n_batches && (!scene_of_batch || !brushes || !counts))) return VX_INVALID_VALUE;
+  for (uint32_t i = 0; i < n_batches; ++i)
+    if (scene_of_batch[i] >= b->scenes.size() || (!brushes[i] && counts[i])) return VX_INVALID_VALUE;
+  b->jobs.clear();
+  uint64_t cand = 0;
+  for (uint32_t s = 0; s < b->scenes.size(); ++s)
+    for (uint32_t i = 0; i < n_batches; ++i)
+      if (scene_of_batch[i] == s) cand += make_jobs(brushes[i], counts[i], 4u << b->scenes[s]->depth, s, cand, &b->jobs);
+  if (candidates) *candidates = cand;
+  if (cand > b->cap_cand || b->jobs.size() > b->cap_jobs) return VX_INVALID_VALUE;
+  if (cand == 0) return VX_OK;
+  hipStream_t s = b->stream;
+  vx_status st;
+  if ((st = upload(b->w, b->devs, &b->jobs, uint32_t(cand), b->staging, s)) != VX_OK) return st;
+  const Scenes sc{b->plan.n_scenes > 1 ? b->w.scenes : nullptr, nullptr, b->plan.codec, b->devs[0]};
+  const BrushLoad load{b->w.jobs, &b->w.counts[kCntJobs], b->scenes[0]->materials, b->plan.codec};
+  if (b->use_graph) {
+    if (!b->exec) {
+      if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) return VX_DEVICE_ERROR;
+      const vx_status cap_st = device_pipeline(nullptr, b->plan, sc, b->w, b->cap_cand, load, s);
+      hipGraph_t g = nullptr;
+      const hipError_t end = hipStreamEndCapture(s, &g);
+      if (cap_st != VX_OK || end != hipSuccess || !g) {
+        if (g) hipGraphDestroy(g);
+        return cap_st != VX_OK ? cap_st : VX_DEVICE_ERROR;
+      }
+      if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        hipGraphDestroy(g);
+        b->exec = nullptr;
+        return VX_DEVICE_ERROR;
+      }
+      b->graph = g;
+    }
+    if (hipGraphLaunch(b->exec, s) != hipSuccess) return VX_DEVICE_ERROR;
+  } else {
+    if ((st = device_pipeline(nullptr, b->plan, sc, b->w, b->cap_cand, load, s)) != VX_OK) return st;
+  }
+  return finish(nullptr, b->scenes, b->w, b->plan, s);
 }

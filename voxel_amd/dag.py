@@ -59,6 +59,10 @@ register({
                                   C.POINTER(_u32)]),
     "vx_dag_batch_workspace_size": (_i32, [_P, _P, _P, _u32, C.POINTER(_sz)]),
     "vx_dag_apply_batches": (_i32, [_P, _P, _P, _u32, _P, _sz, _P]),
+    "vx_batcher_create": (_i32, [_P, _u32, _u64, _u32, _i32, C.POINTER(_P)]),
+    "vx_batcher_destroy": (_i32, [_P]),
+    "vx_batcher_info": (_i32, [_P, C.POINTER(_u64), C.POINTER(_i32)]),
+    "vx_batcher_run": (_i32, [_P, _P, _P, _P, _u32, C.POINTER(_u64)]),
     "vx_dag_profile_enable": (_i32, [_P, _i32]),
     "vx_dag_profile_read": (_i32, [_P, _P]),
 })
@@ -235,6 +239,60 @@ def apply_batches(dags, batches, workspace=None, stream=None):
     wsb = 0 if workspace is None else workspace.numel()
     check(lib().vx_dag_apply_batches(a.dags, a.ptrs, a.counts, a.n, _ptr(workspace), wsb, _stream(stream)),
           "vx_dag_apply_batches")
+
+
+def batch_candidates(scenes, batches) -> int:
+    """Brick candidates of (scene index, brushes) batches (sizes a Batcher)."""
+    total = 0
+    for s, br in batches:
+        world = scenes[s].world
+        for b in br:
+            ext = (b["hx"],) * 3 if b["kind"] == 0 else (b["hx"], b["hy"], b["hz"])
+            n = 1
+            for c, e in zip((b["cx"], b["cy"], b["cz"]), ext):
+                lo, hi = max(0, int(c) - int(e)), min(world - 1, int(c) + int(e))
+                n *= max(0, (hi >> 2) - (lo >> 2) + 1)
+            total += n
+    return total
+
+
+class Batcher:
+    """Persistent batched executor (vx_batcher): one pipeline per batch set,
+    captured once into a hipGraph and replayed."""
+
+    def __init__(self, scenes, max_candidates: int, max_jobs: int, use_graph: bool = True):
+        self.scenes = list(scenes)
+        arr = (_P * len(self.scenes))(*[d.h.value for d in self.scenes])
+        self.h = C.c_void_p()
+        check(lib().vx_batcher_create(arr, len(self.scenes), max_candidates, max_jobs, int(use_graph),
+                                      C.byref(self.h)), "vx_batcher_create")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().vx_batcher_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        b, g = C.c_uint64(), C.c_int32()
+        check(lib().vx_batcher_info(self.h, C.byref(b), C.byref(g)), "vx_batcher_info")
+        return b.value, bool(g.value)
+
+    def run(self, batches) -> int:
+        """batches: list of (scene index, brush array). Returns candidates."""
+        arrays = [np.ascontiguousarray(br, BRUSH_DTYPE) for _, br in batches]
+        n = len(arrays)
+        scene_of = (_u32 * n)(*[s for s, _ in batches])
+        ptrs = (_P * n)(*[a.ctypes.data for a in arrays])
+        counts = (_u32 * n)(*[len(a) for a in arrays])
+        cand = C.c_uint64()
+        check(lib().vx_batcher_run(self.h, scene_of, ptrs, counts, n, C.byref(cand)), "vx_batcher_run")
+        return cand.value
 
 
 def smoke_check(dev) -> None:
