@@ -57,6 +57,13 @@ vx_status vx_exclusive_max_u32(uint32_t* data, uint32_t n, void* ws,
 vx_status vx_inclusive_max_u32(uint32_t* data, uint32_t n, void* ws,
                                size_t ws_bytes, void* stream);
 
+/* Scan tile shape (the reference picks it by template,
+ * cuda/scan/block_based.cuh:13-61): workgroup `block` in {256, 512, 1024} and
+ * uint4 loads per lane `vectors_per_lane` in {4, 8, 16} (block * 4 * vectors
+ * elements per tile; supported pairs: 256x{4,8,16}, 512x{4,8}, 1024x4). */
+vx_status vx_scan_set_tuning(uint32_t block, uint32_t vectors_per_lane);
+vx_status vx_scan_get_tuning(uint32_t* block, uint32_t* vectors_per_lane);
+
 /* Stream compaction: the reference builds it from the fused scan
  * (cuda/scan/fused.cuh:222-269, used at hash_dag_gpu/detail/edit/requests.cuh:51-122).
  * out[k] = in[i] for the k-th i (in order) with flags[i] != 0; *d_count (a

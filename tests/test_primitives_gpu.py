@@ -173,3 +173,24 @@ def test_device_generators_match_oracle(gpu):
     assert np.array_equal(P.gen_morton(2, 1, 10_000, 14).cpu().numpy().view(np.uint64), O.gen_morton(2, 1, 10_000, 14))
     assert np.array_equal(P.gen_unique_u64(2, 4, 0, 10_000).cpu().numpy().view(np.uint64),
                           O.gen_unique_u64(2, 4, 10_000))
+
+
+@pytest.mark.parametrize("block,vec", [(256, 4), (256, 8), (256, 16), (512, 4), (512, 8), (1024, 4)])
+def test_scan_tile_shapes_agree(gpu, block, vec):
+    import voxel_amd.primitives as P
+    old = P.get_scan_tuning()
+    try:
+        P.set_scan_tuning(block, vec)
+        for n in (5000, 70_001, 1 << 20):
+            a = O.gen_u32(11, O.STREAM_SCAN, n, 0, 15)
+            d = _dev_u32(a, gpu)
+            P.inclusive_sum(d)
+            assert np.array_equal(_host_u32(d), O.inclusive_sum(a))
+    finally:
+        P.set_scan_tuning(*old)
+
+
+def test_sort_deep_lookback_many_tiles(gpu):
+    """Enough tiles that predecessor walks cross several look-back windows."""
+    _sort_check(gpu, O.gen_u32(77, O.STREAM_KEYS, 3_000_017, 0, 0xFFFFFFFF))
+    _sort_check(gpu, O.gen_u32(78, O.STREAM_KEYS, 2_000_003, 0, 3))

@@ -60,6 +60,8 @@ SIGNATURES = {
     "vx_exclusive_max_u32": (_i32, [_P, _u32, _P, _sz, _P]),
     "vx_inclusive_max_u32": (_i32, [_P, _u32, _P, _sz, _P]),
     "vx_compact_u32": (_i32, [_P, _P, _u32, _P, _P, _P, _sz, _P]),
+    "vx_scan_set_tuning": (_i32, [_u32, _u32]),
+    "vx_scan_get_tuning": (_i32, [_pu32, _pu32]),
     "vx_sort_workspace_size": (_i32, [_u32, _u32, _u32, _i32, _psz]),
     "vx_sort_pairs_u32": (_i32, [_P, _P, _u32, _u32, _P, _sz, _P]),
     "vx_sort_pairs_u64": (_i32, [_P, _P, _u32, _u32, _P, _sz, _P]),
@@ -91,7 +93,23 @@ def lib():
             f.restype = res
             f.argtypes = args
         _lib = L
+        _apply_tuning(L)
     return _lib
+
+
+def _apply_tuning(L) -> None:
+    """Apply the autotuner's table (voxel_amd/tuning.json) if one was written."""
+    import json
+    path = os.path.join(HERE, "tuning.json")
+    if not os.path.exists(path):
+        return
+    with open(path) as f:
+        t = json.load(f)
+    for key, kb in (("sort_u32", 4), ("sort_u64", 8)):
+        if key in t:
+            L.vx_sort_set_tuning(kb, *t[key])
+    if "scan" in t:
+        L.vx_scan_set_tuning(*t["scan"])
 
 
 def register(extra: dict) -> None:

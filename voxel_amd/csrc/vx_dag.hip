@@ -109,7 +109,10 @@ __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, c
     uint64_t out = res_make(kResFinal, kEmpty);
     bool done = false;
     for (uint64_t probe = 0; probe < limit && !done; ++probe) {
-      uint64_t s = ld_agent(&t.slots[idx]);
+      // Plain (cacheable) read: during a probe launch a slot only goes from
+      // empty to tagged, so a stale read can only show "empty", and the CAS
+      // below then returns the slot's real value.
+      uint64_t s = t.slots[idx];
       if (s == 0) {
         const uint64_t mine = (uint64_t(fp) << 32) | kTag | i;
         const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&t.slots[idx]), 0ull,

@@ -42,19 +42,19 @@ __device__ __forceinline__ uint32_t wave_incl(uint32_t v) {
   else return wave_inclusive_max(v);
 }
 
-// Scan of a tile whose kBlock*16 elements were loaded as 4 uint4 per lane in
+// Scan of a tile whose kBlock*4K elements were loaded as K uint4 per lane in
 // wave-segment order; returns, per lane and round, the exclusive prefix of the
 // first element of the vector within the tile, and the tile aggregate.
-template <Op OP, uint32_t kBlock>
+template <Op OP, uint32_t kBlock, int K>
 struct TileScan {
   static constexpr uint32_t kWaves = kBlock / kWave;
   uint32_t* s_wave;  // [kWaves] LDS
 
-  __device__ uint32_t run(const uint4 (&v)[4], uint32_t (&excl)[4]) {
+  __device__ uint32_t run(const uint4 (&v)[K], uint32_t (&excl)[K]) {
     const uint32_t lane = lane_id(), wave = wave_id();
     uint32_t carry = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < K; ++k) {
       const uint32_t s = combine<OP>(combine<OP>(v[k].x, v[k].y), combine<OP>(v[k].z, v[k].w));
       const uint32_t inc = wave_incl<OP>(s);
       uint32_t ex = __shfl_up(inc, 1, kWave);
@@ -72,7 +72,7 @@ struct TileScan {
       total = combine<OP>(total, t);
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) excl[k] = combine<OP>(before, excl[k]);
+    for (int k = 0; k < K; ++k) excl[k] = combine<OP>(before, excl[k]);
     return total;
   }
 };
@@ -86,10 +86,13 @@ struct TileScan {
 
 namespace {
 
-template <Op OP, bool INCL, uint32_t kBlock>
+// Each wave owns 64*4*K consecutive elements and moves them as K rounds of
+// 64 lanes x uint4 (1 KiB per load instruction).
+template <Op OP, bool INCL, uint32_t kBlock, int K>
 __global__ __launch_bounds__(kBlock) void scan_vec_kernel(uint32_t* __restrict__ data, uint32_t n,
                                                          uint32_t* counter, uint64_t* rec) {
-  constexpr uint32_t kTile = kBlock * 16;
+  constexpr uint32_t kSeg = kWave * 4 * K;
+  constexpr uint32_t kTile = kBlock * 4 * K;
   __shared__ uint32_t s_wave[kBlock / kWave];
   __shared__ uint32_t s_tile, s_prefix;
   const bool multi = gridDim.x > 1;
@@ -97,11 +100,11 @@ __global__ __launch_bounds__(kBlock) void scan_vec_kernel(uint32_t* __restrict__
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint32_t lane = lane_id(), wave = wave_id();
-  const uint64_t seg = uint64_t(tile) * kTile + wave * 1024u;
+  const uint64_t seg = uint64_t(tile) * kTile + uint64_t(wave) * kSeg;
 
-  uint4 v[4];
+  uint4 v[K];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < K; ++k) {
     const uint64_t i = seg + k * 256u + lane * 4u;
     if (i + 3 < n) {
       v[k] = *reinterpret_cast<const uint4*>(data + i);
@@ -112,8 +115,8 @@ __global__ __launch_bounds__(kBlock) void scan_vec_kernel(uint32_t* __restrict__
       v[k].w = 0u;
     }
   }
-  uint32_t excl[4];
-  TileScan<OP, kBlock> ts{s_wave};
+  uint32_t excl[K];
+  TileScan<OP, kBlock, K> ts{s_wave};
   const uint32_t agg = ts.run(v, excl);
 
   if (wave == 0) {
@@ -133,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void scan_vec_kernel(uint32_t* __restrict__
   const uint32_t prefix = s_prefix;
 
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < K; ++k) {
     const uint64_t i = seg + k * 256u + lane * 4u;
     uint32_t a = combine<OP>(prefix, excl[k]);
     uint4 o;
@@ -158,12 +161,20 @@ __global__ __launch_bounds__(kBlock) void scan_vec_kernel(uint32_t* __restrict__
   }
 }
 
-constexpr uint32_t kScanBlock = 256;
-constexpr uint32_t kScanTile = kScanBlock * 16;
+// The smallest tile any configuration uses; the workspace is sized for it so
+// that every tuning fits one workspace query.
+constexpr uint32_t kMinScanTile = 256 * 16;
 
-template <Op OP, bool INCL>
-vx_status launch(uint32_t* data, uint32_t n, void* ws, hipStream_t s) {
-  const uint32_t tiles = ceil_div(n, kScanTile);
+struct ScanTuning {
+  uint32_t block = 256;
+  uint32_t k = 8;
+};
+ScanTuning g_scan_tuning;
+
+template <Op OP, bool INCL, uint32_t kBlock, int K>
+vx_status launch_cfg(uint32_t* data, uint32_t n, void* ws, hipStream_t s) {
+  constexpr uint32_t kTile = kBlock * 4 * K;
+  const uint32_t tiles = ceil_div(n, kTile);
   uint32_t* counter = nullptr;
   uint64_t* rec = nullptr;
   if (tiles > 1) {
@@ -173,21 +184,48 @@ vx_status launch(uint32_t* data, uint32_t n, void* ws, hipStream_t s) {
     const vx_status st = fill_u32(counter, (reinterpret_cast<char*>(rec + tiles) - reinterpret_cast<char*>(counter)) / 4, 0, s);
     if (st != VX_OK) return st;
   }
-  // Unaligned pointers fall back to element loads inside the kernel only at
-  // the tail; the body requires 16-byte alignment.
-  scan_vec_kernel<OP, INCL, kScanBlock><<<tiles, kScanBlock, 0, s>>>(data, n, counter, rec);
+  scan_vec_kernel<OP, INCL, kBlock, K><<<tiles, kBlock, 0, s>>>(data, n, counter, rec);
   return last_launch();
+}
+
+template <Op OP, bool INCL>
+vx_status launch(uint32_t* data, uint32_t n, void* ws, hipStream_t s) {
+  const ScanTuning tn = g_scan_tuning;
+  // A single tile never needs the workspace: use the smallest configuration
+  // that covers n when the caller passed none.
+  if (n <= kMinScanTile) return launch_cfg<OP, INCL, 256, 4>(data, n, ws, s);
+  if (tn.block == 256 && tn.k == 4) return launch_cfg<OP, INCL, 256, 4>(data, n, ws, s);
+  if (tn.block == 256 && tn.k == 16) return launch_cfg<OP, INCL, 256, 16>(data, n, ws, s);
+  if (tn.block == 512 && tn.k == 4) return launch_cfg<OP, INCL, 512, 4>(data, n, ws, s);
+  if (tn.block == 512 && tn.k == 8) return launch_cfg<OP, INCL, 512, 8>(data, n, ws, s);
+  if (tn.block == 1024 && tn.k == 4) return launch_cfg<OP, INCL, 1024, 4>(data, n, ws, s);
+  return launch_cfg<OP, INCL, 256, 8>(data, n, ws, s);
 }
 
 }  // namespace
 
 size_t workspace_bytes(uint32_t n) {
-  const uint32_t tiles = ceil_div(n, kScanTile);
+  const uint32_t tiles = ceil_div(n, kMinScanTile);
   if (tiles <= 1) return 0;
   Carver c(nullptr);
   c.take<uint32_t>(64);
   c.take<uint64_t>(tiles);
   return c.size();
+}
+
+vx_status set_tuning(uint32_t block, uint32_t k) {
+  const bool ok = (block == 256 && (k == 4 || k == 8 || k == 16)) || (block == 512 && (k == 4 || k == 8)) ||
+                  (block == 1024 && k == 4);
+  if (!ok) return VX_UNSUPPORTED;
+  g_scan_tuning = ScanTuning{block, k};
+  return VX_OK;
+}
+
+vx_status get_tuning(uint32_t* block, uint32_t* k) {
+  if (!block || !k) return VX_INVALID_VALUE;
+  *block = g_scan_tuning.block;
+  *k = g_scan_tuning.k;
+  return VX_OK;
 }
 
 vx_status run(uint32_t* data, uint32_t n, bool inclusive, Op op, void* ws, hipStream_t s) {
@@ -275,6 +313,14 @@ extern "C" vx_status vx_exclusive_max_u32(uint32_t* d, uint32_t n, void* ws, siz
 }
 extern "C" vx_status vx_inclusive_max_u32(uint32_t* d, uint32_t n, void* ws, size_t b, void* s) {
   return scan_entry(d, n, ws, b, s, true, vx::scan::kMax);
+}
+
+extern "C" vx_status vx_scan_set_tuning(uint32_t block, uint32_t vectors_per_lane) {
+  return vx::scan::set_tuning(block, vectors_per_lane);
+}
+
+extern "C" vx_status vx_scan_get_tuning(uint32_t* block, uint32_t* vectors_per_lane) {
+  return vx::scan::get_tuning(block, vectors_per_lane);
 }
 
 extern "C" vx_status vx_compact_u32(const uint32_t* in, const uint32_t* flags, uint32_t n, uint32_t* out,

@@ -198,18 +198,31 @@ __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ 
         if constexpr (VALS) s_vals[p] = val[j];
       }
     }
-    // Per-digit look-back over predecessor tiles.
+    // Per-digit look-back over predecessor tiles, kLookWindow records per
+    // round trip: the loads of one window are independent, so a walk across
+    // tiles that have only published aggregates costs one memory latency per
+    // window instead of one per tile.
     for (uint32_t d = t; d < R; d += BLOCK) {
       uint32_t excl = 0;
       if (tile > 0) {
+        constexpr int kLookWindow = 8;
         int64_t j = int64_t(tile) - 1;
-        while (j >= 0) {
-          const uint32_t r = ld_agent(&rec[size_t(j) * R + d]);
-          const uint32_t st = r & ~kValueMask;
-          if (st == 0) continue;
-          excl += r & kValueMask;
-          if (st == kStatusPrefix) break;
-          --j;
+        bool done = false;
+        while (!done) {
+          uint32_t r[kLookWindow];
+#pragma unroll
+          for (int q = 0; q < kLookWindow; ++q)
+            r[q] = j - q >= 0 ? ld_agent(&rec[size_t(j - q) * R + d]) : kStatusPrefix;
+          int consumed = 0;
+#pragma unroll
+          for (int q = 0; q < kLookWindow; ++q) {
+            const uint32_t st = r[q] & ~kValueMask;
+            if (done || consumed != q || st == 0) continue;  // stop at the first unpublished record
+            excl += r[q] & kValueMask;
+            consumed = q + 1;
+            if (st == kStatusPrefix) done = true;
+          }
+          j -= consumed;
         }
         st_agent(&rec[size_t(tile) * R + d], kStatusPrefix | (excl + s_cnt[d]));
       }

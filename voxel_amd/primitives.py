@@ -123,6 +123,16 @@ def get_sort_tuning(key_bytes: int):
     return a.value, b.value, c.value
 
 
+def set_scan_tuning(block: int, vectors_per_lane: int) -> None:
+    check(lib().vx_scan_set_tuning(block, vectors_per_lane), "vx_scan_set_tuning")
+
+
+def get_scan_tuning():
+    a, b = C.c_uint32(), C.c_uint32()
+    check(lib().vx_scan_get_tuning(C.byref(a), C.byref(b)), "vx_scan_get_tuning")
+    return a.value, b.value
+
+
 # ---- generators ---------------------------------------------------------------
 def gen_u32(seed, stream_id, n, lo, hi, device="cuda"):
     out = torch.empty(n, dtype=torch.int32, device=device)
