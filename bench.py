@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-primitives", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--cpu-brushes", type=int, default=1024, help="brushes in the CPU baseline sample")
     return ap.parse_args()
 
@@ -137,6 +139,98 @@ def primitives(torch, P):
     return out
 
 
+def _sample_equal(V, torch, a, b, n=1 << 20, seed=0):
+    """Same material at n random voxels of two scenes (a cheap full-size check
+    beside the exact canonical-export comparisons of the test suite)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    q = torch.from_numpy(rng.integers(0, a.world, (n, 3), dtype=np.int64).astype(np.int32)).cuda()
+    return bool(torch.equal(a.get_voxels(q), b.get_voxels(q)))
+
+
+def leg_batched(V, torch, n_batches=256, seed=3):
+    """256 batches (log-uniform 2^8..2^20 voxel ops, config-2 shapes scaled)
+    over 8 resident scenes of inner depth 8..12 (worlds 1024^3..16384^3,
+    heightfields seeds 10..17): one batched graph-replayed pipeline vs the
+    same batches as sequential plain calls."""
+    from voxel_amd.serving import make_requests
+    depths = [8, 9, 10, 11, 12, 9, 10, 11]
+    worlds = [4 << d for d in depths]
+
+    def scenes():
+        out = []
+        for k, d in enumerate(depths):
+            g = V.HashDagGpu(d, True, **V.scene_capacity(d))
+            g.build_terrain(V.terrain_for_world(4 << d, 10 + k))
+            out.append(g)
+        return out
+
+    plain, batched = scenes(), scenes()
+    warm = [(r.scene, r.brushes) for r in make_requests(n_batches, 8, worlds, seed + 100, 8, 20, 16)]
+    timed = [(r.scene, r.brushes) for r in make_requests(n_batches, 8, worlds, seed, 8, 20, 16)]
+    ops = sum(V.brush_voxel_ops(b, worlds[s]) for s, b in timed)
+    cap = max(V.batch_candidates(batched, warm), V.batch_candidates(batched, timed))
+    jobs = max(sum(len(b) for _, b in warm), sum(len(b) for _, b in timed))
+    ex = V.Batcher(batched, cap, jobs, use_graph=True)
+    ws = torch.empty(max(plain[s].brush_workspace_size(b) for s, b in warm + timed), dtype=torch.uint8,
+                     device="cuda")
+    for s, b in warm:
+        plain[s].apply_brushes(b, workspace=ws)
+    ex.run(warm)  # captures the graph
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, b in timed:
+        plain[s].apply_brushes(b, workspace=ws)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ex.run(timed)
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    same = all(_sample_equal(V, torch, p, b, seed=k) for k, (p, b) in enumerate(zip(plain, batched)))
+    ex.close()
+    for g in plain + batched:
+        g.close()
+    return {
+        "workload": f"{n_batches} batches (2^8..2^20 voxel ops) over 8 scenes, inner depths {depths}",
+        "voxel_ops": ops,
+        "plain_s": t_plain,
+        "batched_s": t_batched,
+        "plain_edits_per_s": ops / t_plain,
+        "batched_edits_per_s": ops / t_batched,
+        "speedup": t_plain / t_batched,
+        "same_result_sampled": same,
+    }
+
+
+def leg_serving(V, torch, n_requests=2000, seed=20, load=0.7):
+    """16 resident scenes (inner depths 10..13 = worlds 4096^3..32768^3,
+    heightfields seeds 20..35); requests of 1..64 brushes (2^6..2^18 voxel
+    ops) arrive as a Poisson process at `load` x the measured saturated rate."""
+    from voxel_amd.serving import Server, make_requests, poisson_arrivals, serve
+    depths = [10, 11, 12, 13] * 4
+    worlds = [4 << d for d in depths]
+    scenes = []
+    for k, d in enumerate(depths):
+        g = V.HashDagGpu(d, True, **V.scene_capacity(d))
+        g.build_terrain(V.terrain_for_world(4 << d, 20 + k))
+        scenes.append(g)
+    cal = make_requests(n_requests // 2, len(depths), worlds, seed + 1000, 6, 18, 64)
+    cap = 1 << 22
+    srv = Server(scenes, max_candidates=cap, max_jobs=1 << 14)
+    sat = serve(srv, cal, [0.0] * len(cal), max_tick=256)
+    rate = load * sat["requests"] / sat["makespan_s"]
+    reqs = make_requests(n_requests, len(depths), worlds, seed, 6, 18, 64)
+    srv.ticks, srv.tick_sizes = 0, []
+    res = serve(srv, reqs, poisson_arrivals(len(reqs), rate, seed))
+    srv.ex.close()
+    for g in scenes:
+        g.close()
+    res.update({"arrival_rate_per_s": rate, "saturated_requests_per_s": sat["requests"] / sat["makespan_s"],
+                "load": load, "scenes": f"16 scenes, inner depths {sorted(set(depths))}"})
+    return res
+
+
 def stage_bytes(prof, leaf_words):
     """Algorithmic HBM bytes per stage (summed over measured edits)."""
     R, L, N, P = prof.records, prof.leaves, prof.nodes, prof.sort_passes
@@ -208,7 +302,14 @@ def main():
     dom = max(stages, key=lambda s: stages[s])
     achieved = sbytes[dom] / (stages[dom] * 1e-3) / 1e9 if stages[dom] > 0 else 0.0
 
+    scene.close()
+    del ws
+    torch.cuda.empty_cache()
     prims = {} if args.no_primitives else primitives(torch, P)
+    torch.cuda.empty_cache()
+    batched = None if args.no_batched else leg_batched(V, torch)
+    torch.cuda.empty_cache()
+    serving = None if args.no_serving else leg_serving(V, torch)
 
     cpu_res = None
     if cpu is not None:
@@ -251,6 +352,8 @@ def main():
         "stages_ms": stages,
         "scene": {"build_s": build_s, "node_words": st.node_words_used, "leaf_words": st.leaf_words_used},
         "primitives": prims,
+        "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu_res,
     }
     print(json.dumps(line))

@@ -9,22 +9,24 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def _scenes(depths, materials=True):
+def _scenes(depths, materials=True, seeds=None):
     from voxel_amd.dag import HashDagGpu, terrain_for_world
     out = []
     for k, d in enumerate(depths):
         g = HashDagGpu(d, materials, node_words=1 << 22, leaf_words=1 << 23)
-        g.build_terrain(terrain_for_world(4 << d, 20 + k))
+        g.build_terrain(terrain_for_world(4 << d, (seeds or [20 + i for i in range(len(depths))])[k]))
         out.append(g)
     return out
 
 
-def test_served_scenes_equal_oracle_replay(gpu):
+@pytest.mark.parametrize("executor", ["direct", "graph"])
+def test_served_scenes_equal_oracle_replay(gpu, executor):
     from voxel_amd.serving import Server, make_requests, poisson_arrivals, serve
     depths = [5, 6, 4, 6]
     scenes = _scenes(depths)
     reqs = make_requests(48, len(depths), [4 << d for d in depths], seed=20, lo_log2=6, hi_log2=12, max_brushes=8)
-    res = serve(Server(scenes), reqs, poisson_arrivals(len(reqs), 4000.0, 1))
+    srv = Server(scenes) if executor == "direct" else Server(scenes, max_candidates=20_000, max_jobs=256)
+    res = serve(srv, reqs, poisson_arrivals(len(reqs), 4000.0, 1))
     assert res["completed"] == len(reqs) and res["rejected"] == 0
     assert res["ticks"] < len(reqs)  # requests were batched
     for s, d in enumerate(depths):
@@ -43,7 +45,7 @@ def test_result_same_alone_and_in_mix(gpu):
     mixed = _scenes(depths)
     serve(Server(mixed), reqs, poisson_arrivals(len(reqs), 5000.0, 2))
     for s in range(len(depths)):
-        alone = _scenes([depths[s]])
+        alone = _scenes([depths[s]], seeds=[20 + s])
         mine = [r for r in reqs if r.scene == s]
         for r in mine:
             r.scene = 0

@@ -106,6 +106,18 @@ def brush_voxel_ops(brushes: np.ndarray, world: int) -> int:
     return n.value
 
 
+def scene_capacity(inner_depth: int, materials: bool = True, headroom: float = 1.6) -> dict:
+    """Pool / table sizes for a heightfield scene of this depth plus edit
+    headroom (measured: the 16384^3 materials scene holds ~9.8e7 node words and
+    ~5.6e7 leaf words; each extra level multiplies both by ~4)."""
+    scale = 4.0 ** (inner_depth - 12)
+    node_words = int(min(2**31 - 64, max(1 << 20, 1.0e8 * scale * headroom + (1 << 22))))
+    leaf_words = int(min(2**31 - 64, max(1 << 20, (5.6e7 if materials else 1.2e7) * scale * headroom + (1 << 22))))
+    node_slots = 1 << max(16, (node_words // 2).bit_length())
+    leaf_slots = 1 << max(16, (leaf_words // (8 if materials else 1)).bit_length())
+    return dict(node_words=node_words, leaf_words=leaf_words, node_slots=node_slots, leaf_slots=leaf_slots)
+
+
 def terrain_for_world(world: int, seed: int, octaves: int = 4) -> Terrain:
     """The 512^3 heightfield (base 256, amplitude 64, period 64) scaled to `world`."""
     return Terrain(seed, world // 2, world // 8, world // 8, octaves)

@@ -49,18 +49,31 @@ def growth_bound(d: V.HashDagGpu, brushes: np.ndarray) -> tuple[int, int]:
 
 
 class Server:
-    def __init__(self, scenes: list[V.HashDagGpu]):
+    """Tick executor. With `max_candidates` set, ticks run on a persistent
+    Batcher (pipeline captured once into a hipGraph, replayed every tick) and
+    admission also keeps each tick within its candidate capacity."""
+
+    def __init__(self, scenes: list[V.HashDagGpu], max_candidates: int | None = None, max_jobs: int = 8192,
+                 use_graph: bool = True):
         self.scenes = scenes
         self.caps = [(d._cap.node_words, d._cap.leaf_words) for d in scenes]
         self.ticks = 0
         self.tick_sizes = []
+        self.max_candidates = max_candidates
+        self.max_jobs = max_jobs
+        self.ex = V.Batcher(scenes, max_candidates, max_jobs, use_graph) if max_candidates else None
 
     def admit(self, pending: list[Request]) -> list[Request]:
         """Requests whose worst-case growth fits their scene's free pool words
-        (cumulative within the tick); the rest stay pending."""
+        (cumulative within the tick) and, with a Batcher, whose candidates fit
+        its capacity; the rest stay pending, in order."""
         used = {}
         out = []
+        cand = jobs = 0
+        blocked = set()
         for r in pending:
+            if r.scene in blocked:
+                continue  # keep per-scene arrival order: nothing overtakes a deferred request
             d = self.scenes[r.scene]
             if r.scene not in used:
                 st = d.stats()
@@ -68,17 +81,27 @@ class Server:
             nw, lw = growth_bound(d, r.brushes)
             cap_n, cap_l = self.caps[r.scene]
             u = used[r.scene]
-            if u[0] + nw <= cap_n and u[1] + lw <= cap_l:
+            c = V.batch_candidates(self.scenes, [(r.scene, r.brushes)]) if self.ex else 0
+            fits_pool = u[0] + nw <= cap_n and u[1] + lw <= cap_l
+            fits_exec = not self.ex or (cand + c <= self.max_candidates and jobs + len(r.brushes) <= self.max_jobs)
+            if fits_pool and fits_exec:
                 u[0] += nw
                 u[1] += lw
+                cand += c
+                jobs += len(r.brushes)
                 out.append(r)
+            else:
+                blocked.add(r.scene)
         return out
 
     def tick(self, reqs: list[Request], workspace=None) -> None:
         """One pipeline call for every request of the tick, in arrival order."""
         if not reqs:
             return
-        V.apply_batches([self.scenes[r.scene] for r in reqs], [r.brushes for r in reqs], workspace=workspace)
+        if self.ex:
+            self.ex.run([(r.scene, r.brushes) for r in reqs])
+        else:
+            V.apply_batches([self.scenes[r.scene] for r in reqs], [r.brushes for r in reqs], workspace=workspace)
         self.ticks += 1
         self.tick_sizes.append(len(reqs))
 
