@@ -55,13 +55,18 @@ def test_result_same_alone_and_in_mix(gpu):
         assert np.array_equal(alone[0].export(), mixed[s].export()), s
 
 
-def test_admission_defers_and_rejects_by_free_pool_space(gpu):
+def test_admission_by_free_pool_space_keeps_scene_order(gpu):
+    """A request whose worst-case growth does not fit its scene's free pool
+    words is deferred, later requests of that scene wait behind it, and other
+    scenes' requests are still admitted."""
     from voxel_amd.dag import HashDagGpu
     from voxel_amd.serving import Request, Server
-    d = HashDagGpu(5, True, node_words=4096, leaf_words=4096, node_slots=1024, leaf_slots=1024)
+    tiny = HashDagGpu(5, True, node_words=4096, leaf_words=4096, node_slots=1024, leaf_slots=1024)
+    roomy = HashDagGpu(5, True)
     big = O.gen_brushes(1, O.STREAM_BRUSH, 4, 128, 0)
     small = np.zeros(1, big.dtype)
     small[0] = (3, 3, 3, 1, 0, 0, 0, 9)
-    srv = Server([d])
-    admitted = srv.admit([Request(0, 0, big), Request(1, 0, small)])
-    assert [r.rid for r in admitted] == [1]
+    srv = Server([tiny, roomy])
+    admitted = srv.admit([Request(0, 0, big), Request(1, 0, small), Request(2, 1, small)])
+    assert [r.rid for r in admitted] == [2]
+    assert [r.rid for r in srv.admit([Request(3, 0, small)])] == [3]
