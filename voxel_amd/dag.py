@@ -106,15 +106,26 @@ def brush_voxel_ops(brushes: np.ndarray, world: int) -> int:
     return n.value
 
 
-def scene_capacity(inner_depth: int, materials: bool = True, headroom: float = 1.6) -> dict:
-    """Pool / table sizes for a heightfield scene of this depth plus edit
-    headroom (measured: the 16384^3 materials scene holds ~9.8e7 node words and
-    ~5.6e7 leaf words; each extra level multiplies both by ~4)."""
-    scale = 4.0 ** (inner_depth - 12)
-    node_words = int(min(2**31 - 64, max(1 << 20, 1.0e8 * scale * headroom + (1 << 22))))
-    leaf_words = int(min(2**31 - 64, max(1 << 20, (5.6e7 if materials else 1.2e7) * scale * headroom + (1 << 22))))
-    node_slots = 1 << max(16, (node_words // 2).bit_length())
-    leaf_slots = 1 << max(16, (leaf_words // (8 if materials else 1)).bit_length())
+# Upper estimates of a heightfield scene's interned nodes and leaves per inner
+# depth (measured on the 512^3-formula terrain scaled to the world: nodes grow
+# ~4x per level, leaves ~2-3x).
+_NODES_EST = {8: 8e4, 9: 3e5, 10: 1.2e6, 11: 4.5e6, 12: 1.6e7, 13: 6.4e7}
+_LEAVES_EST = {8: 1.6e5, 9: 5e5, 10: 1.5e6, 11: 3.2e6, 12: 5e6, 13: 8e6}
+
+
+def scene_capacity(inner_depth: int, materials: bool = True, headroom: float = 1.5,
+                   edit_leaves: float = 1e6) -> dict:
+    """Pool / table sizes for a heightfield scene of this depth plus room for
+    `edit_leaves` new leaves (and their ancestors) from edits."""
+    d = min(max(inner_depth, 8), 13)
+    shrink = 4.0 ** (min(inner_depth, 8) - 8)
+    nodes = _NODES_EST[d] * shrink * headroom + 2 * edit_leaves
+    leaves = _LEAVES_EST[d] * shrink * headroom + edit_leaves
+    lw = 16 if materials else 2
+    node_words = int(min(2**31 - 64, max(1 << 20, 7.5 * nodes)))
+    leaf_words = int(min(2**31 - 64, max(1 << 20, lw * leaves)))
+    node_slots = 1 << max(16, int(2.2 * nodes).bit_length())
+    leaf_slots = 1 << max(16, int(2.2 * leaves).bit_length())
     return dict(node_words=node_words, leaf_words=leaf_words, node_slots=node_slots, leaf_slots=leaf_slots)
 
 
