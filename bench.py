@@ -45,18 +45,27 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--cpu-brushes", type=int, default=1024, help="brushes in the CPU baseline sample")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the full-size check (CPU oracle finishes batch 0; canonical digests compared)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="finish with one 1 GiB read + 1 GiB write elementwise kernel (bitwise_not on int32), "
+                         "the known-size reference for hardware byte counters")
     return ap.parse_args()
 
 
 class CpuBaseline(threading.Thread):
     """CPU oracle (single-threaded restatement of the reference HashDAG) on a
     bounded sample: the same heightfield scene (build untimed), then the first
-    `n` brushes of batch 0 (timed). Runs beside the GPU legs."""
+    `n` brushes of batch 0 (timed). Runs beside the GPU legs. With `parity`
+    (the full batch 0) it then applies the rest of that batch, untimed, and
+    keeps the digest of its canonical export for the full-size check."""
 
-    def __init__(self, depth, seed, brushes):
+    def __init__(self, depth, seed, brushes, parity=None):
         super().__init__(daemon=True)
         self.depth, self.seed, self.brushes = depth, seed, brushes
+        self.parity = parity
         self.result = None
+        self.digest = None
         self.error = None
 
     def run(self):
@@ -77,6 +86,9 @@ class CpuBaseline(threading.Thread):
                            "sample": f"CPU oracle: {self.brushes} brushes of batch 0 on the same "
                                      f"{world}^3 materials heightfield ({ops} voxel ops in {dt:.2f} s; "
                                      f"scene build {setup:.1f} s untimed)"}
+            if self.parity is not None:
+                d.apply_brushes(self.parity[self.brushes:])
+                self.digest = O.digest(d.export())
         except Exception as e:  # reported, never fatal for the GPU figure
             self.error = repr(e)
 
@@ -258,9 +270,13 @@ def main():
     import voxel_amd.primitives as P
     from voxel_amd import dag as V
 
+    world = 4 << args.inner_depth
+    batches = [V.gen_brushes(args.seed, V.STREAM_BRUSH + (k << 8), args.brushes, world, 1)
+               for k in range(args.warmup + args.steps)]
+    check = not args.no_cpu and not args.no_parity and args.warmup > 0
     cpu = None
     if not args.no_cpu:
-        cpu = CpuBaseline(args.inner_depth, args.seed, args.cpu_brushes)
+        cpu = CpuBaseline(args.inner_depth, args.seed, args.cpu_brushes, batches[0] if check else None)
         cpu.start()
 
     torch.cuda.init()
@@ -274,15 +290,17 @@ def main():
     build_s = time.perf_counter() - t0
     st = scene.stats()
 
-    batches = [V.gen_brushes(args.seed, V.STREAM_BRUSH + (k << 8), args.brushes, world, 1)
-               for k in range(args.warmup + args.steps)]
     ops = [V.brush_voxel_ops(b, world) for b in batches]
     ws_bytes = max(scene.brush_workspace_size(b) for b in batches)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
 
+    gpu_digest = None
     for k in range(args.warmup):
         scene.apply_brushes(batches[k], workspace=ws, stream=stream)
+        if k == 0 and check:
+            from oracle import oracle as O  # the checker's digest, applied to the GPU scene's export
+            gpu_digest = O.digest(scene.export())
     torch.cuda.synchronize()
     scene.profile(True)
     torch.cuda.synchronize()
@@ -311,10 +329,24 @@ def main():
     torch.cuda.empty_cache()
     serving = None if args.no_serving else leg_serving(V, torch)
 
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        torch.bitwise_not(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+
     cpu_res = None
+    parity = None
     if cpu is not None:
         cpu.join()
         cpu_res = cpu.result if cpu.result else {"error": cpu.error}
+        if check:
+            parity = {"check": f"canonical DAG digest after batch 0 ({args.brushes} brushes) on the {world}^3 "
+                               "scene: GPU export vs CPU oracle",
+                      "gpu_digest": f"{gpu_digest:016x}" if gpu_digest is not None else None,
+                      "cpu_digest": f"{cpu.digest:016x}" if cpu.digest is not None else None,
+                      "match": gpu_digest is not None and gpu_digest == cpu.digest}
 
     line = {
         "metric": METRIC,
@@ -355,6 +387,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu_res,
+        "parity": parity,
     }
     print(json.dumps(line))
 

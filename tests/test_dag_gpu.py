@@ -233,3 +233,62 @@ def test_batcher_replays_match_sequential(gpu, use_graph):
         for k in range(3):
             assert np.array_equal(a[k].export(), b[k].export()), k
     assert ex.info()[1] == use_graph
+
+
+def _brush_truth(brushes, xyz, base):
+    """Material at each sample after the brushes are applied in order (last
+    writer wins; material 0 erases), evaluated straight from the shapes:
+    spheres |p - c|^2 <= r^2, boxes |p - c| <= h per axis."""
+    out = base.copy()
+    order = np.argsort(xyz[:, 0], kind="stable")
+    xs = xyz[order, 0]
+    p = xyz.astype(np.int64)
+    for b in brushes:
+        kind, hx = int(b["kind"]), int(b["hx"])
+        lo = np.searchsorted(xs, int(b["cx"]) - hx, "left")
+        hi = np.searchsorted(xs, int(b["cx"]) + hx, "right")
+        idx = order[lo:hi]
+        d = p[idx] - np.array([b["cx"], b["cy"], b["cz"]], np.int64)
+        if kind == 0:
+            inside = (d * d).sum(1) <= hx * hx
+        else:
+            inside = (np.abs(d) <= np.array([b["hx"], b["hy"], b["hz"]], np.int64)).all(1)
+        out[idx[inside]] = int(b["material"])
+    return out
+
+
+def test_full_size_edit_sampled_truth(gpu):
+    """16384^3 materials heightfield (inner depth 12, seed 3) + one batch of
+    4096 brushes (50% spheres r[8,64], 50% boxes h[4,48], 60/40 add/remove):
+    2^18 sampled voxels — half uniform over the world, half inside brush
+    bounding boxes — hold exactly the heightfield material overwritten by the
+    brushes in order."""
+    from voxel_amd.dag import STREAM_BRUSH, gen_brushes, terrain_for_world
+    depth = 12
+    world = 4 << depth
+    g = _dag(depth, True, node_words=1 << 28, leaf_words=1 << 29, node_slots=1 << 26, leaf_slots=1 << 26)
+    t = terrain_for_world(world, 3)
+    g.build_terrain(t)
+    brushes = gen_brushes(3, STREAM_BRUSH, 4096, world, 1)
+    g.apply_brushes(brushes)
+    rng = np.random.default_rng(12)
+    n = 1 << 17
+    uni = rng.integers(0, world, (n, 3))
+    pick = brushes[rng.integers(0, len(brushes), n)]
+    ext = np.where(pick["kind"] == 0, pick["hx"], 0)[:, None] + np.where(
+        pick["kind"] == 0, 0, 1)[:, None] * np.stack([pick["hx"], pick["hy"], pick["hz"]], 1)
+    ctr = np.stack([pick["cx"], pick["cy"], pick["cz"]], 1).astype(np.int64)
+    near = np.clip(ctr + rng.integers(-1 << 20, 1 << 20, (n, 3)) % (2 * ext + 1) - ext, 0, world - 1)
+    xyz = np.concatenate([uni, near]).astype(np.int64)
+    ot = O.terrain_for_world(world, 3)
+    heights = {}
+    base = np.zeros(len(xyz), np.int64)
+    for i, (x, y, z) in enumerate(xyz):
+        h = heights.get((x, z))
+        if h is None:
+            h = heights[(x, z)] = O.terrain_height(ot, int(x), int(z))
+        if y <= h:
+            base[i] = 1 + y % 255
+    want = _brush_truth(brushes, xyz, base)
+    got = g.get_voxels(_dev_xyz(xyz, gpu)).cpu().numpy()
+    assert np.array_equal(got.astype(np.int64), want)

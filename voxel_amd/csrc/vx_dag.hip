@@ -226,39 +226,114 @@ struct Record {
   uint32_t mat;
 };
 
-__device__ __forceinline__ uint64_t brick_mask(const vx_brush& b, int32_t X, int32_t Y, int32_t Z) {
-  uint64_t m = 0;
-  if (b.kind == 0) {
-    const int64_t r2 = int64_t(b.hx) * b.hx;
+// Sphere voxels of one brick, row by row: voxel (lx, ly, lz) is inside iff
+// dx^2 <= r^2 - dy^2 - dz^2. Every |d| <= r + 3 for a brick inside the
+// brush's bounding box, so for r < 2^14 all sums stay below 2^30 in int32.
+__device__ __forceinline__ uint64_t sphere_mask32(int32_t dx0, int32_t dy0, int32_t dz0, int32_t r2) {
+  int32_t qx[4], qy[4], qz[4];
 #pragma unroll
-    for (int lz = 0; lz < 4; ++lz) {
-      const int64_t dz = int64_t(Z + lz) - b.cz;
-#pragma unroll
-      for (int ly = 0; ly < 4; ++ly) {
-        const int64_t dy = int64_t(Y + ly) - b.cy;
-        const int64_t dyz = dy * dy + dz * dz;
-#pragma unroll
-        for (int lx = 0; lx < 4; ++lx) {
-          const int64_t dx = int64_t(X + lx) - b.cx;
-          if (dx * dx + dyz <= r2) m |= 1ull << (lx + 4 * ly + 16 * lz);
-        }
-      }
-    }
-  } else {
-    uint32_t xm = 0, ym = 0, zm = 0;
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      if (abs(X + l - b.cx) <= b.hx) xm |= 1u << l;
-      if (abs(Y + l - b.cy) <= b.hy) ym |= 1u << l;
-      if (abs(Z + l - b.cz) <= b.hz) zm |= 1u << l;
-    }
-#pragma unroll
-    for (int lz = 0; lz < 4; ++lz)
-#pragma unroll
-      for (int ly = 0; ly < 4; ++ly)
-        if (((zm >> lz) & 1u) && ((ym >> ly) & 1u)) m |= uint64_t(xm) << (4 * ly + 16 * lz);
+  for (int l = 0; l < 4; ++l) {
+    qx[l] = (dx0 + l) * (dx0 + l);
+    qy[l] = (dy0 + l) * (dy0 + l);
+    qz[l] = (dz0 + l) * (dz0 + l);
   }
+  // Outside iff rem - dx^2 < 0: collect sign bits (plain ALU work, no
+  // compare-to-mask round trips), then invert.
+  uint32_t out[2] = {0u, 0u};
+#pragma unroll
+  for (int lz = 0; lz < 4; ++lz)
+#pragma unroll
+    for (int ly = 0; ly < 4; ++ly) {
+      const int32_t rem = r2 - qy[ly] - qz[lz];
+#pragma unroll
+      for (int lx = 0; lx < 4; ++lx)
+        out[lz >> 1] |= (uint32_t(rem - qx[lx]) >> 31) << (lx + 4 * ly + 16 * (lz & 1));
+    }
+  return ~((uint64_t(out[1]) << 32) | out[0]);
+}
+
+// Same test in 64-bit arithmetic, for radii of 2^14 and more (out of line:
+// it is rare and would otherwise cost every caller its registers).
+__device__ __noinline__ uint64_t sphere_mask64(int64_t dx0, int64_t dy0, int64_t dz0, int64_t r2) {
+  uint64_t m = 0;
+  for (int lz = 0; lz < 4; ++lz)
+    for (int ly = 0; ly < 4; ++ly) {
+      const int64_t rem = r2 - (dy0 + ly) * (dy0 + ly) - (dz0 + lz) * (dz0 + lz);
+      for (int lx = 0; lx < 4; ++lx)
+        if ((dx0 + lx) * (dx0 + lx) <= rem) m |= 1ull << (lx + 4 * ly + 16 * lz);
+    }
   return m;
+}
+
+// Box voxels of one brick in 64-bit arithmetic, for half extents of 2^30 and
+// more (out of line, like sphere_mask64).
+__device__ __noinline__ uint64_t box_mask64(const vx_brush& b, int32_t X, int32_t Y, int32_t Z) {
+  uint32_t xm = 0, ym = 0, zm = 0;
+  for (int l = 0; l < 4; ++l) {
+    xm |= uint32_t(llabs(int64_t(X) + l - b.cx) <= b.hx) << l;
+    ym |= uint32_t(llabs(int64_t(Y) + l - b.cy) <= b.hy) << l;
+    zm |= uint32_t(llabs(int64_t(Z) + l - b.cz) <= b.hz) << l;
+  }
+  uint64_t m = 0;
+  for (int lz = 0; lz < 4; ++lz)
+    for (int ly = 0; ly < 4; ++ly)
+      if (((zm >> lz) & 1u) && ((ym >> ly) & 1u)) m |= uint64_t(xm) << (4 * ly + 16 * lz);
+  return m;
+}
+
+// Voxel mask of brick (X, Y, Z) (its minimum corner) under brush b; bit
+// lx + 4 ly + 16 lz (the reference's leaf bit order, detail/node.cuh:90-94).
+// The brick lies inside the brush's clipped bounding box, so every offset
+// from the centre is at most the half extent + 3: 32-bit arithmetic is exact
+// below the stated extents.
+__device__ __forceinline__ uint64_t brick_mask(const vx_brush& b, int32_t X, int32_t Y, int32_t Z) {
+  if (b.kind == 0) {
+    if (b.hx < (1 << 14)) return sphere_mask32(X - b.cx, Y - b.cy, Z - b.cz, b.hx * b.hx);
+    return sphere_mask64(int64_t(X) - b.cx, int64_t(Y) - b.cy, int64_t(Z) - b.cz, int64_t(b.hx) * b.hx);
+  }
+  if (b.hx >= (1 << 30) || b.hy >= (1 << 30) || b.hz >= (1 << 30)) return box_mask64(b, X, Y, Z);
+  const int32_t dx = X - b.cx, dy = Y - b.cy, dz = Z - b.cz;
+  uint32_t xm = 0, ym = 0, zm = 0;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    xm |= uint32_t(abs(dx + l) <= b.hx) << l;
+    ym |= uint32_t(abs(dy + l) <= b.hy) << l;
+    zm |= uint32_t(abs(dz + l) <= b.hz) << l;
+  }
+  // Spread the z and y bits to one nibble per (ly, lz) row, then replicate
+  // the x row into every selected nibble.
+  uint32_t rows = 0;
+#pragma unroll
+  for (int lz = 0; lz < 4; ++lz) rows |= (((zm >> lz) & 1u) * ym) << (4 * lz);
+  uint64_t m = 0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) m |= uint64_t(((rows >> r) & 1u) * xm) << (4 * r);
+  return m;
+}
+
+// Whether brick (X, Y, Z) of a job holds any voxel of its brush, without the
+// mask. Every brick of a box's clipped brick range overlaps the box on every
+// axis. For a sphere, the brick's lattice point nearest to the centre
+// (clamped axis by axis) minimises each squared term, so the brick is touched
+// iff that point is inside.
+__device__ __noinline__ bool sphere_touched64(const vx_brush& b, int32_t X, int32_t Y, int32_t Z) {
+  const int64_t c[3] = {b.cx, b.cy, b.cz};
+  const int64_t lo[3] = {X, Y, Z};
+  int64_t d2 = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int64_t p = c[a] < lo[a] ? lo[a] : (c[a] > lo[a] + 3 ? lo[a] + 3 : c[a]);
+    d2 += (p - c[a]) * (p - c[a]);
+  }
+  return d2 <= int64_t(b.hx) * b.hx;
+}
+
+__device__ __forceinline__ bool brick_touched(const vx_brush& b, int32_t X, int32_t Y, int32_t Z) {
+  if (b.kind != 0) return true;
+  if (b.hx >= (1 << 14)) return sphere_touched64(b, X, Y, Z);
+  const int32_t dx = min(max(X - b.cx, 0), X + 3 - b.cx) ;
+  const int32_t dy = min(max(Y - b.cy, 0), Y + 3 - b.cy);
+  const int32_t dz = min(max(Z - b.cz, 0), Z + 3 - b.cz);
+  return dx * dx + dy * dy + dz * dz <= b.hx * b.hx;
 }
 
 struct RecordOut {
@@ -269,27 +344,76 @@ struct RecordOut {
   uint32_t* n_rec;
 };
 
-struct BrushLoad {
-  using Payload = Record;
+// Brush candidates: candidate i is one brick of one job's clipped brick box.
+struct BrushCands {
   const BrushJob* jobs;
   const uint32_t* n_jobs;  // device count
   bool materials;
   Codec codec;
-  __device__ uint32_t operator()(uint32_t i, Record& rec) const {
-    uint32_t lo = 0, hi = *n_jobs;  // last job with start <= i
+  // The lanes of a wave hold consecutive candidates: one wave-uniform binary
+  // search (scalar loads) for the lowest active lane, then each lane steps
+  // forward over the few jobs that start inside its wave's window.
+  __device__ __forceinline__ uint32_t job_of(uint32_t i) const {
+    const uint32_t nj = *n_jobs;
+    const uint32_t i0 = __builtin_amdgcn_readfirstlane(i);
+    uint32_t lo = 0, hi = nj;  // last job with start <= i0
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (jobs[mid].start <= i) lo = mid;
+      if (jobs[mid].start <= i0) lo = mid;
       else hi = mid;
     }
-    const BrushJob& j = jobs[lo];
+    while (lo + 1 < nj && jobs[lo + 1].start <= i) ++lo;
+    return lo;
+  }
+  __device__ __forceinline__ void brick_of(const BrushJob& j, uint32_t i, uint32_t& gx, uint32_t& gy,
+                                           uint32_t& gz) const {
     const uint32_t local = i - j.start;
-    const uint32_t bx = local % j.nx, by = (local / j.nx) % j.ny, bz = local / (j.nx * j.ny);
-    const uint32_t gx = j.bx0 + bx, gy = j.by0 + by, gz = j.bz0 + bz;
-    rec.mask = brick_mask(j.b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4));
-    rec.key = (uint64_t(j.scene) << codec.shift) | morton3(gx, gy, gz);
-    rec.mat = materials ? j.b.material : (j.b.material ? 1u : 0u);
-    return rec.mask != 0;
+    const uint32_t row = local / j.nx;
+    gx = j.bx0 + (local - row * j.nx);
+    gy = j.by0 + row % j.ny;
+    gz = j.bz0 + row / j.ny;
+  }
+};
+
+// A touched brick travelling from the Load half to the Store half: its job
+// and brick coordinates (each below 2^15 at inner depth <= 15).
+struct BrickRef {
+  uint32_t job;
+  uint32_t gxy;  // gx | gy << 16
+  uint32_t gz;
+};
+
+// Load half: the touched test only.
+struct BrushLoad {
+  using Payload = BrickRef;
+  BrushCands c;
+  __device__ uint32_t operator()(uint32_t i, BrickRef& r) const {
+    r.job = c.job_of(i);
+    const BrushJob& j = c.jobs[r.job];
+    uint32_t gx, gy, gz;
+    c.brick_of(j, i, gx, gy, gz);
+    r.gxy = gx | (gy << 16);
+    r.gz = gz;
+    return brick_touched(j.b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4)) ? 1u : 0u;
+  }
+};
+
+// Store half: mask, key and material of each touched brick, at its compacted
+// position (brush order is preserved).
+struct BrushStore {
+  BrushCands c;
+  RecordOut out;
+  const uint32_t* n_ptr;  // candidates (device count)
+  __device__ void operator()(uint32_t i, const BrickRef& r, uint32_t v, uint32_t excl, uint32_t incl) const {
+    if (v) {
+      const vx_brush& b = c.jobs[r.job].b;
+      const uint32_t gx = r.gxy & 0xffffu, gy = r.gxy >> 16, gz = r.gz;
+      out.key[excl] = (uint64_t(c.jobs[r.job].scene) << c.codec.shift) | morton3(gx, gy, gz);
+      out.val[excl] = excl;
+      out.mask[excl] = brick_mask(b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4));
+      out.mat[excl] = c.materials ? b.material : (b.material ? 1u : 0u);
+    }
+    if (i == *n_ptr - 1) *out.n_rec = incl;
   }
 };
 
@@ -751,6 +875,15 @@ vx_status upload(EditWs& w, const std::vector<Dev>& devs, const std::vector<Brus
   return VX_OK;
 }
 
+// The record writer that matches each candidate kind.
+inline RecordOut record_out(EditWs& w) { return RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0]}; }
+inline RecordStore store_for(const VoxelLoad&, EditWs& w) {
+  return RecordStore{record_out(w), &w.counts[kCntCandidates]};
+}
+inline BrushStore store_for(const BrushLoad& l, EditWs& w) {
+  return BrushStore{l.c, record_out(w), &w.counts[kCntCandidates]};
+}
+
 // All device work of an edit (no host synchronisation, no memset nodes), so a
 // stream capture of it replays as a hipGraph. Grids are sized by the
 // capacities in `w`; the actual counts are read on the device.
@@ -763,10 +896,8 @@ vx_status device_pipeline(vx_dag* prof_dag, const Plan& p, const Scenes& sc, Edi
   begin_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes);
   if ((st = last_launch()) != VX_OK) return st;
   mark(prof_dag, 0, s);
-  if ((st = scan::fused_dev<scan::kSum>(&w.counts[kCntCandidates], cand_cap, load,
-                                        RecordStore{RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0]},
-                                                    &w.counts[kCntCandidates]},
-                                        w.fused_ws, s)) != VX_OK)
+  if ((st = scan::fused_dev<scan::kSum>(&w.counts[kCntCandidates], cand_cap, load, store_for(load, w), w.fused_ws,
+                                        s)) != VX_OK)
     return st;
   return rebuild(prof_dag, p, sc, w, cand_cap, s);
 }
@@ -951,7 +1082,8 @@ extern "C" vx_status vx_dag_apply_brushes(vx_dag* g, const vx_brush* b, uint32_t
   const Plan p = make_plan({g});
   EditWs w = carve(ws, p, cand, uint32_t(jobs.size()), cand);
   if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
-  return run_edit({g}, g, p, w, &jobs, uint32_t(cand), BrushLoad{w.jobs, &w.counts[kCntJobs], g->materials, p.codec},
+  return run_edit({g}, g, p, w, &jobs, uint32_t(cand),
+                  BrushLoad{BrushCands{w.jobs, &w.counts[kCntJobs], g->materials, p.codec}},
                   static_cast<hipStream_t>(stream));
 }
 
@@ -1005,7 +1137,7 @@ extern "C" vx_status vx_dag_apply_batches(vx_dag* const* dags, const vx_brush* c
   EditWs w = carve(ws, p, cand, uint32_t(jobs.size()), cand);
   if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
   return run_edit(scenes, scenes[0], p, w, &jobs, uint32_t(cand),
-                  BrushLoad{w.jobs, &w.counts[kCntJobs], scenes[0]->materials, p.codec},
+                  BrushLoad{BrushCands{w.jobs, &w.counts[kCntJobs], scenes[0]->materials, p.codec}},
                   static_cast<hipStream_t>(stream));
 }
 
@@ -1262,7 +1394,7 @@ extern "C" vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batc
   vx_status st;
   if ((st = upload(b->w, b->devs, &b->jobs, uint32_t(cand), b->staging, s)) != VX_OK) return st;
   const Scenes sc{b->plan.n_scenes > 1 ? b->w.scenes : nullptr, nullptr, b->plan.codec, b->devs[0]};
-  const BrushLoad load{b->w.jobs, &b->w.counts[kCntJobs], b->scenes[0]->materials, b->plan.codec};
+  const BrushLoad load{BrushCands{b->w.jobs, &b->w.counts[kCntJobs], b->scenes[0]->materials, b->plan.codec}};
   if (b->use_graph) {
     if (!b->exec) {
       if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) return VX_DEVICE_ERROR;
