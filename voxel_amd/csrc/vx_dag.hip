@@ -88,6 +88,55 @@ __device__ __forceinline__ Table table_of(const Dev& d, bool leaves) {
                 : Table{d.node_slots, d.node_mask, d.node_pool, d.node_cap, d.ctl + kCtlNodeUsed, d.ctl + kCtlNodeBase};
 }
 
+// Item words in registers, every load issued before any use: a word-by-word
+// copy or compare loop would wait one memory latency per word. kMax is the
+// widest item of the set (9 for nodes, the leaf width for leaves); aligned
+// full-width items move as 16-byte vectors.
+template <uint32_t kMax>
+__device__ __forceinline__ void load_item(const uint32_t* p, uint32_t n, uint32_t (&w)[kMax]) {
+  if constexpr (kMax % 4 == 0) {
+    if (n == kMax && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
+#pragma unroll
+      for (uint32_t k = 0; k < kMax; k += 4) {
+        const uint4 v = *reinterpret_cast<const uint4*>(p + k);
+        w[k] = v.x;
+        w[k + 1] = v.y;
+        w[k + 2] = v.z;
+        w[k + 3] = v.w;
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kMax; ++k) w[k] = k < n ? p[k] : 0u;
+}
+
+template <uint32_t kMax>
+__device__ __forceinline__ bool item_equal(const uint32_t (&a)[kMax], const uint32_t* p, uint32_t n) {
+  uint32_t b[kMax];
+  load_item<kMax>(p, n, b);
+  uint32_t diff = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kMax; ++k) diff |= a[k] ^ b[k];
+  return diff == 0;
+}
+
+template <uint32_t kMax>
+__device__ __forceinline__ void store_item(uint32_t* p, uint32_t n, const uint32_t (&w)[kMax]) {
+  if constexpr (kMax % 4 == 0) {
+    if (n == kMax && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
+#pragma unroll
+      for (uint32_t k = 0; k < kMax; k += 4)
+        *reinterpret_cast<uint4*>(p + k) = make_uint4(w[k], w[k + 1], w[k + 2], w[k + 3]);
+      return;
+    }
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kMax; ++k)
+    if (k < n) p[k] = w[k];
+}
+
+template <uint32_t kMax>
 __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, const uint32_t* n_ptr, uint64_t* res,
                              Scenes sc, uint32_t n_scenes, bool leaves) {
   if (blockIdx.x == 0)
@@ -102,6 +151,8 @@ __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, c
     const Table t = table_of(d, leaves);
     const uint32_t* it = set.items + size_t(i) * set.stride;
     const uint32_t w = set.width(it);
+    uint32_t mine[kMax];
+    load_item<kMax>(it, w, mine);
     const uint64_t h = hashes[i];
     const uint32_t fp = uint32_t(h >> 32);
     uint64_t idx = h & t.mask;
@@ -114,9 +165,9 @@ __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, c
       // below then returns the slot's real value.
       uint64_t s = t.slots[idx];
       if (s == 0) {
-        const uint64_t mine = (uint64_t(fp) << 32) | kTag | i;
+        const uint64_t tagged = (uint64_t(fp) << 32) | kTag | i;
         const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&t.slots[idx]), 0ull,
-                                        static_cast<unsigned long long>(mine));
+                                        static_cast<unsigned long long>(tagged));
         if (prev == 0) {
           out = res_make(kResRep, uint32_t(idx));
           done = true;
@@ -128,13 +179,16 @@ __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, c
         const uint32_t v = uint32_t(s);
         if (v & kTag) {
           const uint32_t j = v & ~kTag;
-          if (words_equal(it, set.items + size_t(j) * set.stride, w)) {
+          if (item_equal<kMax>(mine, set.items + size_t(j) * set.stride, w)) {
             out = res_make(kResDup, j);
             done = true;
             break;
           }
         } else if (v >= kPoolStart) {
-          if (words_equal(it, t.pool + v, w)) {
+          // An item of width w that would run past the pool's end cannot be
+          // the one stored at v (equal headers mean equal widths), and must
+          // not be read.
+          if (uint64_t(v) + w <= t.cap && item_equal<kMax>(mine, t.pool + v, w)) {
             out = res_make(kResFinal, v);
             done = true;
             break;
@@ -148,64 +202,119 @@ __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, c
   }
 }
 
-// Commit: segmented scan (one segment per scene) of representative widths.
+// Commit, part 1: segmented scan (one segment per scene) of representative
+// widths. For each representative it stores the final slot value
+// {fingerprint, pool offset} in hashes[i] (the fingerprint rides in the scan
+// payload, so nothing is read back); a representative that does not fit is
+// tombstoned here and resolves to EMPTY. Part 2 moves the items.
+// Everything the store half needs is loaded by the load half (whose loads
+// are all in flight together), so the store half only writes.
+struct CommitPayload {
+  uint64_t r;
+  uint32_t fp;
+  uint32_t base;  // the scene's pool words in use before this level
+  bool last;      // last entry of its scene's segment
+};
+
 struct CommitLoad {
-  using Payload = uint64_t;
+  using Payload = CommitPayload;
   const uint64_t* res;
+  const uint64_t* hashes;
+  const uint32_t* n_ptr;
   ItemSet set;
   Scenes sc;
-  __device__ uint64_t operator()(uint32_t i, uint64_t& r) const {
-    r = res[i];
-    const uint32_t w = res_kind(r) == kResRep ? set.width(set.items + size_t(i) * set.stride) : 0u;
-    const bool head = i == 0 || sc.id(i - 1) != sc.id(i);
+  bool leaves;
+  __device__ uint64_t operator()(uint32_t i, CommitPayload& p) const {
+    p.r = res[i];
+    const bool rep = res_kind(p.r) == kResRep;
+    p.fp = rep ? uint32_t(hashes[i] >> 32) : 0u;
+    const uint32_t w = rep ? set.width(set.items + size_t(i) * set.stride) : 0u;
+    const uint32_t id = sc.id(i);
+    p.base = *table_of(sc.of(i), leaves).base;
+    p.last = i == *n_ptr - 1 || sc.id(i + 1) != id;
+    const bool head = i == 0 || sc.id(i - 1) != id;
     return scan::SegSumOp::make(w, head);
   }
 };
 
 struct CommitStore {
-  ItemSet set;
   uint64_t* res;
+  uint64_t* slot_vals;  // the hashes array, reused
   Scenes sc;
   bool leaves;
-  const uint32_t* n_ptr;
-  __device__ void operator()(uint32_t i, const uint64_t& r, uint64_t v, uint64_t, uint64_t incl) const {
+  __device__ void operator()(uint32_t i, const CommitPayload& p, uint64_t v, uint64_t, uint64_t incl, uint32_t) const {
     const Dev& d = sc.of(i);
     const Table t = table_of(d, leaves);
-    const uint32_t base = *t.base;
+    const uint32_t base = p.base;
     const uint32_t w = uint32_t(v), seg_incl = uint32_t(incl);
     if (w) {
-      uint64_t* slot = &t.slots[uint32_t(r)];
-      const uint64_t fp_hi = ld_agent(slot) & 0xFFFFFFFF00000000ull;
+      const uint64_t fp_hi = uint64_t(p.fp) << 32;
       const uint64_t off = uint64_t(base) + seg_incl - w;
       if (off + w > t.cap || off >= kTag) {
         set_status(d.ctl, kStatusOverflow);
-        st_agent(slot, fp_hi | kTomb);
+        st_agent(&t.slots[uint32_t(p.r)], fp_hi | kTomb);
         res[i] = res_make(kResFinal, kEmpty);
       } else {
-        const uint32_t* it = set.items + size_t(i) * set.stride;
-        for (uint32_t k = 0; k < w; ++k) t.pool[off + k] = it[k];
-        st_agent(slot, fp_hi | uint32_t(off));
-        res[i] = res_make(kResFinal, uint32_t(off));
+        slot_vals[i] = fp_hi | uint32_t(off);
       }
     }
-    const uint32_t n = *n_ptr;
-    if (i == n - 1 || sc.id(i + 1) != sc.id(i)) {
+    if (p.last) {
       const uint64_t used = uint64_t(base) + seg_incl;
       *t.used = used > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(used);
     }
   }
 };
 
-vx_status intern_scenes(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet set, const uint64_t* hashes,
-                        const uint32_t* n_ptr, uint64_t n_ub, uint64_t* res, void* fused_ws, hipStream_t s) {
-  probe_kernel<<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, n_scenes, leaves);
-  vx_status st = last_launch();
-  if (st != VX_OK) return st;
-  return scan::fused_dev_m<scan::SegSumOp>(n_ptr, n_ub, CommitLoad{res, set, sc},
-                                           CommitStore{set, res, sc, leaves, n_ptr}, fused_ws, s);
+// Commit, part 2: one thread per entry; each representative copies its item
+// to the pool, publishes its slot and becomes FINAL. Independent entries, so
+// the item loads of many entries are in flight at once.
+template <uint32_t kMax>
+__global__ void place_kernel(ItemSet set, const uint64_t* __restrict__ slot_vals, const uint32_t* n_ptr,
+                             uint64_t* res, Scenes sc, bool leaves) {
+  const uint32_t n = *n_ptr;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t r = res[i];
+    if (res_kind(r) != kResRep) continue;
+    const Table t = table_of(sc.of(i), leaves);
+    const uint64_t sv = slot_vals[i];
+    const uint32_t off = uint32_t(sv);
+    const uint32_t* it = set.items + size_t(i) * set.stride;
+    const uint32_t w = set.width(it);
+    uint32_t words[kMax];
+    load_item<kMax>(it, w, words);
+    store_item<kMax>(t.pool + off, w, words);
+    st_agent(&t.slots[uint32_t(r)], sv);
+    res[i] = res_make(kResFinal, off);
+  }
 }
 
-vx_status intern(const Dev& d, bool leaves, ItemSet set, const uint64_t* hashes, const uint32_t* n_ptr, uint64_t n_ub,
+template <uint32_t kMax>
+vx_status intern_width(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet set, uint64_t* hashes,
+                       const uint32_t* n_ptr, uint64_t n_ub, uint64_t* res, void* fused_ws, hipStream_t s,
+                       bool clear_ws) {
+  probe_kernel<kMax><<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, n_scenes, leaves);
+  vx_status st = last_launch();
+  if (st != VX_OK) return st;
+  if ((st = scan::fused_dev_m<scan::SegSumOp>(n_ptr, n_ub, CommitLoad{res, hashes, n_ptr, set, sc, leaves},
+                                              CommitStore{res, hashes, sc, leaves}, fused_ws, s, clear_ws)) != VX_OK)
+    return st;
+  place_kernel<kMax><<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, leaves);
+  return last_launch();
+}
+
+vx_status intern_scenes(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet set, uint64_t* hashes,
+                        const uint32_t* n_ptr, uint64_t n_ub, uint64_t* res, void* fused_ws, hipStream_t s,
+                        bool clear_ws = true) {
+  if (!leaves)
+    return intern_width<kNodeStride>(sc, n_scenes, leaves, set, hashes, n_ptr, n_ub, res, fused_ws, s, clear_ws);
+  if (set.fixed_width == 16)
+    return intern_width<16>(sc, n_scenes, leaves, set, hashes, n_ptr, n_ub, res, fused_ws, s, clear_ws);
+  if (set.fixed_width == 2)
+    return intern_width<2>(sc, n_scenes, leaves, set, hashes, n_ptr, n_ub, res, fused_ws, s, clear_ws);
+  return VX_INVALID_VALUE;
+}
+
+vx_status intern(const Dev& d, bool leaves, ItemSet set, uint64_t* hashes, const uint32_t* n_ptr, uint64_t n_ub,
                  uint64_t* res, void* fused_ws, hipStream_t s) {
   const Scenes sc{nullptr, nullptr, Codec{63}, d};
   return intern_scenes(sc, 1, leaves, set, hashes, n_ptr, n_ub, res, fused_ws, s);
@@ -403,8 +512,8 @@ struct BrushLoad {
 struct BrushStore {
   BrushCands c;
   RecordOut out;
-  const uint32_t* n_ptr;  // candidates (device count)
-  __device__ void operator()(uint32_t i, const BrickRef& r, uint32_t v, uint32_t excl, uint32_t incl) const {
+  __device__ void operator()(uint32_t i, const BrickRef& r, uint32_t v, uint32_t excl, uint32_t incl,
+                             uint32_t n) const {
     if (v) {
       const vx_brush& b = c.jobs[r.job].b;
       const uint32_t gx = r.gxy & 0xffffu, gy = r.gxy >> 16, gz = r.gz;
@@ -413,7 +522,7 @@ struct BrushStore {
       out.mask[excl] = brick_mask(b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4));
       out.mat[excl] = c.materials ? b.material : (b.material ? 1u : 0u);
     }
-    if (i == *n_ptr - 1) *out.n_rec = incl;
+    if (i == n - 1) *out.n_rec = incl;
   }
 };
 
@@ -434,15 +543,15 @@ struct VoxelLoad {
 
 struct RecordStore {
   RecordOut out;
-  const uint32_t* n_ptr;  // candidates (device count)
-  __device__ void operator()(uint32_t i, const Record& rec, uint32_t v, uint32_t excl, uint32_t incl) const {
+  __device__ void operator()(uint32_t i, const Record& rec, uint32_t v, uint32_t excl, uint32_t incl,
+                             uint32_t n) const {
     if (v) {
       out.key[excl] = rec.key;
       out.val[excl] = excl;
       out.mask[excl] = rec.mask;
       out.mat[excl] = rec.mat;
     }
-    if (i == *n_ptr - 1) *out.n_rec = incl;
+    if (i == n - 1) *out.n_rec = incl;
   }
 };
 
@@ -461,13 +570,12 @@ struct RunStore {
   uint64_t* keys0;
   uint32_t* first0;
   uint32_t* n0;
-  const uint32_t* n_ptr;
-  __device__ void operator()(uint32_t i, const uint64_t& k, uint32_t f, uint32_t excl, uint32_t incl) const {
+  __device__ void operator()(uint32_t i, const uint64_t& k, uint32_t f, uint32_t excl, uint32_t incl, uint32_t n) const {
     if (f) {
       keys0[excl] = k;
       first0[excl] = i;
     }
-    if (i == *n_ptr - 1) {
+    if (i == n - 1) {
       *n0 = incl;
       first0[incl] = i + 1;
     }
@@ -503,15 +611,15 @@ struct UpStore {
   uint32_t* parent;
   uint32_t* up_n;
   uint32_t* root_pos;  // per scene: index of its root entry at its root height
-  const uint32_t* n_ptr;
-  __device__ void operator()(uint32_t i, const UpPayload& p, uint32_t f, uint32_t excl, uint32_t incl) const {
+  __device__ void operator()(uint32_t i, const UpPayload& p, uint32_t f, uint32_t excl, uint32_t incl,
+                             uint32_t n) const {
     parent[i] = p.inc ? incl - 1 : kNone;
     if (!p.inc) root_pos[p.scene] = i;
     if (f) {
       up_keys[excl] = codec.parent(p.key);
       up_first[excl] = i;
     }
-    if (i == *n_ptr - 1) {
+    if (i == n - 1) {
       *up_n = incl;
       up_first[incl] = i + 1;
     }
@@ -656,7 +764,14 @@ struct EditWs {
   Heights hs;
   uint32_t* items;
   uint64_t* hashes;
-  void* fused_ws;
+  // Workspaces of the pipeline's fused scans, one per scan so that a single
+  // fill at the start of an edit clears them all.
+  uint32_t* fused_all;
+  size_t fused_bytes;
+  void* rec_ws;
+  void* run_ws;
+  void* up_ws[kMaxHeights + 1];
+  void* commit_ws[kMaxHeights + 1];
   BrushJob* jobs;
   Dev* scenes;
   size_t bytes;
@@ -714,8 +829,18 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
   }
   w.items = c.take<uint32_t>(max_items);
   w.hashes = c.take<uint64_t>(max_cap);
-  w.fused_ws = c.take<char>(std::max(scan::fused_dev_workspace_bytes(std::max(max_cap, R)),
-                                     scan::fused_workspace_bytes(std::max<uint64_t>(candidates, 1))));
+  {
+    char* start = c.take<char>(0);
+    w.rec_ws = c.take<char>(scan::fused_dev_workspace_bytes(std::max<uint64_t>(candidates, 1)));
+    w.run_ws = c.take<char>(scan::fused_dev_workspace_bytes(R));
+    for (uint32_t h = 0; h <= H; ++h) {
+      w.up_ws[h] = c.take<char>(scan::fused_dev_workspace_bytes(w.hs.cap[h]));
+      w.commit_ws[h] = c.take<char>(scan::fused_dev_workspace_bytes(w.hs.cap[h]));
+    }
+    char* end = c.take<char>(0);
+    w.fused_all = reinterpret_cast<uint32_t*>(start);
+    w.fused_bytes = size_t(end - start);
+  }
   w.jobs = c.take<BrushJob>(std::max<uint32_t>(1, n_jobs));
   w.scenes = c.take<Dev>(p.n_scenes);
   w.bytes = c.size();
@@ -741,16 +866,16 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
   mark(g, 2, s);
   const uint64_t* skey = in_alt ? w.rkey_alt : w.rkey;
   const uint32_t* sval = in_alt ? w.rval_alt : w.rval;
-  if ((st = scan::fused_dev<scan::kSum>(&cnt[0], r_ub, RunLoad{skey}, RunStore{L.keys[0], L.first[0], &cnt[1], &cnt[0]},
-                                        w.fused_ws, s)) != VX_OK)
+  if ((st = scan::fused_dev<scan::kSum>(&cnt[0], r_ub, RunLoad{skey}, RunStore{L.keys[0], L.first[0], &cnt[1]},
+                                        w.run_ws, s, false)) != VX_OK)
     return st;
   for (uint32_t h = 0; h <= H; ++h) {
     Scenes sh = sc;
     sh.keys = L.keys[h];
     if ((st = scan::fused_dev<scan::kSum>(&cnt[1 + h], L.cap[h], UpLoad{L.keys[h], sh, h},
                                           UpStore{p.codec, L.keys[h + 1], L.first[h + 1], L.parent[h], &cnt[2 + h],
-                                                  w.root_pos, &cnt[1 + h]},
-                                          w.fused_ws, s)) != VX_OK)
+                                                  w.root_pos},
+                                          w.up_ws[h], s, false)) != VX_OK)
       return st;
   }
   mark(g, 3, s);
@@ -772,7 +897,7 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
                                                                w.rmat, w.items, w.hashes, L.res[0]);
     if ((st = last_launch()) != VX_OK) return st;
     if ((st = intern_scenes(sh, p.n_scenes, true, ItemSet{w.items, p.leaf_words, p.leaf_words}, w.hashes, &cnt[1],
-                            L.cap[0], L.res[0], w.fused_ws, s)) != VX_OK)
+                            L.cap[0], L.res[0], w.commit_ws[0], s, false)) != VX_OK)
       return st;
   }
   mark(g, 5, s);
@@ -783,7 +908,7 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
                                                          L.parent[h - 1], L.res[h - 1], w.items, w.hashes, L.res[h]);
     if ((st = last_launch()) != VX_OK) return st;
     if ((st = intern_scenes(sh, p.n_scenes, false, ItemSet{w.items, kNodeStride, 0}, w.hashes, &cnt[1 + h], L.cap[h],
-                            L.res[h], w.fused_ws, s)) != VX_OK)
+                            L.res[h], w.commit_ws[h], s, false)) != VX_OK)
       return st;
   }
   mark(g, 6, s);
@@ -878,10 +1003,10 @@ vx_status upload(EditWs& w, const std::vector<Dev>& devs, const std::vector<Brus
 // The record writer that matches each candidate kind.
 inline RecordOut record_out(EditWs& w) { return RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0]}; }
 inline RecordStore store_for(const VoxelLoad&, EditWs& w) {
-  return RecordStore{record_out(w), &w.counts[kCntCandidates]};
+  return RecordStore{record_out(w)};
 }
 inline BrushStore store_for(const BrushLoad& l, EditWs& w) {
-  return BrushStore{l.c, record_out(w), &w.counts[kCntCandidates]};
+  return BrushStore{l.c, record_out(w)};
 }
 
 // All device work of an edit (no host synchronisation, no memset nodes), so a
@@ -893,11 +1018,12 @@ vx_status device_pipeline(vx_dag* prof_dag, const Plan& p, const Scenes& sc, Edi
   vx_status st;
   if ((st = fill_u32(w.counts, kCntCandidates, 0, s)) != VX_OK) return st;
   if ((st = fill_u32(w.root_pos, p.n_scenes, kNone, s)) != VX_OK) return st;
+  if ((st = fill_u32(w.fused_all, w.fused_bytes / 4, 0, s)) != VX_OK) return st;
   begin_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes);
   if ((st = last_launch()) != VX_OK) return st;
   mark(prof_dag, 0, s);
-  if ((st = scan::fused_dev<scan::kSum>(&w.counts[kCntCandidates], cand_cap, load, store_for(load, w), w.fused_ws,
-                                        s)) != VX_OK)
+  if ((st = scan::fused_dev<scan::kSum>(&w.counts[kCntCandidates], cand_cap, load, store_for(load, w), w.rec_ws, s,
+                                        false)) != VX_OK)
     return st;
   return rebuild(prof_dag, p, sc, w, cand_cap, s);
 }

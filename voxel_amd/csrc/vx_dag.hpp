@@ -173,9 +173,9 @@ struct ItemSet {
 
 // Probe + commit one level's items into the node (leaves=false) or leaf
 // table. res[i] must be FINAL (sentinel) or PENDING (hashes[i] set) on entry;
-// every entry is FINAL or DUP afterwards. fused_ws holds
+// every entry is FINAL or DUP afterwards, and hashes[] is overwritten. fused_ws holds
 // scan::fused_dev_workspace_bytes(n_ub).
-vx_status intern(const Dev& d, bool leaves, ItemSet set, const uint64_t* hashes, const uint32_t* n_ptr, uint64_t n_ub,
+vx_status intern(const Dev& d, bool leaves, ItemSet set, uint64_t* hashes, const uint32_t* n_ptr, uint64_t n_ub,
                  uint64_t* res, void* fused_ws, hipStream_t s);
 
 }  // namespace dag

@@ -248,7 +248,7 @@ template <Op OP>
 struct InplaceStoreOp {
   uint32_t* p;
   bool inclusive;
-  __device__ void operator()(uint32_t i, const NoPayload&, uint32_t, uint32_t excl, uint32_t incl) const {
+  __device__ void operator()(uint32_t i, const NoPayload&, uint32_t, uint32_t excl, uint32_t incl, uint32_t) const {
     p[i] = inclusive ? incl : excl;
   }
 };
@@ -262,8 +262,7 @@ struct CompactStore {
   const uint32_t* in;
   uint32_t* out;
   uint32_t* count;
-  uint32_t n;
-  __device__ void operator()(uint32_t i, const NoPayload&, uint32_t v, uint32_t excl, uint32_t incl) const {
+  __device__ void operator()(uint32_t i, const NoPayload&, uint32_t v, uint32_t excl, uint32_t incl, uint32_t n) const {
     if (v) out[excl] = in[i];
     if (i == n - 1) *count = incl;
   }
@@ -279,7 +278,7 @@ vx_status generic_inplace(uint32_t* data, uint32_t n, bool inclusive, Op op, voi
 vx_status compact(const uint32_t* in, const uint32_t* flags, uint32_t n, uint32_t* out, uint32_t* d_count, void* ws,
                   hipStream_t s) {
   if (n == 0) return fill_u32(d_count, 1, 0, s);
-  return fused<kSum>(n, FlagLoad{flags}, CompactStore{in, out, d_count, n}, ws, s);
+  return fused<kSum>(n, FlagLoad{flags}, CompactStore{in, out, d_count}, ws, s);
 }
 
 }  // namespace scan

@@ -1,5 +1,5 @@
 // Fused single-pass scan: Load(i, payload) -> scan -> Store(i, payload,
-// value, exclusive, inclusive), the building block for stream compaction and
+// value, exclusive, inclusive, n), the building block for stream compaction and
 // for the DAG pipeline's run/level compactions. Counterpart of the reference's
 // decoupled_lookback_fused_kernel (cuda/scan/fused.cuh:102-159), but with
 // dynamic tile ids, a wave-wide look-back window, a per-element payload that
@@ -181,7 +181,7 @@ __device__ __forceinline__ void fused_tile(uint32_t tile, uint32_t n, bool multi
     const uint32_t i = seg + k * kWave + lane;
     if (i < n) {
       const T e = M::combine(prefix, ex[k]);
-      st(i, pl[k], v[k], e, M::combine(e, v[k]));
+      st(i, pl[k], v[k], e, M::combine(e, v[k]), n);
     }
   }
 }
@@ -245,15 +245,21 @@ vx_status fused(uint32_t n, Load ld, Store st, void* ws, hipStream_t s) {
 }
 
 // Host launcher for a device-resident count bounded by n_upper. `ws` must hold
-// fused_dev_workspace_bytes(n_upper) bytes.
+// fused_dev_workspace_bytes(n_upper) bytes; with `clear` false the caller
+// guarantees they are zero (a pipeline clears all its scans' workspaces with
+// one fill instead of one per scan).
 template <class M, class Load, class Store>
-vx_status fused_dev_m(const uint32_t* n_ptr, uint64_t n_upper, Load ld, Store st, void* ws, hipStream_t s) {
+vx_status fused_dev_m(const uint32_t* n_ptr, uint64_t n_upper, Load ld, Store st, void* ws, hipStream_t s,
+                      bool clear = true) {
   const uint32_t tiles = std::max<uint32_t>(1, ceil_div(n_upper, kFusedTile));
   Carver c(ws);
   uint32_t* counter = c.take<uint32_t>(64);
   uint64_t* rec = c.take<uint64_t>(tiles);
-  const vx_status r = fill_u32(counter, (reinterpret_cast<char*>(rec + tiles) - reinterpret_cast<char*>(counter)) / 4, 0, s);
-  if (r != VX_OK) return r;
+  if (clear) {
+    const vx_status r =
+        fill_u32(counter, (reinterpret_cast<char*>(rec + tiles) - reinterpret_cast<char*>(counter)) / 4, 0, s);
+    if (r != VX_OK) return r;
+  }
   const uint32_t blocks = std::min(tiles, kFusedMaxBlocks);
   const uint32_t n_cap = uint32_t(std::min<uint64_t>(n_upper, 0xFFFFFFFFu));
   fused_dev_kernel<M, Load, Store><<<blocks, kFusedBlock, 0, s>>>(n_ptr, n_cap, ld, st, counter, rec);
@@ -261,8 +267,9 @@ vx_status fused_dev_m(const uint32_t* n_ptr, uint64_t n_upper, Load ld, Store st
 }
 
 template <Op OP, class Load, class Store>
-vx_status fused_dev(const uint32_t* n_ptr, uint64_t n_upper, Load ld, Store st, void* ws, hipStream_t s) {
-  return fused_dev_m<typename OpOf<OP>::type>(n_ptr, n_upper, ld, st, ws, s);
+vx_status fused_dev(const uint32_t* n_ptr, uint64_t n_upper, Load ld, Store st, void* ws, hipStream_t s,
+                    bool clear = true) {
+  return fused_dev_m<typename OpOf<OP>::type>(n_ptr, n_upper, ld, st, ws, s, clear);
 }
 
 }  // namespace scan

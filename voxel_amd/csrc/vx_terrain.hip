@@ -109,8 +109,7 @@ struct ExpandStore {
   uint32_t* first;   // per node: index of its first partial child
   uint64_t* child_keys;
   uint32_t* n_child;
-  const uint32_t* n_ptr;
-  __device__ void operator()(uint32_t j, const uint64_t& key, uint32_t, uint32_t excl, uint32_t incl) const {
+  __device__ void operator()(uint32_t j, const uint64_t& key, uint32_t, uint32_t excl, uint32_t incl, uint32_t n) const {
     uint32_t bits = 0, r = excl, y0;
     for (uint32_t c = 0; c < 8; ++c) {
       const uint64_t ck = (key << 3) | c;
@@ -120,7 +119,7 @@ struct ExpandStore {
     }
     cls[j] = bits;
     first[j] = excl;
-    if (j == *n_ptr - 1) *n_child = incl;
+    if (j == n - 1) *n_child = incl;
   }
 };
 
@@ -385,7 +384,7 @@ extern "C" vx_status vx_dag_build_terrain(vx_dag* g, const vx_terrain* t, void* 
       if ((st = dev_alloc(fws, scan::fused_dev_workspace_bytes(n))) != VX_OK) return st;
       if ((st = scan::fused_dev<scan::kSum>(cnt + l, n, ExpandLoad{keys[l].get<uint64_t>(), p, l + 1},
                                             ExpandStore{p, l + 1, cls[l].get<uint32_t>(), first[l].get<uint32_t>(),
-                                                        keys[l + 1].get<uint64_t>(), cnt + l + 1, cnt + l},
+                                                        keys[l + 1].get<uint64_t>(), cnt + l + 1},
                                             fws.p, s)) != VX_OK)
         return st;
       n_level[l + 1] = read_u32(cnt + l + 1, s);
