@@ -30,6 +30,28 @@ sys.path.insert(0, ROOT)
 
 METRIC = "voxel edits/s + HBM GB/s, batched vs plain, bit-exact DAG; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+# Kernel kinds of the edit pipeline -> the device symbol rocprofv3 reports.
+KERNEL_SYMBOL = {
+    "records": "fused_dev_kernel<SumOp, BrushLoad, BrushStore>",
+    "sort_partition": "partition_kernel<unsigned long, true, ...>",
+    "leaf_build": "leaf_build_kernel<16u>",
+    "leaf_probe": "probe_kernel<16u>",
+    "leaf_commit": "fused_dev_kernel<SegSumOp, CommitLoad, CommitStore>",
+    "node_build": "node_build_kernel",
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_edit_16384.json")
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this benchmark, scaled by
+    the same counters on a copy kernel of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def parse():
@@ -40,6 +62,8 @@ def parse():
     ap.add_argument("--inner-depth", type=int, default=12)
     ap.add_argument("--brushes", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=2,
+                    help="untimed edits with every kernel launch timed, before the timed steps")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-primitives", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
@@ -47,6 +71,9 @@ def parse():
     ap.add_argument("--cpu-brushes", type=int, default=1024, help="brushes in the CPU baseline sample")
     ap.add_argument("--no-parity", action="store_true",
                     help="skip the full-size check (CPU oracle finishes batch 0; canonical digests compared)")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarise two rocprofv3 --pmc counter CSVs of this benchmark (FETCH_SIZE, WRITE_SIZE) "
+                         "into the per-kernel traffic file used for roofline.traffic, then exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="finish with one 1 GiB read + 1 GiB write elementwise kernel (bitwise_not on int32), "
                          "the known-size reference for hardware byte counters")
@@ -243,23 +270,113 @@ def leg_serving(V, torch, n_requests=2000, seed=20, load=0.7):
     return res
 
 
-def stage_bytes(prof, leaf_words):
-    """Algorithmic HBM bytes per stage (summed over measured edits)."""
+def kernel_bytes(prof, leaf_words):
+    """Algorithmic HBM bytes of every launch of each kernel kind, summed over
+    the measured edits, from the edit's sizes: R brick records, L distinct
+    leaves, N distinct inner nodes, P radix passes, lw leaf payload bytes.
+    Random 4..8-byte reads count their own size (the line-granular cost is
+    what the hardware byte counters report)."""
     R, L, N, P = prof.records, prof.leaves, prof.nodes, prof.sort_passes
     lw = 4 * leaf_words
     return {
-        "records": 24 * R,
-        "sort": R * (8 + 24 * P),
-        "levels": 8 * R + 36 * L + 24 * N,
-        "old_refs": 28 * (L + N),
-        "leaves": 16 * R + L * (12 + 2 * lw + 16) + L * (16 + lw + 8) + L * 8,
-        "nodes": N * (12 + 36 + 16) + N * (16 + 36 + 8) + N * 8,
-        "publish": 0,
+        "records": (prof.sort_record_bytes + 12) * R,   # sort record + mask 8 + material 4 written
+        "sort_histogram": 8 * R,                        # keys read once for every pass's histogram
+        "sort_partition": 2 * prof.sort_record_bytes * R * P,  # per pass: each record read and written
+        "runs": 8 * R + 12 * L,                         # sorted keys read; leaf keys + run starts written
+        "up": 12 * (L + N) + 12 * N,                    # keys read + parent links; parent keys + starts
+        "topdown": 48 * (L + N),                        # parent index, parent ref, parent item, own ref
+        "leaf_build": L * (28 + 2 * lw) + (prof.sort_record_bytes + 8) * R,  # old leaf, item, hash, result;
+                                                        # per record: its sorted entry, mask and material
+        "leaf_probe": L * (32 + lw),                    # result, item, hash, slot, result
+        "leaf_commit": 28 * L,                          # result, hash, header, slot value
+        "leaf_place": L * (24 + 2 * lw),                # result, slot value, item copy, slot, result
+        "node_build": 100 * N + 20 * (L + N),           # old item, starts, item, hash, result; child entries
+        "node_probe": 68 * N,
+        "node_commit": 28 * N,
+        "node_place": 96 * N,
     }
+
+
+PMC_COMMAND = ("cd /tmp && export TMPDIR=/tmp && cd <repo> && "
+               "rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmc_{tag} -o {tag} -- "
+               "python3 bench.py --warmup 1 --profile-steps 0 --steps 1 --no-cpu --no-primitives --no-batched "
+               "--no-serving --copy-calibration")
+
+
+def _kind_of(name, last_probe):
+    """Edit kernel kind of a device symbol (see KERNEL_SYMBOL)."""
+    table = [("BrushLoad", "records"), ("VoxelLoad", "records"), ("histogram_kernel", "sort_histogram"),
+             ("hist_scan_kernel", "sort_scan"), ("partition_kernel", "sort_partition"), ("copy_kernel", "sort_copy"),
+             ("RunLoad", "runs"), ("UpLoad", "up"), ("topdown_kernel", "topdown"), ("leaf_build_kernel", "leaf_build"),
+             ("node_build_kernel", "node_build"), ("publish_kernel", "publish"), ("fill_u32_kernel", "fill"),
+             ("begin_kernel", "other")]
+    if "probe_kernel" in name:
+        return "node_probe" if "<9u>" in name else "leaf_probe"
+    if "place_kernel" in name:
+        return "node_place" if "<9u>" in name else "leaf_place"
+    if "CommitLoad" in name:
+        return "node_commit" if last_probe == "node_probe" else "leaf_commit"
+    for key, kind in table:
+        if key in name:
+            return kind
+    return None
+
+
+def traffic_summary(fetch_csv, write_csv, out_path):
+    """Per-kernel HBM bytes per launch of the last edit in two rocprofv3 --pmc
+    CSVs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by the ratio
+    of known bytes to the same counter on the calibration kernel (bitwise_not
+    over 1 GiB of int32: 1 GiB read, 1 GiB written)."""
+    import csv
+
+    def load(path, counter):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        return [(r["Kernel_Name"], float(r["Counter_Value"])) for r in rows]
+
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    cal_f = [v for n, v in fetch if "bitwise_not" in n][-1]
+    cal_w = [v for n, v in write if "bitwise_not" in n][-1]
+    f_scale, w_scale = (1 << 30) / (cal_f * 1024), (1 << 30) / (cal_w * 1024)
+
+    def last_edit(rows):
+        starts = [i for i, (n, _) in enumerate(rows) if "begin_kernel" in n]
+        return [r for r in rows[starts[-1]:] if "bitwise_not" not in r[0] and "at::native" not in r[0]]
+
+    fe, we = last_edit(fetch), last_edit(write)
+    kinds, last_probe = {}, None
+    for (name, fv), (name2, wv) in zip(fe, we):
+        assert name == name2
+        kind = _kind_of(name, last_probe)
+        if kind in ("leaf_probe", "node_probe"):
+            last_probe = kind
+        if kind is None:
+            continue
+        k = kinds.setdefault(kind, {"launches": 0, "fetch": 0.0, "write": 0.0})
+        k["launches"] += 1
+        k["fetch"] += fv * 1024 * f_scale
+        k["write"] += wv * 1024 * w_scale
+    out = {
+        "workload": "16384^3 materials heightfield seed 3, one edit batch of 4096 brushes (batch 1 of the seed-3 stream)",
+        "commands": [PMC_COMMAND.format(counter=c, tag=t) for c, t in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write"))],
+        "calibration": {"kernel": "bitwise_not over 2^28 int32 (1 GiB read + 1 GiB write)",
+                        "fetch_kib": cal_f, "write_kib": cal_w, "fetch_scale": f_scale, "write_scale": w_scale},
+        "kernels": {kind: {"launches": v["launches"],
+                           "fetch_bytes_per_launch": v["fetch"] / v["launches"],
+                           "write_bytes_per_launch": v["write"] / v["launches"],
+                           "bytes_per_launch": (v["fetch"] + v["write"]) / v["launches"]}
+                    for kind, v in kinds.items()},
+    }
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out["kernels"], indent=1))
 
 
 def main():
     args = parse()
+    if args.traffic:
+        traffic_summary(args.traffic[0], args.traffic[1], TRAFFIC_FILE)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: every box here has one GPU, so --gpus {args.gpus} is not run "
               "(the multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
@@ -271,8 +388,10 @@ def main():
     from voxel_amd import dag as V
 
     world = 4 << args.inner_depth
+    # Batch order: warmup, then the profiled pass, then the timed steps.
+    n_prof = args.profile_steps
     batches = [V.gen_brushes(args.seed, V.STREAM_BRUSH + (k << 8), args.brushes, world, 1)
-               for k in range(args.warmup + args.steps)]
+               for k in range(args.warmup + n_prof + args.steps)]
     check = not args.no_cpu and not args.no_parity and args.warmup > 0
     cpu = None
     if not args.no_cpu:
@@ -302,23 +421,43 @@ def main():
             from oracle import oracle as O  # the checker's digest, applied to the GPU scene's export
             gpu_digest = O.digest(scene.export())
     torch.cuda.synchronize()
+    # Profiled pass (every launch timed): per-kernel table and the dominant
+    # kernel, the one with the largest summed time per edit.
     scene.profile(True)
+    for k in range(args.warmup, args.warmup + n_prof):
+        scene.apply_brushes(batches[k], workspace=ws, stream=stream)
+    torch.cuda.synchronize()
+    pp = scene.profile_read()
+    kbytes_p = kernel_bytes(pp, 16)
+    kernels = {k: {"launches_per_edit": pp.kernel_launches[i] / max(pp.edits, 1),
+                   "ms_per_edit": pp.kernel_ms[i] / max(pp.edits, 1),
+                   "avg_launch_us": pp.kernel_ms[i] * 1e3 / max(pp.kernel_launches[i], 1),
+                   "gbs_algorithmic": (kbytes_p[k] / (pp.kernel_ms[i] * 1e-3) / 1e9)
+                   if k in kbytes_p and pp.kernel_ms[i] > 0 else None}
+               for i, k in enumerate(V.KERNELS) if pp.kernel_launches[i]}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_edit"]) if kernels else "sort_partition"
+    # Timed steps: stage events plus events around the dominant kernel's
+    # launches only (its average launch duration is measured over these steps).
+    scene.profile(False)
+    scene.profile(True)
+    scene.profile_kernels([dom])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.warmup, args.warmup + args.steps):
+    for k in range(args.warmup + n_prof, args.warmup + n_prof + args.steps):
         scene.apply_brushes(batches[k], workspace=ws, stream=stream)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     prof = scene.profile_read()
     scene.profile(False)
-    timed_ops = sum(ops[args.warmup:])
+    timed_ops = sum(ops[args.warmup + n_prof:])
     value = timed_ops / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
 
     stages = {name: prof.ms[i] / max(prof.edits, 1) for i, name in enumerate(V.STAGES)}
-    sbytes = {k: v / max(prof.edits, 1) for k, v in stage_bytes(prof, 16).items()}
-    dom = max(stages, key=lambda s: stages[s])
-    achieved = sbytes[dom] / (stages[dom] * 1e-3) / 1e9 if stages[dom] > 0 else 0.0
+    di = V.KERNELS.index(dom)
+    avg_us = prof.kernel_ms[di] * 1e3 / max(prof.kernel_launches[di], 1)
+    bytes_per_launch = kernel_bytes(prof, 16).get(dom, 0) / max(prof.kernel_launches[di], 1)
+    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
 
     scene.close()
     del ws
@@ -374,14 +513,17 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"edit stage '{dom}'",
+            "kernel": f"edit kernel '{dom}' ({KERNEL_SYMBOL.get(dom, dom)})",
+            "avg_launch_us": avg_us,
+            "bytes_per_launch": bytes_per_launch,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_for(dom),
         },
         "stages_ms": stages,
+        "kernels": kernels,
         "scene": {"build_s": build_s, "node_words": st.node_words_used, "leaf_words": st.leaf_words_used},
         "primitives": prims,
         "batched": batched,

@@ -91,10 +91,15 @@ vx_status vx_sort_pairs_u64(uint64_t* keys, uint32_t* vals, uint32_t n,
                             void* stream);
 
 /* Sort tuning (the reference selects these by template arguments,
- * cuda/sort/radix.cuh:55-68). radix_bits in {4, 8}, keys_per_thread in
- * {8, 15, 23}, block in {256, 512, 1024}; combinations whose tile does not
- * fit LDS are VX_UNSUPPORTED. Takes effect for later calls of this process.
- * key_bytes selects the table row (4 or 8). */
+ * cuda/sort/radix.cuh:55-68). radix_bits in {4, 8} with keys_per_thread in
+ * {8, 15, 23} and block in {256, 512, 1024}, or radix_bits 9 with
+ * (keys_per_thread, block) in {(8, 512), (8, 1024), (15, 512)};
+ * combinations whose tile does not fit LDS are VX_UNSUPPORTED. Takes effect
+ * for later calls of this process. key_bytes selects the table row: 4 or 8
+ * for (key, value) sorts, or'ed with VX_SORT_KEYS_ONLY for sorts without
+ * values (the edit pipeline's packed record keys use the 8-byte keys-only
+ * row). */
+#define VX_SORT_KEYS_ONLY 0x100u
 vx_status vx_sort_set_tuning(uint32_t key_bytes, uint32_t radix_bits,
                              uint32_t keys_per_thread, uint32_t block);
 vx_status vx_sort_get_tuning(uint32_t key_bytes, uint32_t* radix_bits,
@@ -302,8 +307,33 @@ vx_status vx_brush_voxel_ops(const vx_brush* h_brushes, uint32_t n, uint32_t wor
 /* Per-stage edit timing (hipEvents recorded on the edit's stream) and the
  * sizes each stage worked on, accumulated over edits while enabled.
  * Stages: 0 records, 1 sort, 2 level structure, 3 old-ref walk,
- * 4 leaves (build + probe + commit), 5 nodes (all levels), 6 publish. */
+ * 4 leaves (build + probe + commit), 5 nodes (all levels), 6 publish.
+ * Kernels: launches and summed duration of each kernel kind of the edit
+ * pipeline (a hipEvent pair around every launch of a direct edit call;
+ * graph replays are not timed). */
 #define VX_DAG_STAGES 7
+enum {
+  VX_K_RECORDS = 0,
+  VX_K_SORT_HISTOGRAM,
+  VX_K_SORT_SCAN,
+  VX_K_SORT_PARTITION,
+  VX_K_SORT_COPY,
+  VX_K_RUNS,
+  VX_K_UP,
+  VX_K_TOPDOWN,
+  VX_K_LEAF_BUILD,
+  VX_K_LEAF_PROBE,
+  VX_K_LEAF_COMMIT,
+  VX_K_LEAF_PLACE,
+  VX_K_NODE_BUILD,
+  VX_K_NODE_PROBE,
+  VX_K_NODE_COMMIT,
+  VX_K_NODE_PLACE,
+  VX_K_PUBLISH,
+  VX_K_FILL,
+  VX_K_OTHER,
+  VX_DAG_KERNELS
+};
 typedef struct vx_dag_profile {
   double ms[VX_DAG_STAGES];
   uint64_t edits;        /* edit calls measured */
@@ -311,9 +341,16 @@ typedef struct vx_dag_profile {
   uint64_t leaves;       /* distinct leaves touched (sum) */
   uint64_t nodes;        /* distinct inner nodes touched (sum) */
   uint64_t sort_passes;  /* radix passes per edit */
+  uint64_t sort_record_bytes; /* bytes one record occupies in the sort: 12 (key, index) or 8 (packed) */
+  uint64_t kernel_launches[VX_DAG_KERNELS];
+  double kernel_ms[VX_DAG_KERNELS];
 } vx_dag_profile;
 vx_status vx_dag_profile_enable(vx_dag* dag, int32_t enable);
 vx_status vx_dag_profile_read(const vx_dag* dag, vx_dag_profile* out);
+/* Which kernel kinds get per-launch events while profiled (bit k = VX_K_k;
+ * default all): timing only the kernel under study keeps the event overhead
+ * out of the other launches. */
+vx_status vx_dag_profile_kernel_mask(vx_dag* dag, uint64_t mask);
 
 #ifdef __cplusplus
 }

@@ -292,3 +292,40 @@ def test_full_size_edit_sampled_truth(gpu):
     want = _brush_truth(brushes, xyz, base)
     got = g.get_voxels(_dev_xyz(xyz, gpu)).cpu().numpy()
     assert np.array_equal(got.astype(np.int64), want)
+
+
+def test_kernel_profile_counts_every_launch(gpu):
+    """Per-kernel timing while profiled: one partition launch per radix pass,
+    one records launch per edit, and the kernel times add up to (at most) the
+    stage times they fall in."""
+    from voxel_amd.dag import KERNELS, STREAM_BRUSH, gen_brushes, terrain_for_world
+    d = _dag(7, True, node_words=1 << 24, leaf_words=1 << 25)
+    d.build_terrain(terrain_for_world(512, 1))
+    d.profile(True)
+    for k in range(3):
+        d.apply_brushes(gen_brushes(1, STREAM_BRUSH + (k << 8), 64, 512, 0))
+    p = d.profile_read()
+    launches = dict(zip(KERNELS, list(p.kernel_launches)))
+    ms = dict(zip(KERNELS, list(p.kernel_ms)))
+    assert p.edits == 3
+    assert launches["records"] == 3
+    assert launches["sort_partition"] == 3 * p.sort_passes
+    assert launches["leaf_probe"] == 3 and launches["node_probe"] == 3 * 7
+    assert sum(ms.values()) <= sum(p.ms) * 1.05 + 0.05
+    assert all(v > 0 for k, v in ms.items() if launches[k])
+
+
+def test_pair_sort_layout_matches_packed(gpu, monkeypatch):
+    """The (key, index) pair sort, used when a packed key would not fit 64
+    bits, gives the same scene as the packed-key sort."""
+    from voxel_amd.dag import STREAM_BRUSH, gen_brushes, terrain_for_world
+    br = [gen_brushes(41, STREAM_BRUSH + (k << 8), 96, 1024, 1) for k in range(2)]
+    out = []
+    for force in ("0", "1"):
+        monkeypatch.setenv("VOXEL_AMD_SORT_PAIRS", force)
+        d = _dag(8, True, node_words=1 << 23, leaf_words=1 << 24)
+        d.build_terrain(terrain_for_world(1024, 5))
+        for b in br:
+            d.apply_brushes(b)
+        out.append(d.export())
+    assert np.array_equal(out[0], out[1])

@@ -136,22 +136,27 @@ def test_sort_keys_only(gpu):
     _sort_check(gpu, O.gen_u32(321, O.STREAM_KEYS, 50_000, 0, 0xFFFFFFFF), with_vals=False)
 
 
-@pytest.mark.parametrize("rb,kpt,block", [(4, 8, 256), (4, 15, 512), (8, 8, 1024), (8, 23, 256), (8, 15, 512)])
+@pytest.mark.parametrize("rb,kpt,block", [(4, 8, 256), (4, 15, 512), (8, 8, 1024), (8, 23, 256), (8, 15, 512),
+                                          (9, 8, 1024), (9, 8, 512), (9, 15, 512)])
 def test_sort_tunings_agree(gpu, rb, kpt, block):
     import voxel_amd.primitives as P
     from voxel_amd import VxError
-    old32, old64 = P.get_sort_tuning(4), P.get_sort_tuning(8)
+    from voxel_amd._lib import SORT_KEYS_ONLY
+    rows = (4, 8, 4 | SORT_KEYS_ONLY, 8 | SORT_KEYS_ONLY)
+    old = [P.get_sort_tuning(r) for r in rows]
     try:
         try:
-            P.set_sort_tuning(4, rb, kpt, block)
-            P.set_sort_tuning(8, rb, kpt, block)
+            for r in rows:
+                P.set_sort_tuning(r, rb, kpt, block)
         except VxError:
             pytest.skip("tile does not fit LDS")
         _sort_check(gpu, O.gen_u32(654, O.STREAM_KEYS, 300_001, 0, 0xFFFFFFFF))
         _sort_check(gpu, O.gen_morton(987, O.STREAM_KEYS, 200_001, 14), key_bits=42)
+        _sort_check(gpu, O.gen_u32(655, O.STREAM_KEYS, 300_001, 0, 0xFFFFFFFF), with_vals=False)
+        _sort_check(gpu, O.gen_morton(988, O.STREAM_KEYS, 200_001, 12), key_bits=36, with_vals=False)
     finally:
-        P.set_sort_tuning(4, *old32)
-        P.set_sort_tuning(8, *old64)
+        for r, t in zip(rows, old):
+            P.set_sort_tuning(r, *t)
 
 
 def test_sort_rejects_bad_arguments(gpu):

@@ -19,6 +19,8 @@ VX_OUT_OF_CAPACITY = 2
 VX_DEVICE_ERROR = 3
 VX_UNSUPPORTED = 4
 
+SORT_KEYS_ONLY = 0x100  # VX_SORT_KEYS_ONLY: the keys-only row of the sort tuning table
+
 STATUS_NAMES = {0: "VX_OK", 1: "VX_INVALID_VALUE", 2: "VX_OUT_OF_CAPACITY", 3: "VX_DEVICE_ERROR",
                 4: "VX_UNSUPPORTED"}
 
@@ -105,7 +107,7 @@ def _apply_tuning(L) -> None:
         return
     with open(path) as f:
         t = json.load(f)
-    for key, kb in (("sort_u32", 4), ("sort_u64", 8)):
+    for key, kb in (("sort_u32", 4), ("sort_u64", 8), ("sort_u64_keys", 8 | SORT_KEYS_ONLY)):
         if key in t:
             L.vx_sort_set_tuning(kb, *t[key])
     if "scan" in t:

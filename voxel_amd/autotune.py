@@ -1,10 +1,12 @@
 """Autotuner for the sort and scan tile shapes.
 
-Sweeps the onesweep sort over radix bits {4, 8} x keys/thread {8, 15, 23} x
-workgroup {256, 512, 1024} (combinations whose tile does not fit LDS are
-skipped) for u32 and u64 keys, and the scan over its tile shapes, timing each
-with hipEvents on the workload it is asked for; the best shape per key width
-is written to voxel_amd/tuning.json, which `voxel_amd` applies at import.
+Sweeps the onesweep sort over radix bits {4, 8, 9} x keys/thread {8, 15, 23}
+x workgroup {256, 512, 1024} (combinations that are not built or whose tile
+does not fit LDS are skipped) for u32 and u64 (key, index) pairs and for u64
+keys alone (36-bit keys, the edit pipeline's packed record keys), and the
+scan over its tile shapes, timing each with hipEvents on the workload it is
+asked for; the best shape per row is written to voxel_amd/tuning.json, which
+`voxel_amd` applies at import.
 
     python -m voxel_amd.autotune [--sort-keys 26] [--batched]
 
@@ -23,7 +25,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING_PATH = os.path.join(HERE, "tuning.json")
 
-RADIX_BITS = (4, 8)
+RADIX_BITS = (4, 8, 9)
 KEYS_PER_THREAD = (8, 15, 23)
 BLOCKS = (256, 512, 1024)
 SCAN_SHAPES = ((256, 4), (256, 8), (256, 16), (512, 4), (512, 8), (1024, 4))
@@ -39,9 +41,9 @@ def load() -> dict:
 def apply(tuning: dict | None = None) -> None:
     """Apply a tuning table (default: tuning.json) to this process."""
     from . import primitives as P
-    from ._lib import VxError
+    from ._lib import SORT_KEYS_ONLY, VxError
     t = load() if tuning is None else tuning
-    for key, kb in (("sort_u32", 4), ("sort_u64", 8)):
+    for key, kb in (("sort_u32", 4), ("sort_u64", 8), ("sort_u64_keys", 8 | SORT_KEYS_ONLY)):
         if key in t:
             try:
                 P.set_sort_tuning(kb, *t[key])
@@ -74,29 +76,41 @@ def _timer():
     return timed
 
 
-def sweep_sort(log2_keys: int, key_bytes: int, key_bits: int) -> list:
+def sweep_sort(log2_keys: int, key_bytes: int, key_bits: int, with_values: bool = True) -> list:
     import torch
 
     from . import primitives as P
-    from ._lib import VxError
+    from ._lib import SORT_KEYS_ONLY, VxError
     n = 1 << log2_keys
     timed = _timer()
-    src_k = P.gen_u32(2, 1, n, 0, 0xFFFFFFFF) if key_bytes == 4 else P.gen_morton(2, 1, n, 14)
-    src_v = P.iota(n)
-    k, v = torch.empty_like(src_k), torch.empty_like(src_v)
+    if key_bytes == 4:
+        src_k = P.gen_u32(2, 1, n, 0, 0xFFFFFFFF)
+    elif key_bits == 42:
+        src_k = P.gen_morton(2, 1, n, 14)
+    else:
+        src_k = P.gen_morton(2, 1, n, key_bits // 3)
+    src_v = P.iota(n) if with_values else None
+    k = torch.empty_like(src_k)
+    v = torch.empty_like(src_v) if with_values else None
     ws = torch.empty(P.required_sort_workspace_size(n, key_bytes, key_bits, True) * 2 + (1 << 20),
                      dtype=torch.uint8, device="cuda")
-    base = timed(lambda: (k.copy_(src_k), v.copy_(src_v)))
+    row = key_bytes if with_values else key_bytes | SORT_KEYS_ONLY
+
+    def copy_in():
+        k.copy_(src_k)
+        if with_values:
+            v.copy_(src_v)
+
+    base = timed(copy_in)
     out = []
     for rb, kpt, blk in itertools.product(RADIX_BITS, KEYS_PER_THREAD, BLOCKS):
         try:
-            P.set_sort_tuning(key_bytes, rb, kpt, blk)
+            P.set_sort_tuning(row, rb, kpt, blk)
         except VxError:
             continue
 
         def run():
-            k.copy_(src_k)
-            v.copy_(src_v)
+            copy_in()
             P.sort_by_key(k, v, key_bits, workspace=ws)
 
         ms = timed(run) - base
@@ -158,26 +172,24 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--sort-keys", type=int, default=26, help="log2 of the key count of the sort sweep")
     ap.add_argument("--scan-elems", type=int, default=28)
+    ap.add_argument("--record-keys", type=int, default=24,
+                    help="log2 of the key count of the packed record-key sweep (an edit of 4096 brushes at "
+                         "16384^3 sorts about 2^24 records)")
     ap.add_argument("--out", default=TUNING_PATH)
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
+    rows = ("sort_u32", "sort_u64", "sort_u64_keys")
     res = {"sort_u32": sweep_sort(args.sort_keys, 4, 32), "sort_u64": sweep_sort(args.sort_keys - 1, 8, 42),
+           "sort_u64_keys": sweep_sort(args.record_keys, 8, 36, with_values=False),
            "scan": sweep_scan(args.scan_elems)}
-    best = {
-        "sort_u32": min(res["sort_u32"], key=lambda r: r["ms"]),
-        "sort_u64": min(res["sort_u64"], key=lambda r: r["ms"]),
-        "scan": min(res["scan"], key=lambda r: r["ms"]),
-    }
-    table = {
-        "sort_u32": [best["sort_u32"]["radix_bits"], best["sort_u32"]["keys_per_thread"], best["sort_u32"]["block"]],
-        "sort_u64": [best["sort_u64"]["radix_bits"], best["sort_u64"]["keys_per_thread"], best["sort_u64"]["block"]],
-        "scan": [best["scan"]["block"], best["scan"]["vectors_per_lane"]],
-        "sweep": res,
-    }
+    best = {k: min(res[k], key=lambda r: r["ms"]) for k in res}
+    table = {k: [best[k]["radix_bits"], best[k]["keys_per_thread"], best[k]["block"]] for k in rows}
+    table["scan"] = [best["scan"]["block"], best["scan"]["vectors_per_lane"]]
+    table["sweep"] = res
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1)
-    print(json.dumps({k: table[k] for k in ("sort_u32", "sort_u64", "scan")}))
-    for k in ("sort_u32", "sort_u64", "scan"):
+    print(json.dumps({k: table[k] for k in rows + ("scan",)}))
+    for k in rows + ("scan",):
         for r in sorted(res[k], key=lambda r: r["ms"]):
             print(k, r)
 

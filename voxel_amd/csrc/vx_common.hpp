@@ -98,6 +98,39 @@ __host__ __device__ __forceinline__ uint32_t uniform_u32(uint64_t r, uint32_t lo
   return lo + uint32_t(((r >> 32) * span) >> 32);
 }
 
+// Optional per-launch timing. While a KernelTimer is installed on the calling
+// host thread, every launch bracketed by a TimedLaunch gets a hipEvent pair
+// on its stream, tagged with its kernel kind (VX_K_*). Scopes do not nest:
+// an inner TimedLaunch inside an outer one is a no-op.
+struct KernelTimer {
+  static constexpr uint32_t kEvents = 1024;
+  hipEvent_t ev[kEvents] = {};
+  uint32_t kind[kEvents / 2] = {};
+  uint32_t used = 0;
+  uint32_t depth = 0;
+  uint64_t mask = ~0ull;  // kinds that are timed
+};
+KernelTimer*& current_timer();
+
+struct TimedLaunch {
+  KernelTimer* t = nullptr;
+  uint32_t slot = 0;
+  hipStream_t s;
+  TimedLaunch(uint32_t kind, hipStream_t stream) : s(stream) {
+    KernelTimer* k = current_timer();
+    if (!k || k->depth++ > 0 || k->used + 2 > KernelTimer::kEvents || !((k->mask >> kind) & 1u)) return;
+    t = k;
+    slot = k->used;
+    k->kind[slot / 2] = kind;
+    if (hipEventRecord(k->ev[slot], s) != hipSuccess) t = nullptr;
+  }
+  ~TimedLaunch() {
+    KernelTimer* k = current_timer();
+    if (k) --k->depth;
+    if (t && hipEventRecord(t->ev[slot + 1], s) == hipSuccess) t->used = slot + 2;
+  }
+};
+
 // Zero a u32 range (graph-capturable replacement for hipMemsetAsync).
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v);
 vx_status fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);

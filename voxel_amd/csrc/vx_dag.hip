@@ -33,6 +33,7 @@
 // Status: first error wins per scene (detail/edit_common.cuh:96-99); a scene's
 // root is published only if that scene saw no error (detail/edit/root.cuh:13-25).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <unordered_map>
@@ -292,13 +293,22 @@ template <uint32_t kMax>
 vx_status intern_width(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet set, uint64_t* hashes,
                        const uint32_t* n_ptr, uint64_t n_ub, uint64_t* res, void* fused_ws, hipStream_t s,
                        bool clear_ws) {
-  probe_kernel<kMax><<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, n_scenes, leaves);
+  {
+    TimedLaunch tl(leaves ? VX_K_LEAF_PROBE : VX_K_NODE_PROBE, s);
+    probe_kernel<kMax><<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, n_scenes, leaves);
+  }
   vx_status st = last_launch();
   if (st != VX_OK) return st;
-  if ((st = scan::fused_dev_m<scan::SegSumOp>(n_ptr, n_ub, CommitLoad{res, hashes, n_ptr, set, sc, leaves},
-                                              CommitStore{res, hashes, sc, leaves}, fused_ws, s, clear_ws)) != VX_OK)
-    return st;
-  place_kernel<kMax><<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, leaves);
+  {
+    TimedLaunch tl(leaves ? VX_K_LEAF_COMMIT : VX_K_NODE_COMMIT, s);
+    st = scan::fused_dev_m<scan::SegSumOp>(n_ptr, n_ub, CommitLoad{res, hashes, n_ptr, set, sc, leaves},
+                                           CommitStore{res, hashes, sc, leaves}, fused_ws, s, clear_ws);
+  }
+  if (st != VX_OK) return st;
+  {
+    TimedLaunch tl(leaves ? VX_K_LEAF_PLACE : VX_K_NODE_PLACE, s);
+    place_kernel<kMax><<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, leaves);
+  }
   return last_launch();
 }
 
@@ -445,12 +455,35 @@ __device__ __forceinline__ bool brick_touched(const vx_brush& b, int32_t X, int3
   return dx * dx + dy * dy + dz * dz <= b.hx * b.hx;
 }
 
+// Record outputs. With index_bits > 0 the sort key is packed above the
+// record index (key << index_bits | index) and sorted by its upper bits, so
+// the sort moves 8 bytes per record instead of a 12-byte (key, index) pair.
 struct RecordOut {
   uint64_t* key;
   uint32_t* val;
   uint64_t* mask;
   uint32_t* mat;
   uint32_t* n_rec;
+  uint32_t index_bits;
+  __device__ __forceinline__ void put_key(uint32_t pos, uint64_t k) const {
+    if (index_bits) {
+      key[pos] = (k << index_bits) | pos;
+    } else {
+      key[pos] = k;
+      val[pos] = pos;
+    }
+  }
+};
+
+// Sorted records, in either layout of RecordOut.
+struct SortedRecs {
+  const uint64_t* keys;
+  const uint32_t* vals;
+  uint32_t index_bits;
+  __device__ __forceinline__ uint64_t key(uint32_t i) const { return index_bits ? keys[i] >> index_bits : keys[i]; }
+  __device__ __forceinline__ uint32_t index(uint32_t i) const {
+    return index_bits ? uint32_t(keys[i] & ((uint64_t(1) << index_bits) - 1)) : vals[i];
+  }
 };
 
 // Brush candidates: candidate i is one brick of one job's clipped brick box.
@@ -517,8 +550,7 @@ struct BrushStore {
     if (v) {
       const vx_brush& b = c.jobs[r.job].b;
       const uint32_t gx = r.gxy & 0xffffu, gy = r.gxy >> 16, gz = r.gz;
-      out.key[excl] = (uint64_t(c.jobs[r.job].scene) << c.codec.shift) | morton3(gx, gy, gz);
-      out.val[excl] = excl;
+      out.put_key(excl, (uint64_t(c.jobs[r.job].scene) << c.codec.shift) | morton3(gx, gy, gz));
       out.mask[excl] = brick_mask(b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4));
       out.mat[excl] = c.materials ? b.material : (b.material ? 1u : 0u);
     }
@@ -546,8 +578,7 @@ struct RecordStore {
   __device__ void operator()(uint32_t i, const Record& rec, uint32_t v, uint32_t excl, uint32_t incl,
                              uint32_t n) const {
     if (v) {
-      out.key[excl] = rec.key;
-      out.val[excl] = excl;
+      out.put_key(excl, rec.key);
       out.mask[excl] = rec.mask;
       out.mat[excl] = rec.mat;
     }
@@ -559,10 +590,10 @@ struct RecordStore {
 // Height 0: distinct leaf keys of the sorted records, with record ranges.
 struct RunLoad {
   using Payload = uint64_t;
-  const uint64_t* keys;
+  SortedRecs recs;
   __device__ uint32_t operator()(uint32_t i, uint64_t& k) const {
-    k = keys[i];
-    return i == 0 || keys[i - 1] != k;
+    k = recs.key(i);
+    return i == 0 || recs.key(i - 1) != k;
   }
 };
 
@@ -639,7 +670,7 @@ __global__ void topdown_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t*
 // ---- rebuild ---------------------------------------------------------------
 template <uint32_t LW>
 __global__ void leaf_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ old,
-                                  const uint32_t* __restrict__ first, const uint32_t* __restrict__ sval,
+                                  const uint32_t* __restrict__ first, SortedRecs recs,
                                   const uint64_t* __restrict__ rmask, const uint32_t* __restrict__ rmat,
                                   uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
                                   uint64_t* __restrict__ res) {
@@ -654,13 +685,11 @@ __global__ void leaf_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32
 #pragma unroll
       for (uint32_t k = 0; k < LW; ++k) w[k] = 0xffffffffu;
     } else {
-      const uint32_t* lp = sc.of(j).leaf_pool + ref;
-#pragma unroll
-      for (uint32_t k = 0; k < LW; ++k) w[k] = lp[k];
+      load_item<LW>(sc.of(j).leaf_pool + ref, LW, w);
     }
     const uint32_t r1 = first[j + 1];
     for (uint32_t r = first[j]; r < r1; ++r) {
-      const uint32_t idx = sval[r];
+      const uint32_t idx = recs.index(r);
       const uint64_t m = rmask[idx];
       const uint32_t mat = rmat[idx];
       if constexpr (LW == 2) {
@@ -761,6 +790,7 @@ struct EditWs {
   uint64_t *rkey, *rkey_alt, *rmask;
   uint32_t *rval, *rval_alt, *rmat;
   void* sort_ws;
+  uint32_t index_bits;  // > 0: records sort as packed u64 keys (RecordOut)
   Heights hs;
   uint32_t* items;
   uint64_t* hashes;
@@ -803,13 +833,21 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
   w.counts = c.take<uint32_t>(64);
   w.root_pos = c.take<uint32_t>(p.n_scenes);
   const uint64_t R = std::max<uint64_t>(1, r_ub);
+  // Packed records when the index of the last record fits below the key
+  // (VOXEL_AMD_SORT_PAIRS=1 forces the pair layout, for testing).
+  uint32_t ib = 1;
+  while (ib < 32 && (uint64_t(1) << ib) < R) ++ib;
+  const char* force_pairs = std::getenv("VOXEL_AMD_SORT_PAIRS");
+  const bool pairs = force_pairs && force_pairs[0] == '1';
+  w.index_bits = !pairs && std::max(p.key_bits, 1u) + ib <= 64 ? ib : 0u;
   w.rkey = c.take<uint64_t>(R);
   w.rkey_alt = c.take<uint64_t>(R);
   w.rmask = c.take<uint64_t>(R);
-  w.rval = c.take<uint32_t>(R);
-  w.rval_alt = c.take<uint32_t>(R);
+  w.rval = w.index_bits ? nullptr : c.take<uint32_t>(R);
+  w.rval_alt = w.index_bits ? nullptr : c.take<uint32_t>(R);
   w.rmat = c.take<uint32_t>(R);
-  w.sort_ws = c.take<char>(sort::pairs_u64_dev_workspace_bytes(R, p.key_bits));
+  w.sort_ws = c.take<char>(w.index_bits ? sort::keys_u64_dev_workspace_bytes(R, std::max(p.key_bits, 1u))
+                                        : sort::pairs_u64_dev_workspace_bytes(R, p.key_bits));
   uint64_t max_cap = 1, max_items = 1;
   const uint32_t H = p.max_depth;
   for (uint32_t h = 0; h <= H + 1; ++h) {
@@ -860,41 +898,56 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
   vx_status st;
   bool in_alt = false;
   mark(g, 1, s);
-  if ((st = sort::pairs_u64_dev(w.rkey, w.rval, w.rkey_alt, w.rval_alt, &cnt[0], std::max<uint64_t>(r_ub, 1),
-                                std::max(p.key_bits, 1u), w.sort_ws, &in_alt, s)) != VX_OK)
-    return st;
+  if (w.index_bits)
+    st = sort::keys_u64_dev(w.rkey, w.rkey_alt, &cnt[0], std::max<uint64_t>(r_ub, 1), w.index_bits,
+                            std::max(p.key_bits, 1u), w.sort_ws, &in_alt, s);
+  else
+    st = sort::pairs_u64_dev(w.rkey, w.rval, w.rkey_alt, w.rval_alt, &cnt[0], std::max<uint64_t>(r_ub, 1),
+                             std::max(p.key_bits, 1u), w.sort_ws, &in_alt, s);
+  if (st != VX_OK) return st;
   mark(g, 2, s);
-  const uint64_t* skey = in_alt ? w.rkey_alt : w.rkey;
-  const uint32_t* sval = in_alt ? w.rval_alt : w.rval;
-  if ((st = scan::fused_dev<scan::kSum>(&cnt[0], r_ub, RunLoad{skey}, RunStore{L.keys[0], L.first[0], &cnt[1]},
-                                        w.run_ws, s, false)) != VX_OK)
-    return st;
+  const SortedRecs recs{in_alt ? w.rkey_alt : w.rkey, in_alt ? w.rval_alt : w.rval, w.index_bits};
+  {
+    TimedLaunch tl(VX_K_RUNS, s);
+    st = scan::fused_dev<scan::kSum>(&cnt[0], r_ub, RunLoad{recs}, RunStore{L.keys[0], L.first[0], &cnt[1]}, w.run_ws,
+                                     s, false);
+  }
+  if (st != VX_OK) return st;
   for (uint32_t h = 0; h <= H; ++h) {
     Scenes sh = sc;
     sh.keys = L.keys[h];
-    if ((st = scan::fused_dev<scan::kSum>(&cnt[1 + h], L.cap[h], UpLoad{L.keys[h], sh, h},
-                                          UpStore{p.codec, L.keys[h + 1], L.first[h + 1], L.parent[h], &cnt[2 + h],
-                                                  w.root_pos},
-                                          w.up_ws[h], s, false)) != VX_OK)
-      return st;
+    {
+      TimedLaunch tl(VX_K_UP, s);
+      st = scan::fused_dev<scan::kSum>(&cnt[1 + h], L.cap[h], UpLoad{L.keys[h], sh, h},
+                                       UpStore{p.codec, L.keys[h + 1], L.first[h + 1], L.parent[h], &cnt[2 + h],
+                                               w.root_pos},
+                                       w.up_ws[h], s, false);
+    }
+    if (st != VX_OK) return st;
   }
   mark(g, 3, s);
   for (uint32_t h = H + 1; h-- > 0;) {
     Scenes sh = sc;
     sh.keys = L.keys[h];
-    topdown_kernel<<<grid_for(L.cap[h]), 256, 0, s>>>(sh, &cnt[1 + h], L.parent[h], L.old[h + 1], L.old[h], h);
+    {
+      TimedLaunch tl(VX_K_TOPDOWN, s);
+      topdown_kernel<<<grid_for(L.cap[h]), 256, 0, s>>>(sh, &cnt[1 + h], L.parent[h], L.old[h + 1], L.old[h], h);
+    }
     if ((st = last_launch()) != VX_OK) return st;
   }
   mark(g, 4, s);
   {
     Scenes sh = sc;
     sh.keys = L.keys[0];
-    if (p.leaf_words == 2)
-      leaf_build_kernel<2><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.old[0], L.first[0], sval, w.rmask, w.rmat,
-                                                              w.items, w.hashes, L.res[0]);
-    else
-      leaf_build_kernel<16><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.old[0], L.first[0], sval, w.rmask,
-                                                               w.rmat, w.items, w.hashes, L.res[0]);
+    {
+      TimedLaunch tl(VX_K_LEAF_BUILD, s);
+      if (p.leaf_words == 2)
+        leaf_build_kernel<2><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.old[0], L.first[0], recs, w.rmask,
+                                                                w.rmat, w.items, w.hashes, L.res[0]);
+      else
+        leaf_build_kernel<16><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.old[0], L.first[0], recs, w.rmask,
+                                                                 w.rmat, w.items, w.hashes, L.res[0]);
+    }
     if ((st = last_launch()) != VX_OK) return st;
     if ((st = intern_scenes(sh, p.n_scenes, true, ItemSet{w.items, p.leaf_words, p.leaf_words}, w.hashes, &cnt[1],
                             L.cap[0], L.res[0], w.commit_ws[0], s, false)) != VX_OK)
@@ -904,8 +957,11 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
   for (uint32_t h = 1; h <= H; ++h) {
     Scenes sh = sc;
     sh.keys = L.keys[h];
-    node_build_kernel<<<grid_for(L.cap[h]), 256, 0, s>>>(sh, &cnt[1 + h], L.old[h], L.first[h], L.keys[h - 1],
-                                                         L.parent[h - 1], L.res[h - 1], w.items, w.hashes, L.res[h]);
+    {
+      TimedLaunch tl(VX_K_NODE_BUILD, s);
+      node_build_kernel<<<grid_for(L.cap[h]), 256, 0, s>>>(sh, &cnt[1 + h], L.old[h], L.first[h], L.keys[h - 1],
+                                                           L.parent[h - 1], L.res[h - 1], w.items, w.hashes, L.res[h]);
+    }
     if ((st = last_launch()) != VX_OK) return st;
     if ((st = intern_scenes(sh, p.n_scenes, false, ItemSet{w.items, kNodeStride, 0}, w.hashes, &cnt[1 + h], L.cap[h],
                             L.res[h], w.commit_ws[h], s, false)) != VX_OK)
@@ -914,7 +970,10 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
   mark(g, 6, s);
   ResByHeight rh{};
   for (uint32_t h = 0; h <= H && h < kMaxHeights; ++h) rh.p[h] = L.res[h];
-  publish_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes, w.root_pos, rh);
+  {
+    TimedLaunch tl(VX_K_PUBLISH, s);
+    publish_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes, w.root_pos, rh);
+  }
   st = last_launch();
   mark(g, 7, s);
   return st;
@@ -973,7 +1032,19 @@ vx_status finish(vx_dag* g, const std::vector<vx_dag*>& dags, const EditWs& w, c
     g->prof.records += counts[0];
     g->prof.leaves += counts[1];
     for (uint32_t h = 1; h <= p.max_depth; ++h) g->prof.nodes += counts[1 + h];
-    g->prof.sort_passes = (p.key_bits + 7) / 8;
+    const uint32_t kb = std::max(p.key_bits, 1u);
+    g->prof.sort_passes = w.index_bits ? sort::keys_u64_dev_passes(kb) : sort::pairs_u64_dev_passes(kb);
+    g->prof.sort_record_bytes = w.index_bits ? 8 : 12;
+    if (g->timer) {
+      KernelTimer& t = *g->timer;
+      for (uint32_t k = 0; k + 1 < t.used; k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.ev[k], t.ev[k + 1]) != hipSuccess) continue;
+        g->prof.kernel_ms[t.kind[k / 2]] += ms;
+        g->prof.kernel_launches[t.kind[k / 2]] += 1;
+      }
+      t.used = 0;
+    }
   }
   for (uint32_t v : status)
     if (v != kStatusOk) return VX_OUT_OF_CAPACITY;
@@ -1001,7 +1072,9 @@ vx_status upload(EditWs& w, const std::vector<Dev>& devs, const std::vector<Brus
 }
 
 // The record writer that matches each candidate kind.
-inline RecordOut record_out(EditWs& w) { return RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0]}; }
+inline RecordOut record_out(EditWs& w) {
+  return RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0], w.index_bits};
+}
 inline RecordStore store_for(const VoxelLoad&, EditWs& w) {
   return RecordStore{record_out(w)};
 }
@@ -1019,12 +1092,17 @@ vx_status device_pipeline(vx_dag* prof_dag, const Plan& p, const Scenes& sc, Edi
   if ((st = fill_u32(w.counts, kCntCandidates, 0, s)) != VX_OK) return st;
   if ((st = fill_u32(w.root_pos, p.n_scenes, kNone, s)) != VX_OK) return st;
   if ((st = fill_u32(w.fused_all, w.fused_bytes / 4, 0, s)) != VX_OK) return st;
-  begin_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes);
+  {
+    TimedLaunch tl(VX_K_OTHER, s);
+    begin_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes);
+  }
   if ((st = last_launch()) != VX_OK) return st;
   mark(prof_dag, 0, s);
-  if ((st = scan::fused_dev<scan::kSum>(&w.counts[kCntCandidates], cand_cap, load, store_for(load, w), w.rec_ws, s,
-                                        false)) != VX_OK)
-    return st;
+  {
+    TimedLaunch tl(VX_K_RECORDS, s);
+    st = scan::fused_dev<scan::kSum>(&w.counts[kCntCandidates], cand_cap, load, store_for(load, w), w.rec_ws, s, false);
+  }
+  if (st != VX_OK) return st;
   return rebuild(prof_dag, p, sc, w, cand_cap, s);
 }
 
@@ -1038,7 +1116,21 @@ vx_status run_edit(const std::vector<vx_dag*>& dags, vx_dag* prof_dag, const Pla
   uint32_t staging[2];
   if ((st = upload(w, devs, jobs, n_cand, staging, s)) != VX_OK) return st;
   const Scenes sc{p.n_scenes > 1 ? w.scenes : nullptr, nullptr, p.codec, devs[0]};
-  if ((st = device_pipeline(prof_dag, p, sc, w, n_cand, load, s)) != VX_OK) return st;
+  {
+    // Per-launch timing of this call's kernels while the dag is profiled.
+    struct Install {
+      KernelTimer* prev;
+      explicit Install(KernelTimer* t) : prev(current_timer()) {
+        if (t) {
+          t->used = 0;
+          t->depth = 0;
+          current_timer() = t;
+        }
+      }
+      ~Install() { current_timer() = prev; }
+    } install(prof_dag && prof_dag->profile ? prof_dag->timer : nullptr);
+    if ((st = device_pipeline(prof_dag, p, sc, w, n_cand, load, s)) != VX_OK) return st;
+  }
   return finish(prof_dag, dags, w, p, s);
 }
 
@@ -1150,6 +1242,11 @@ extern "C" vx_status vx_dag_destroy(vx_dag* g) {
   if (g->ctl) hipFree(g->ctl);
   for (auto& e : g->ev)
     if (e) hipEventDestroy(e);
+  if (g->timer) {
+    for (auto& e : g->timer->ev)
+      if (e) hipEventDestroy(e);
+    delete g->timer;
+  }
   delete g;
   return VX_OK;
 }
@@ -1374,9 +1471,21 @@ extern "C" vx_status vx_dag_profile_enable(vx_dag* g, int32_t enable) {
   if (enable && !g->profile) {
     for (auto& e : g->ev)
       if (!e && hipEventCreate(&e) != hipSuccess) return VX_DEVICE_ERROR;
+    if (!g->timer) {
+      g->timer = new (std::nothrow) KernelTimer();
+      if (!g->timer) return VX_DEVICE_ERROR;
+      for (auto& e : g->timer->ev)
+        if (hipEventCreate(&e) != hipSuccess) return VX_DEVICE_ERROR;
+    }
   }
   g->profile = enable != 0;
   g->prof = vx_dag_profile{};
+  return VX_OK;
+}
+
+extern "C" vx_status vx_dag_profile_kernel_mask(vx_dag* g, uint64_t mask) {
+  if (!g || !g->timer) return VX_INVALID_VALUE;  // enable profiling first
+  g->timer->mask = mask;
   return VX_OK;
 }
 

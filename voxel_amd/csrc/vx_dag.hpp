@@ -192,8 +192,10 @@ struct vx_dag {
   uint64_t* node_slots = nullptr;
   uint64_t* leaf_slots = nullptr;
   uint32_t* ctl = nullptr;
-  // Optional per-stage timing of edits (hipEvents on the edit's stream).
+  // Optional per-stage and per-kernel timing of edits (hipEvents on the
+  // edit's stream).
   bool profile = false;
   hipEvent_t ev[VX_DAG_STAGES + 1] = {};
+  vx::KernelTimer* timer = nullptr;
   vx_dag_profile prof{};
 };

@@ -22,8 +22,14 @@ __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
 vx_status fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
   if (n == 0) return VX_OK;
   const uint32_t blocks = uint32_t(std::min<uint64_t>(ceil_div(n, 256), 4096));
+  TimedLaunch tl(VX_K_FILL, s);
   fill_u32_kernel<<<blocks, 256, 0, s>>>(p, n, v);
   return last_launch();
+}
+
+KernelTimer*& current_timer() {
+  static thread_local KernelTimer* t = nullptr;
+  return t;
 }
 
 namespace scan {

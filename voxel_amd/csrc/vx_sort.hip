@@ -52,7 +52,7 @@ __device__ __forceinline__ uint32_t count_of(const uint32_t* n_ptr, uint32_t n_a
 // ---- pass 0: histograms of every digit position in one read ----------------
 template <class K, uint32_t RB>
 __global__ __launch_bounds__(256) void histogram_kernel(const K* __restrict__ keys, const uint32_t* n_ptr,
-                                                        uint32_t n_arg, uint32_t key_bits,
+                                                        uint32_t n_arg, uint32_t begin_bit, uint32_t key_bits,
                                                         uint32_t* __restrict__ hist) {
   constexpr uint32_t R = 1u << RB;
   constexpr uint32_t MAXP = (sizeof(K) * 8 + RB - 1) / RB;
@@ -67,9 +67,8 @@ __global__ __launch_bounds__(256) void histogram_kernel(const K* __restrict__ ke
 #pragma unroll
     for (uint32_t p = 0; p < MAXP; ++p) {
       if (p < passes) {
-        const uint32_t shift = p * RB;
-        const uint32_t bits = min(RB, key_bits - shift);
-        atomicAdd(&s[p * R + digit_of(key, shift, (1u << bits) - 1)], 1u);
+        const uint32_t bits = min(RB, key_bits - p * RB);
+        atomicAdd(&s[p * R + digit_of(key, begin_bit + p * RB, (1u << bits) - 1)], 1u);
       }
     }
   }
@@ -78,11 +77,12 @@ __global__ __launch_bounds__(256) void histogram_kernel(const K* __restrict__ ke
     if (s[i]) atomicAdd(&hist[i], s[i]);
 }
 
-// Exclusive scan of each pass's R counters (one block per pass).
+// Exclusive scan of each pass's R counters (one block of max(R, 256)
+// threads per pass).
 template <uint32_t RB>
-__global__ __launch_bounds__(256) void hist_scan_kernel(uint32_t* hist) {
+__global__ __launch_bounds__((1u << RB) > 256 ? (1u << RB) : 256) void hist_scan_kernel(uint32_t* hist) {
   constexpr uint32_t R = 1u << RB;
-  __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_wave[(R > 256 ? R : 256) / 64];
   uint32_t* h = hist + blockIdx.x * R;
   const uint32_t t = threadIdx.x;
   const uint32_t v = t < R ? h[t] : 0u;
@@ -251,7 +251,10 @@ struct Tuning {
   uint32_t rb, kpt, block;
 };
 std::mutex g_mu;
-Tuning g_tuning[2] = {{8, 15, 256}, {8, 15, 256}};  // [u32 keys, u64 keys]
+// Rows: u32 pairs, u64 pairs, u32 keys only, u64 keys only.
+Tuning g_tuning[4] = {{8, 15, 256}, {8, 15, 256}, {8, 15, 256}, {9, 8, 1024}};
+
+uint32_t tuning_row(size_t key_bytes, bool vals) { return (key_bytes == 8 ? 1u : 0u) + (vals ? 0u : 2u); }
 
 template <class K, bool VALS>
 struct Launcher {
@@ -317,17 +320,28 @@ const DeviceLimits& limits() {
   return cached;
 }
 
+// 9-bit digits (4 passes instead of 5 for 33..36-bit keys, 5 instead of 6 for
+// 41..45): 512 digits need a block of at least 512 threads.
+template <class K, bool VALS>
+bool pick9(Launcher<K, VALS>& L, uint32_t block, uint32_t kpt) {
+  if (block == 512 && kpt == 8) return pick3<K, VALS, 9, 512, 8>(L);
+  if (block == 1024 && kpt == 8) return pick3<K, VALS, 9, 1024, 8>(L);
+  if (block == 512 && kpt == 15) return pick3<K, VALS, 9, 512, 15>(L);
+  return false;
+}
+
 template <class K, bool VALS>
 bool pick(Launcher<K, VALS>& L, const Tuning& tn) {
   bool ok = false;
   if (tn.rb == 8) ok = pick1<K, VALS, 8>(L, tn.block, tn.kpt);
   else if (tn.rb == 4) ok = pick1<K, VALS, 4>(L, tn.block, tn.kpt);
+  else if (tn.rb == 9) ok = pick9<K, VALS>(L, tn.block, tn.kpt);
   return ok && L.lds <= limits().max_lds;
 }
 
-Tuning tuning_for(size_t key_bytes) {
+Tuning tuning_for(size_t key_bytes, bool vals = true) {
   std::lock_guard<std::mutex> g(g_mu);
-  return g_tuning[key_bytes == 8 ? 1 : 0];
+  return g_tuning[tuning_row(key_bytes, vals)];
 }
 
 struct Layout {
@@ -360,19 +374,27 @@ Layout layout(void* ws, uint64_t n, size_t key_bytes, uint32_t key_bits, bool va
 // buffer; *in_alt reports where the result ended up.
 template <class K, bool VALS>
 vx_status drive(K* keys, uint32_t* vals, K* alt_keys, uint32_t* alt_vals, const uint32_t* n_ptr, uint32_t n,
-                uint32_t key_bits, const Layout& lay, const Launcher<K, VALS>& L, const Tuning& tn, bool* in_alt,
-                hipStream_t s) {
+                uint32_t begin_bit, uint32_t key_bits, const Layout& lay, const Launcher<K, VALS>& L, const Tuning& tn,
+                bool* in_alt, hipStream_t s) {
   const uint32_t R = 1u << tn.rb;
   // hist, counters and rec0 are contiguous: clear them with one fill.
   const size_t clear_words = (reinterpret_cast<char*>(lay.rec0 + size_t(lay.tiles) * R) - reinterpret_cast<char*>(lay.hist)) / 4;
   vx_status st = fill_u32(lay.hist, clear_words, 0, s);
   if (st != VX_OK) return st;
   const uint32_t hblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(n, 256 * 16), 2048));
-  if (tn.rb == 8) histogram_kernel<K, 8><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, key_bits, lay.hist);
-  else histogram_kernel<K, 4><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, key_bits, lay.hist);
+  {
+    TimedLaunch tl(VX_K_SORT_HISTOGRAM, s);
+    if (tn.rb == 8) histogram_kernel<K, 8><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
+    else if (tn.rb == 9) histogram_kernel<K, 9><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
+    else histogram_kernel<K, 4><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
+  }
   if ((st = last_launch()) != VX_OK) return st;
-  if (tn.rb == 8) hist_scan_kernel<8><<<lay.passes, 256, 0, s>>>(lay.hist);
-  else hist_scan_kernel<4><<<lay.passes, 256, 0, s>>>(lay.hist);
+  {
+    TimedLaunch tl(VX_K_SORT_SCAN, s);
+    if (tn.rb == 8) hist_scan_kernel<8><<<lay.passes, 256, 0, s>>>(lay.hist);
+    else if (tn.rb == 9) hist_scan_kernel<9><<<lay.passes, 512, 0, s>>>(lay.hist);
+    else hist_scan_kernel<4><<<lay.passes, 256, 0, s>>>(lay.hist);
+  }
   if ((st = last_launch()) != VX_OK) return st;
 
   hipFuncSetAttribute(reinterpret_cast<const void*>(L.kernel), hipFuncAttributeMaxDynamicSharedMemorySize, int(L.lds));
@@ -384,12 +406,15 @@ vx_status drive(K* keys, uint32_t* vals, K* alt_keys, uint32_t* alt_vals, const 
   uint32_t* vsrc = vals;
   uint32_t* vdst = alt_vals;
   for (uint32_t p = 0; p < lay.passes; ++p) {
-    const uint32_t shift = p * tn.rb;
-    const uint32_t bits = std::min(tn.rb, key_bits - shift);
+    const uint32_t shift = begin_bit + p * tn.rb;
+    const uint32_t bits = std::min(tn.rb, key_bits - p * tn.rb);
     uint32_t* rec = (p & 1) ? lay.rec1 : lay.rec0;
     uint32_t* rec_next = (p & 1) ? lay.rec0 : lay.rec1;
-    L.kernel<<<grid, L.block, L.lds, s>>>(src, dst, vsrc, vdst, n_ptr, n, shift, (1u << bits) - 1,
-                                          lay.hist + size_t(p) * R, lay.counters + p, rec, rec_next);
+    {
+      TimedLaunch tl(VX_K_SORT_PARTITION, s);
+      L.kernel<<<grid, L.block, L.lds, s>>>(src, dst, vsrc, vdst, n_ptr, n, shift, (1u << bits) - 1,
+                                            lay.hist + size_t(p) * R, lay.counters + p, rec, rec_next);
+    }
     if ((st = last_launch()) != VX_OK) return st;
     std::swap(src, dst);
     std::swap(vsrc, vdst);
@@ -401,7 +426,7 @@ vx_status drive(K* keys, uint32_t* vals, K* alt_keys, uint32_t* alt_vals, const 
 template <class K, bool VALS>
 size_t ws_bytes(uint32_t n, uint32_t key_bits) {
   if (n <= 1) return 0;
-  const Tuning tn = tuning_for(sizeof(K));
+  const Tuning tn = tuning_for(sizeof(K), VALS);
   Launcher<K, VALS> L;
   if (!pick(L, tn)) return 0;
   return layout(nullptr, n, sizeof(K), key_bits, VALS, tn, L.tile, true).bytes;
@@ -410,14 +435,14 @@ size_t ws_bytes(uint32_t n, uint32_t key_bits) {
 template <class K, bool VALS>
 vx_status run(K* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, void* ws, size_t ws_size, hipStream_t s) {
   if (n <= 1) return VX_OK;
-  const Tuning tn = tuning_for(sizeof(K));
+  const Tuning tn = tuning_for(sizeof(K), VALS);
   Launcher<K, VALS> L;
   if (!pick(L, tn)) return VX_UNSUPPORTED;
   const Layout lay = layout(ws, n, sizeof(K), key_bits, VALS, tn, L.tile, true);
   if (!ws || ws_size < lay.bytes) return VX_INVALID_VALUE;
   bool in_alt = false;
-  vx_status st = drive<K, VALS>(keys, vals, static_cast<K*>(lay.alt_keys), lay.alt_vals, nullptr, n, key_bits, lay, L,
-                                tn, &in_alt, s);
+  vx_status st = drive<K, VALS>(keys, vals, static_cast<K*>(lay.alt_keys), lay.alt_vals, nullptr, n, 0, key_bits, lay,
+                                L, tn, &in_alt, s);
   if (st != VX_OK) return st;
   if (in_alt) {
     const uint32_t blocks = std::min<uint32_t>(ceil_div(n, 256), 8192);
@@ -448,6 +473,11 @@ vx_status sort_u64(uint64_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits
               : run<uint64_t, false>(keys, nullptr, n, key_bits, ws, ws_size, s);
 }
 
+uint32_t pairs_u64_dev_passes(uint32_t key_bits) {
+  const Tuning tn = tuning_for(8);
+  return (key_bits + tn.rb - 1) / tn.rb;
+}
+
 size_t pairs_u64_dev_workspace_bytes(uint64_t n_upper, uint32_t key_bits) {
   const Tuning tn = tuning_for(8);
   Launcher<uint64_t, true> L;
@@ -462,31 +492,73 @@ vx_status pairs_u64_dev(uint64_t* keys, uint32_t* vals, uint64_t* alt_keys, uint
   if (!pick(L, tn)) return VX_UNSUPPORTED;
   if (n_upper >= (1u << 30)) return VX_INVALID_VALUE;
   const Layout lay = layout(ws, n_upper, 8, key_bits, true, tn, L.tile, false);
-  return drive<uint64_t, true>(keys, vals, alt_keys, alt_vals, n_ptr, uint32_t(n_upper), key_bits, lay, L, tn, in_alt,
-                               s);
+  return drive<uint64_t, true>(keys, vals, alt_keys, alt_vals, n_ptr, uint32_t(n_upper), 0, key_bits, lay, L, tn,
+                               in_alt, s);
+}
+
+// Packed u64 keys sorted by bits [begin_bit, begin_bit + key_bits): the
+// u64 keys-only tuning row; 8-bit digits where 9-bit ones save no pass.
+Tuning packed_tuning(uint32_t key_bits) {
+  Tuning tn = tuning_for(8, false);
+  if (tn.rb == 9 && (key_bits + 8) / 9 == (key_bits + 7) / 8) {
+    Launcher<uint64_t, false> L;
+    const Tuning t8{8, tn.kpt, tn.block};
+    if (pick(L, t8)) tn = t8;
+  }
+  Launcher<uint64_t, false> L;
+  if (!pick(L, tn)) tn = Tuning{8, 8, 1024};
+  return tn;
+}
+
+uint32_t keys_u64_dev_passes(uint32_t key_bits) {
+  const Tuning tn = packed_tuning(key_bits);
+  return (key_bits + tn.rb - 1) / tn.rb;
+}
+
+size_t keys_u64_dev_workspace_bytes(uint64_t n_upper, uint32_t key_bits) {
+  const Tuning tn = packed_tuning(key_bits);
+  Launcher<uint64_t, false> L;
+  if (!pick(L, tn)) return 0;
+  return layout(nullptr, n_upper, 8, key_bits, false, tn, L.tile, false).bytes;
+}
+
+vx_status keys_u64_dev(uint64_t* keys, uint64_t* alt_keys, const uint32_t* n_ptr, uint64_t n_upper, uint32_t begin_bit,
+                       uint32_t key_bits, void* ws, bool* in_alt, hipStream_t s) {
+  const Tuning tn = packed_tuning(key_bits);
+  Launcher<uint64_t, false> L;
+  if (!pick(L, tn)) return VX_UNSUPPORTED;
+  if (n_upper >= (1u << 30) || begin_bit + key_bits > 64) return VX_INVALID_VALUE;
+  const Layout lay = layout(ws, n_upper, 8, key_bits, false, tn, L.tile, false);
+  return drive<uint64_t, false>(keys, nullptr, alt_keys, nullptr, n_ptr, uint32_t(n_upper), begin_bit, key_bits, lay,
+                                L, tn, in_alt, s);
 }
 
 vx_status set_tuning(uint32_t key_bytes, uint32_t rb, uint32_t kpt, uint32_t block) {
   const Tuning tn{rb, kpt, block};
+  const bool vals = !(key_bytes & VX_SORT_KEYS_ONLY);
+  const uint32_t kb = key_bytes & ~VX_SORT_KEYS_ONLY;
   bool ok;
-  if (key_bytes == 8) {
-    Launcher<uint64_t, true> L;
-    ok = pick(L, tn);
-  } else if (key_bytes == 4) {
-    Launcher<uint32_t, true> L;
-    ok = pick(L, tn);
+  if (kb == 8) {
+    Launcher<uint64_t, true> Lv;
+    Launcher<uint64_t, false> Lk;
+    ok = vals ? pick(Lv, tn) : pick(Lk, tn);
+  } else if (kb == 4) {
+    Launcher<uint32_t, true> Lv;
+    Launcher<uint32_t, false> Lk;
+    ok = vals ? pick(Lv, tn) : pick(Lk, tn);
   } else {
     return VX_INVALID_VALUE;
   }
   if (!ok) return VX_UNSUPPORTED;
   std::lock_guard<std::mutex> g(g_mu);
-  g_tuning[key_bytes == 8 ? 1 : 0] = tn;
+  g_tuning[tuning_row(kb, vals)] = tn;
   return VX_OK;
 }
 
 vx_status get_tuning(uint32_t key_bytes, uint32_t* rb, uint32_t* kpt, uint32_t* block) {
-  if ((key_bytes != 4 && key_bytes != 8) || !rb || !kpt || !block) return VX_INVALID_VALUE;
-  const Tuning tn = tuning_for(key_bytes);
+  const uint32_t kb = key_bytes & ~VX_SORT_KEYS_ONLY;
+  if ((kb != 4 && kb != 8) || !rb || !kpt || !block) return VX_INVALID_VALUE;
+  const Tuning tn = tuning_for(kb, !(key_bytes & VX_SORT_KEYS_ONLY));
   *rb = tn.rb;
   *kpt = tn.kpt;
   *block = tn.block;

@@ -65,14 +65,21 @@ register({
     "vx_batcher_run": (_i32, [_P, _P, _P, _P, _u32, C.POINTER(_u64)]),
     "vx_dag_profile_enable": (_i32, [_P, _i32]),
     "vx_dag_profile_read": (_i32, [_P, _P]),
+    "vx_dag_profile_kernel_mask": (_i32, [_P, _u64]),
 })
 
 STAGES = ("records", "sort", "levels", "old_refs", "leaves", "nodes", "publish")
 
 
+KERNELS = ("records", "sort_histogram", "sort_scan", "sort_partition", "sort_copy", "runs", "up", "topdown",
+           "leaf_build", "leaf_probe", "leaf_commit", "leaf_place", "node_build", "node_probe", "node_commit",
+           "node_place", "publish", "fill", "other")
+
+
 class Profile(C.Structure):
     _fields_ = [("ms", C.c_double * 7), ("edits", C.c_uint64), ("records", C.c_uint64), ("leaves", C.c_uint64),
-                ("nodes", C.c_uint64), ("sort_passes", C.c_uint64)]
+                ("nodes", C.c_uint64), ("sort_passes", C.c_uint64), ("sort_record_bytes", C.c_uint64),
+                ("kernel_launches", C.c_uint64 * len(KERNELS)), ("kernel_ms", C.c_double * len(KERNELS))]
 
 
 STREAM_BRUSH = 5
@@ -217,6 +224,11 @@ class HashDagGpu:
 
     def profile(self, enable: bool = True):
         check(lib().vx_dag_profile_enable(self.h, int(enable)), "vx_dag_profile_enable")
+
+    def profile_kernels(self, kinds=None):
+        """Time only these kernel kinds (names from KERNELS; None = all)."""
+        mask = (1 << 64) - 1 if kinds is None else sum(1 << KERNELS.index(k) for k in kinds)
+        check(lib().vx_dag_profile_kernel_mask(self.h, mask), "vx_dag_profile_kernel_mask")
 
     def profile_read(self) -> Profile:
         p = Profile()
