@@ -332,6 +332,9 @@ enum {
   VX_K_PUBLISH,
   VX_K_FILL,
   VX_K_OTHER,
+  VX_K_TAIL_UP,      /* single-workgroup kernels for the small top heights */
+  VX_K_TAIL_TOPDOWN,
+  VX_K_TAIL_NODES,
   VX_DAG_KERNELS
 };
 typedef struct vx_dag_profile {

@@ -137,17 +137,13 @@ __device__ __forceinline__ void store_item(uint32_t* p, uint32_t n, const uint32
     if (k < n) p[k] = w[k];
 }
 
+// Probe of entry i (PENDING entries only): find an equal item in the table or
+// in this batch, or claim an empty slot as the representative.
 template <uint32_t kMax>
-__global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, const uint32_t* n_ptr, uint64_t* res,
-                             Scenes sc, uint32_t n_scenes, bool leaves) {
-  if (blockIdx.x == 0)
-    for (uint32_t s = threadIdx.x; s < n_scenes; s += blockDim.x) {
-      const Table t = table_of(sc.at(s), leaves);
-      *t.base = *t.used;
-    }
-  const uint32_t n = *n_ptr;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (res_kind(res[i]) != kResPending) continue;
+__device__ __forceinline__ void probe_entry(const ItemSet& set, const uint64_t* __restrict__ hashes, uint64_t* res,
+                                            const Scenes& sc, bool leaves, uint32_t i) {
+  if (res_kind(res[i]) != kResPending) return;
+  {
     const Dev& d = sc.of(i);
     const Table t = table_of(d, leaves);
     const uint32_t* it = set.items + size_t(i) * set.stride;
@@ -201,6 +197,19 @@ __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, c
     if (!done) set_status(d.ctl, kStatusOverflow);
     res[i] = out;
   }
+}
+
+template <uint32_t kMax>
+__global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, const uint32_t* n_ptr, uint64_t* res,
+                             Scenes sc, uint32_t n_scenes, bool leaves) {
+  if (blockIdx.x == 0)
+    for (uint32_t s = threadIdx.x; s < n_scenes; s += blockDim.x) {
+      const Table t = table_of(sc.at(s), leaves);
+      *t.base = *t.used;
+    }
+  const uint32_t n = *n_ptr;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    probe_entry<kMax>(set, hashes, res, sc, leaves, i);
 }
 
 // Commit, part 1: segmented scan (one segment per scene) of representative
@@ -270,23 +279,28 @@ struct CommitStore {
 // to the pool, publishes its slot and becomes FINAL. Independent entries, so
 // the item loads of many entries are in flight at once.
 template <uint32_t kMax>
+__device__ __forceinline__ void place_entry(const ItemSet& set, const uint64_t* __restrict__ slot_vals, uint64_t* res,
+                                            const Scenes& sc, bool leaves, uint32_t i) {
+  const uint64_t r = res[i];
+  if (res_kind(r) != kResRep) return;
+  const Table t = table_of(sc.of(i), leaves);
+  const uint64_t sv = slot_vals[i];
+  const uint32_t off = uint32_t(sv);
+  const uint32_t* it = set.items + size_t(i) * set.stride;
+  const uint32_t w = set.width(it);
+  uint32_t words[kMax];
+  load_item<kMax>(it, w, words);
+  store_item<kMax>(t.pool + off, w, words);
+  st_agent(&t.slots[uint32_t(r)], sv);
+  res[i] = res_make(kResFinal, off);
+}
+
+template <uint32_t kMax>
 __global__ void place_kernel(ItemSet set, const uint64_t* __restrict__ slot_vals, const uint32_t* n_ptr,
                              uint64_t* res, Scenes sc, bool leaves) {
   const uint32_t n = *n_ptr;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint64_t r = res[i];
-    if (res_kind(r) != kResRep) continue;
-    const Table t = table_of(sc.of(i), leaves);
-    const uint64_t sv = slot_vals[i];
-    const uint32_t off = uint32_t(sv);
-    const uint32_t* it = set.items + size_t(i) * set.stride;
-    const uint32_t w = set.width(it);
-    uint32_t words[kMax];
-    load_item<kMax>(it, w, words);
-    store_item<kMax>(t.pool + off, w, words);
-    st_agent(&t.slots[uint32_t(r)], sv);
-    res[i] = res_make(kResFinal, off);
-  }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    place_entry<kMax>(set, slot_vals, res, sc, leaves, i);
 }
 
 template <uint32_t kMax>
@@ -331,13 +345,39 @@ vx_status intern(const Dev& d, bool leaves, ItemSet set, uint64_t* hashes, const
 }
 
 // ---- records ----------------------------------------------------------------
-struct BrushJob {
+// 80 bytes, 16-byte aligned: a job is read as five 16-byte loads.
+struct alignas(16) BrushJob {
   vx_brush b;
   int32_t bx0, by0, bz0;
   uint32_t nx, ny, nz;
   uint32_t start;
   uint32_t scene;
+  float inv_nx, inv_ny;  // 1/nx, 1/ny rounded to nearest (see divmod_small)
+  uint32_t pad[2];
 };
+static_assert(sizeof(BrushJob) == 80, "BrushJob layout");
+
+// a / d and a % d for a < 2^22 and d >= 1 from inv = float(1/d): a is exact
+// in float and the product carries a relative error below 2^-23, so it is
+// within 1/2 of a / d (< 2^22) and one correction step either way makes the
+// truncated quotient exact. Larger a take the integer division.
+__device__ __forceinline__ void divmod_small(uint32_t a, uint32_t d, float inv, uint32_t& q, uint32_t& r) {
+  if (a < (1u << 22)) {
+    q = uint32_t(float(a) * inv);
+    int32_t rem = int32_t(a) - int32_t(q * d);
+    if (rem < 0) {
+      --q;
+      rem += int32_t(d);
+    } else if (rem >= int32_t(d)) {
+      ++q;
+      rem -= int32_t(d);
+    }
+    r = uint32_t(rem);
+  } else {
+    q = a / d;
+    r = a - q * d;
+  }
+}
 
 struct Record {
   uint64_t key;
@@ -386,12 +426,14 @@ __device__ __noinline__ uint64_t sphere_mask64(int64_t dx0, int64_t dy0, int64_t
 
 // Box voxels of one brick in 64-bit arithmetic, for half extents of 2^30 and
 // more (out of line, like sphere_mask64).
-__device__ __noinline__ uint64_t box_mask64(const vx_brush& b, int32_t X, int32_t Y, int32_t Z) {
+// (Arguments by value: a reference would force the caller's brush copy into
+// scratch memory.)
+__device__ __noinline__ uint64_t box_mask64(int64_t dx, int64_t dy, int64_t dz, int64_t hx, int64_t hy, int64_t hz) {
   uint32_t xm = 0, ym = 0, zm = 0;
   for (int l = 0; l < 4; ++l) {
-    xm |= uint32_t(llabs(int64_t(X) + l - b.cx) <= b.hx) << l;
-    ym |= uint32_t(llabs(int64_t(Y) + l - b.cy) <= b.hy) << l;
-    zm |= uint32_t(llabs(int64_t(Z) + l - b.cz) <= b.hz) << l;
+    xm |= uint32_t(llabs(dx + l) <= hx) << l;
+    ym |= uint32_t(llabs(dy + l) <= hy) << l;
+    zm |= uint32_t(llabs(dz + l) <= hz) << l;
   }
   uint64_t m = 0;
   for (int lz = 0; lz < 4; ++lz)
@@ -410,7 +452,8 @@ __device__ __forceinline__ uint64_t brick_mask(const vx_brush& b, int32_t X, int
     if (b.hx < (1 << 14)) return sphere_mask32(X - b.cx, Y - b.cy, Z - b.cz, b.hx * b.hx);
     return sphere_mask64(int64_t(X) - b.cx, int64_t(Y) - b.cy, int64_t(Z) - b.cz, int64_t(b.hx) * b.hx);
   }
-  if (b.hx >= (1 << 30) || b.hy >= (1 << 30) || b.hz >= (1 << 30)) return box_mask64(b, X, Y, Z);
+  if (b.hx >= (1 << 30) || b.hy >= (1 << 30) || b.hz >= (1 << 30))
+    return box_mask64(int64_t(X) - b.cx, int64_t(Y) - b.cy, int64_t(Z) - b.cz, b.hx, b.hy, b.hz);
   const int32_t dx = X - b.cx, dy = Y - b.cy, dz = Z - b.cz;
   uint32_t xm = 0, ym = 0, zm = 0;
 #pragma unroll
@@ -435,20 +478,21 @@ __device__ __forceinline__ uint64_t brick_mask(const vx_brush& b, int32_t X, int
 // axis. For a sphere, the brick's lattice point nearest to the centre
 // (clamped axis by axis) minimises each squared term, so the brick is touched
 // iff that point is inside.
-__device__ __noinline__ bool sphere_touched64(const vx_brush& b, int32_t X, int32_t Y, int32_t Z) {
-  const int64_t c[3] = {b.cx, b.cy, b.cz};
+__device__ __noinline__ bool sphere_touched64(int32_t cx, int32_t cy, int32_t cz, int32_t r, int32_t X, int32_t Y,
+                                             int32_t Z) {
+  const int64_t c[3] = {cx, cy, cz};
   const int64_t lo[3] = {X, Y, Z};
   int64_t d2 = 0;
   for (int a = 0; a < 3; ++a) {
     const int64_t p = c[a] < lo[a] ? lo[a] : (c[a] > lo[a] + 3 ? lo[a] + 3 : c[a]);
     d2 += (p - c[a]) * (p - c[a]);
   }
-  return d2 <= int64_t(b.hx) * b.hx;
+  return d2 <= int64_t(r) * r;
 }
 
 __device__ __forceinline__ bool brick_touched(const vx_brush& b, int32_t X, int32_t Y, int32_t Z) {
   if (b.kind != 0) return true;
-  if (b.hx >= (1 << 14)) return sphere_touched64(b, X, Y, Z);
+  if (b.hx >= (1 << 14)) return sphere_touched64(b.cx, b.cy, b.cz, b.hx, X, Y, Z);
   const int32_t dx = min(max(X - b.cx, 0), X + 3 - b.cx) ;
   const int32_t dy = min(max(Y - b.cy, 0), Y + 3 - b.cy);
   const int32_t dz = min(max(Z - b.cz, 0), Z + 3 - b.cz);
@@ -492,28 +536,34 @@ struct BrushCands {
   const uint32_t* n_jobs;  // device count
   bool materials;
   Codec codec;
-  // The lanes of a wave hold consecutive candidates: one wave-uniform binary
-  // search (scalar loads) for the lowest active lane, then each lane steps
-  // forward over the few jobs that start inside its wave's window.
-  __device__ __forceinline__ uint32_t job_of(uint32_t i) const {
-    const uint32_t nj = *n_jobs;
-    const uint32_t i0 = __builtin_amdgcn_readfirstlane(i);
-    uint32_t lo = 0, hi = nj;  // last job with start <= i0
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (jobs[mid].start <= i0) lo = mid;
-      else hi = mid;
+  // The job of candidate i: the last job whose start is <= i. The lanes of a
+  // wave hold consecutive candidates: one wave-uniform binary search (scalar
+  // loads) for the lowest active lane, then each lane steps forward over the
+  // few jobs that start inside its wave's window. With a hint (the job of an
+  // earlier candidate of the same lane) the search is skipped.
+  __device__ __forceinline__ uint32_t job_of(uint32_t i, uint32_t nj, uint32_t hint) const {
+    uint32_t lo = hint;
+    if (hint == kNone) {
+      const uint32_t i0 = __builtin_amdgcn_readfirstlane(i);
+      uint32_t hi = nj;
+      lo = 0;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].start <= i0) lo = mid;
+        else hi = mid;
+      }
     }
     while (lo + 1 < nj && jobs[lo + 1].start <= i) ++lo;
     return lo;
   }
   __device__ __forceinline__ void brick_of(const BrushJob& j, uint32_t i, uint32_t& gx, uint32_t& gy,
                                            uint32_t& gz) const {
-    const uint32_t local = i - j.start;
-    const uint32_t row = local / j.nx;
-    gx = j.bx0 + (local - row * j.nx);
-    gy = j.by0 + row % j.ny;
-    gz = j.bz0 + row / j.ny;
+    uint32_t row, bx, by, bz;
+    divmod_small(i - j.start, j.nx, j.inv_nx, row, bx);
+    divmod_small(row, j.ny, j.inv_ny, bz, by);
+    gx = j.bx0 + bx;
+    gy = j.by0 + by;
+    gz = j.bz0 + bz;
   }
 };
 
@@ -525,13 +575,19 @@ struct BrickRef {
   uint32_t gz;
 };
 
-// Load half: the touched test only.
+// Load half: the touched test only. A thread's candidates increase from one
+// call to the next, so the previous job is the next call's search hint (the
+// scan calls Load on a per-thread copy).
 struct BrushLoad {
   using Payload = BrickRef;
   BrushCands c;
-  __device__ uint32_t operator()(uint32_t i, BrickRef& r) const {
-    r.job = c.job_of(i);
-    const BrushJob& j = c.jobs[r.job];
+  uint32_t hint = kNone;
+  uint32_t nj = 0;
+  __device__ uint32_t operator()(uint32_t i, BrickRef& r) {
+    if (hint == kNone) nj = *c.n_jobs;
+    r.job = c.job_of(i, nj, hint);
+    hint = r.job;
+    const BrushJob j = c.jobs[r.job];
     uint32_t gx, gy, gz;
     c.brick_of(j, i, gx, gy, gz);
     r.gxy = gx | (gy << 16);
@@ -548,9 +604,10 @@ struct BrushStore {
   __device__ void operator()(uint32_t i, const BrickRef& r, uint32_t v, uint32_t excl, uint32_t incl,
                              uint32_t n) const {
     if (v) {
-      const vx_brush& b = c.jobs[r.job].b;
+      const BrushJob j = c.jobs[r.job];
+      const vx_brush& b = j.b;
       const uint32_t gx = r.gxy & 0xffffu, gy = r.gxy >> 16, gz = r.gz;
-      out.put_key(excl, (uint64_t(c.jobs[r.job].scene) << c.codec.shift) | morton3(gx, gy, gz));
+      out.put_key(excl, (uint64_t(j.scene) << c.codec.shift) | morton3(gx, gy, gz));
       out.mask[excl] = brick_mask(b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4));
       out.mat[excl] = c.materials ? b.material : (b.material ? 1u : 0u);
     }
@@ -657,14 +714,20 @@ struct UpStore {
   }
 };
 
+// The ref entry j of height h had before the edit: its scene's root at the
+// root height, else the matching child of its parent's old node.
+__device__ __forceinline__ void topdown_entry(const Scenes& sc, const uint32_t* __restrict__ parent,
+                                              const uint32_t* __restrict__ old_up, uint32_t* __restrict__ old,
+                                              uint32_t h, uint32_t j) {
+  const Dev& d = sc.of(j);
+  old[j] = d.depth == h ? d.ctl[kCtlRoot] : child_of(d.node_pool, old_up[parent[j]], uint32_t(sc.keys[j] & 7u));
+}
+
 __global__ void topdown_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ parent,
                                const uint32_t* __restrict__ old_up, uint32_t* __restrict__ old, uint32_t h) {
   const uint32_t n = *n_ptr;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    const Dev& d = sc.of(j);
-    old[j] = d.depth == h ? d.ctl[kCtlRoot]
-                          : child_of(d.node_pool, old_up[parent[j]], uint32_t(sc.keys[j] & 7u));
-  }
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    topdown_entry(sc, parent, old_up, old, h, j);
 }
 
 // ---- rebuild ---------------------------------------------------------------
@@ -732,28 +795,37 @@ __global__ void leaf_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32
 
 // Node at height h: old item plus the changed children (entries of height
 // h - 1 whose parent is this entry).
+// Node item of entry j: the old node's children, overwritten by the new refs
+// of the edited children (the entries of the height below whose parent is j).
+__device__ __forceinline__ void node_build_entry(const Scenes& sc, const uint32_t* __restrict__ old,
+                                                 const uint32_t* __restrict__ first, const uint64_t* __restrict__ ckeys,
+                                                 const uint32_t* __restrict__ cparent, const uint64_t* __restrict__ cres,
+                                                 uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
+                                                 uint64_t* __restrict__ res, uint32_t j) {
+  uint32_t ch[8];
+  children_of(sc.of(j).node_pool, old[j], ch);
+  const uint32_t k1 = first[j + 1];
+  for (uint32_t k = first[j]; k < k1; ++k)
+    if (cparent[k] == j) ch[uint32_t(ckeys[k] & 7u)] = resolve(cres, k);
+  uint32_t* it = items + size_t(j) * kNodeStride;
+  uint32_t ref = 0;
+  const uint32_t w = make_node_item(ch, it, &ref);
+  if (!w) {
+    res[j] = res_make(kResFinal, ref);
+  } else {
+    hashes[j] = hash_words(it, w);
+    res[j] = res_make(kResPending, 0);
+  }
+}
+
 __global__ void node_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ old,
                                   const uint32_t* __restrict__ first, const uint64_t* __restrict__ ckeys,
                                   const uint32_t* __restrict__ cparent, const uint64_t* __restrict__ cres,
                                   uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
                                   uint64_t* __restrict__ res) {
   const uint32_t n = *n_ptr;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    uint32_t ch[8];
-    children_of(sc.of(j).node_pool, old[j], ch);
-    const uint32_t k1 = first[j + 1];
-    for (uint32_t k = first[j]; k < k1; ++k)
-      if (cparent[k] == j) ch[uint32_t(ckeys[k] & 7u)] = resolve(cres, k);
-    uint32_t* it = items + size_t(j) * kNodeStride;
-    uint32_t ref = 0;
-    const uint32_t w = make_node_item(ch, it, &ref);
-    if (!w) {
-      res[j] = res_make(kResFinal, ref);
-    } else {
-      hashes[j] = hash_words(it, w);
-      res[j] = res_make(kResPending, 0);
-    }
-  }
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    node_build_entry(sc, old, first, ckeys, cparent, cres, items, hashes, res, j);
 }
 
 struct ResByHeight {
@@ -806,6 +878,87 @@ struct EditWs {
   Dev* scenes;
   size_t bytes;
 };
+
+// ---- tail: the small top heights in one workgroup ---------------------------
+// Heights with at most one entry per thread (capacity <= 1024) are processed
+// by a single 1024-thread workgroup that walks them in order with a barrier
+// between steps, instead of one launch per step and height: near the root
+// every launch would handle a handful of entries and its fixed cost (launch,
+// first memory round trip, drain) would dominate. (With several entries per
+// thread the dependent loads of each entry would run back to back, so larger
+// heights stay on the grid kernels.) Global writes of one step are visible
+// to the whole workgroup after the barrier.
+constexpr uint64_t kTailCap = scan::kFusedBlock;
+
+struct TailArgs {
+  Heights hs;
+  Scenes sc;  // keys set per height
+  Codec codec;
+  uint32_t* cnt;
+  uint32_t* root_pos;
+  uint32_t* items;
+  uint64_t* hashes;
+  uint32_t n_scenes;
+  uint32_t h_lo, h_hi;  // heights [h_lo, h_hi]
+};
+
+__device__ __forceinline__ uint32_t tail_count(const TailArgs& a, uint32_t h) {
+  return min(a.cnt[1 + h], uint32_t(scan::kFusedTile));
+}
+
+// Parents of heights h_lo..h_hi (the UpLoad / UpStore scan, one tile each).
+__global__ __launch_bounds__(scan::kFusedBlock) void tail_up_kernel(TailArgs a) {
+  __shared__ uint32_t s_wave[scan::kFusedBlock / kWave];
+  __shared__ uint32_t s_prefix;
+  for (uint32_t h = a.h_lo; h <= a.h_hi; ++h) {
+    Scenes sh = a.sc;
+    sh.keys = a.hs.keys[h];
+    scan::fused_tile<scan::SumOp>(0u, tail_count(a, h), false, UpLoad{a.hs.keys[h], sh, h},
+                                  UpStore{a.codec, a.hs.keys[h + 1], a.hs.first[h + 1], a.hs.parent[h], &a.cnt[2 + h],
+                                          a.root_pos},
+                                  nullptr, s_wave, &s_prefix);
+    __syncthreads();
+  }
+}
+
+// Old refs of heights h_hi down to h_lo.
+__global__ __launch_bounds__(scan::kFusedBlock) void tail_topdown_kernel(TailArgs a) {
+  for (uint32_t h = a.h_hi + 1; h-- > a.h_lo;) {
+    Scenes sh = a.sc;
+    sh.keys = a.hs.keys[h];
+    const uint32_t n = tail_count(a, h);
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) topdown_entry(sh, a.hs.parent[h], a.hs.old[h + 1], a.hs.old[h], h, j);
+    __syncthreads();
+  }
+}
+
+// Node items of heights h_lo..h_hi: build, probe, commit, place.
+__global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs a) {
+  __shared__ uint64_t s_wave[scan::kFusedBlock / kWave];
+  __shared__ uint64_t s_prefix;
+  const ItemSet set{a.items, kNodeStride, 0};
+  for (uint32_t h = a.h_lo; h <= a.h_hi; ++h) {
+    Scenes sh = a.sc;
+    sh.keys = a.hs.keys[h];
+    uint64_t* res = a.hs.res[h];
+    const uint32_t n = tail_count(a, h);
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x)
+      node_build_entry(sh, a.hs.old[h], a.hs.first[h], a.hs.keys[h - 1], a.hs.parent[h - 1], a.hs.res[h - 1], a.items,
+                       a.hashes, res, j);
+    for (uint32_t sidx = threadIdx.x; sidx < a.n_scenes; sidx += blockDim.x) {
+      const Table t = table_of(sh.at(sidx), false);
+      *t.base = *t.used;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) probe_entry<kNodeStride>(set, a.hashes, res, sh, false, i);
+    __syncthreads();
+    scan::fused_tile<scan::SegSumOp>(0u, n, false, CommitLoad{res, a.hashes, &a.cnt[1 + h], set, sh, false},
+                                     CommitStore{res, a.hashes, sh, false}, nullptr, s_wave, &s_prefix);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) place_entry<kNodeStride>(set, a.hashes, res, sh, false, i);
+    __syncthreads();
+  }
+}
 
 struct Plan {
   uint32_t n_scenes = 1;
@@ -913,7 +1066,15 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
                                      s, false);
   }
   if (st != VX_OK) return st;
-  for (uint32_t h = 0; h <= H; ++h) {
+  // Heights from hu up go to the single-workgroup tail kernels.
+  uint32_t hu = H + 1;
+  for (uint32_t h = 1; h <= H; ++h)
+    if (L.cap[h] <= kTailCap) {
+      hu = h;
+      break;
+    }
+  const TailArgs ta{L, sc, p.codec, cnt, w.root_pos, w.items, w.hashes, p.n_scenes, hu, H};
+  for (uint32_t h = 0; h < hu; ++h) {
     Scenes sh = sc;
     sh.keys = L.keys[h];
     {
@@ -925,8 +1086,18 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
     }
     if (st != VX_OK) return st;
   }
+  if (hu <= H) {
+    TimedLaunch tl(VX_K_TAIL_UP, s);
+    tail_up_kernel<<<1, scan::kFusedBlock, 0, s>>>(ta);
+  }
+  if ((st = last_launch()) != VX_OK) return st;
   mark(g, 3, s);
-  for (uint32_t h = H + 1; h-- > 0;) {
+  if (hu <= H) {
+    TimedLaunch tl(VX_K_TAIL_TOPDOWN, s);
+    tail_topdown_kernel<<<1, scan::kFusedBlock, 0, s>>>(ta);
+  }
+  if ((st = last_launch()) != VX_OK) return st;
+  for (uint32_t h = hu; h-- > 0;) {
     Scenes sh = sc;
     sh.keys = L.keys[h];
     {
@@ -954,7 +1125,7 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
       return st;
   }
   mark(g, 5, s);
-  for (uint32_t h = 1; h <= H; ++h) {
+  for (uint32_t h = 1; h < hu; ++h) {
     Scenes sh = sc;
     sh.keys = L.keys[h];
     {
@@ -967,6 +1138,11 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
                             L.res[h], w.commit_ws[h], s, false)) != VX_OK)
       return st;
   }
+  if (hu <= H) {
+    TimedLaunch tl(VX_K_TAIL_NODES, s);
+    tail_nodes_kernel<<<1, scan::kFusedBlock, 0, s>>>(ta);
+  }
+  if ((st = last_launch()) != VX_OK) return st;
   mark(g, 6, s);
   ResByHeight rh{};
   for (uint32_t h = 0; h <= H && h < kMaxHeights; ++h) rh.p[h] = L.res[h];
@@ -1007,6 +1183,8 @@ uint64_t make_jobs(const vx_brush* b, uint32_t n, uint32_t world, uint32_t scene
     j.nz = uint32_t(cnt[2]);
     j.start = uint32_t(total);
     j.scene = scene;
+    j.inv_nx = float(1.0 / double(j.nx));
+    j.inv_ny = float(1.0 / double(j.ny));
     total += uint64_t(cnt[0]) * cnt[1] * cnt[2];
     if (jobs) jobs->push_back(j);
   }

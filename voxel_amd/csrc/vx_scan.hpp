@@ -142,10 +142,11 @@ __device__ __forceinline__ void fused_tile(uint32_t tile, uint32_t n, bool multi
   T v[kFusedItems], ex[kFusedItems];
   P pl[kFusedItems];
   T carry = M::identity();
+  Load lc = ld;  // per-thread copy: a Load may keep state from call to call
 #pragma unroll
   for (uint32_t k = 0; k < kFusedItems; ++k) {
     const uint32_t i = seg + k * kWave + lane;
-    v[k] = i < n ? T(ld(i, pl[k])) : M::identity();
+    v[k] = i < n ? T(lc(i, pl[k])) : M::identity();
     const T inc = wave_inclusive<M>(v[k]);
     T e = __shfl_up(inc, 1, kWave);
     if (lane == 0) e = M::identity();

@@ -73,7 +73,7 @@ STAGES = ("records", "sort", "levels", "old_refs", "leaves", "nodes", "publish")
 
 KERNELS = ("records", "sort_histogram", "sort_scan", "sort_partition", "sort_copy", "runs", "up", "topdown",
            "leaf_build", "leaf_probe", "leaf_commit", "leaf_place", "node_build", "node_probe", "node_commit",
-           "node_place", "publish", "fill", "other")
+           "node_place", "publish", "fill", "other", "tail_up", "tail_topdown", "tail_nodes")
 
 
 class Profile(C.Structure):
