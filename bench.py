@@ -310,6 +310,10 @@ def _kind_of(name, last_probe):
              ("RunLoad", "runs"), ("UpLoad", "up"), ("topdown_kernel", "topdown"), ("leaf_build_kernel", "leaf_build"),
              ("node_build_kernel", "node_build"), ("publish_kernel", "publish"), ("fill_u32_kernel", "fill"),
              ("begin_kernel", "other")]
+    for key, kind in (("tail_up_kernel", "tail_up"), ("tail_topdown_kernel", "tail_topdown"),
+                      ("tail_nodes_kernel", "tail_nodes")):
+        if key in name:
+            return kind
     if "probe_kernel" in name:
         return "node_probe" if "<9u>" in name else "leaf_probe"
     if "place_kernel" in name:
