@@ -818,14 +818,53 @@ __device__ __forceinline__ void node_build_entry(const Scenes& sc, const uint32_
   }
 }
 
-__global__ void node_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ old,
-                                  const uint32_t* __restrict__ first, const uint64_t* __restrict__ ckeys,
-                                  const uint32_t* __restrict__ cparent, const uint64_t* __restrict__ cres,
-                                  uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
-                                  uint64_t* __restrict__ res) {
+// Grid form: a block takes 256 consecutive parents; their child entries are
+// one contiguous range of the height below, read coalesced by the whole block
+// into per-parent LDS slots (a per-thread walk over each parent's children
+// touches ~6 scattered entries per lane per step, and the lines it drags in
+// are evicted before the next step), then each thread merges its slots into
+// the old node's children.
+constexpr uint32_t kBuildBlock = 256;
+
+__global__ __launch_bounds__(kBuildBlock) void node_build_kernel(
+    Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ old, const uint32_t* __restrict__ first,
+    const uint64_t* __restrict__ ckeys, const uint32_t* __restrict__ cparent, const uint64_t* __restrict__ cres,
+    uint32_t* __restrict__ items, uint64_t* __restrict__ hashes, uint64_t* __restrict__ res) {
+  __shared__ uint32_t s_ch[kBuildBlock * 8];
   const uint32_t n = *n_ptr;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-    node_build_entry(sc, old, first, ckeys, cparent, cres, items, hashes, res, j);
+  const uint32_t t = threadIdx.x;
+  for (uint32_t j0 = blockIdx.x * kBuildBlock; j0 < n; j0 += gridDim.x * kBuildBlock) {
+    const uint32_t jn = min(j0 + kBuildBlock, n);
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c) s_ch[c * kBuildBlock + t] = kNone;
+    __syncthreads();
+    const uint32_t k1 = first[jn];
+    for (uint32_t k = first[j0] + t; k < k1; k += kBuildBlock) {
+      const uint32_t pj = cparent[k];
+      if (pj >= j0 && pj < jn) s_ch[uint32_t(ckeys[k] & 7u) * kBuildBlock + (pj - j0)] = resolve(cres, k);
+    }
+    __syncthreads();
+    const uint32_t j = j0 + t;
+    if (j < n) {
+      uint32_t ch[8];
+      children_of(sc.of(j).node_pool, old[j], ch);
+#pragma unroll
+      for (uint32_t c = 0; c < 8; ++c) {
+        const uint32_t v = s_ch[c * kBuildBlock + t];
+        if (v != kNone) ch[c] = v;
+      }
+      uint32_t* it = items + size_t(j) * kNodeStride;
+      uint32_t ref = 0;
+      const uint32_t w = make_node_item(ch, it, &ref);
+      if (!w) {
+        res[j] = res_make(kResFinal, ref);
+      } else {
+        hashes[j] = hash_words(it, w);
+        res[j] = res_make(kResPending, 0);
+      }
+    }
+    __syncthreads();
+  }
 }
 
 struct ResByHeight {
@@ -1130,8 +1169,9 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
     sh.keys = L.keys[h];
     {
       TimedLaunch tl(VX_K_NODE_BUILD, s);
-      node_build_kernel<<<grid_for(L.cap[h]), 256, 0, s>>>(sh, &cnt[1 + h], L.old[h], L.first[h], L.keys[h - 1],
-                                                           L.parent[h - 1], L.res[h - 1], w.items, w.hashes, L.res[h]);
+      node_build_kernel<<<grid_for(L.cap[h]), kBuildBlock, 0, s>>>(sh, &cnt[1 + h], L.old[h], L.first[h],
+                                                                   L.keys[h - 1], L.parent[h - 1], L.res[h - 1],
+                                                                   w.items, w.hashes, L.res[h]);
     }
     if ((st = last_launch()) != VX_OK) return st;
     if ((st = intern_scenes(sh, p.n_scenes, false, ItemSet{w.items, kNodeStride, 0}, w.hashes, &cnt[1 + h], L.cap[h],
