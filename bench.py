@@ -274,26 +274,28 @@ def kernel_bytes(prof, leaf_words):
     """Algorithmic HBM bytes of every launch of each kernel kind, summed over
     the measured edits, from the edit's sizes: R brick records, L distinct
     leaves, N distinct inner nodes, P radix passes, lw leaf payload bytes.
-    Random 4..8-byte reads count their own size (the line-granular cost is
-    what the hardware byte counters report)."""
+    Lower bounds: work that only some entries do (reading a non-sentinel old
+    leaf, probing a non-sentinel item, copying a new item) is counted only
+    where every entry does it; random 4..8-byte reads count their own size
+    (the line-granular cost is what the hardware byte counters report)."""
     R, L, N, P = prof.records, prof.leaves, prof.nodes, prof.sort_passes
     lw = 4 * leaf_words
     return {
         "records": (prof.sort_record_bytes + 12) * R,   # sort record + mask 8 + material 4 written
         "sort_histogram": 8 * R,                        # keys read once for every pass's histogram
         "sort_partition": 2 * prof.sort_record_bytes * R * P,  # per pass: each record read and written
-        "runs": 8 * R + 12 * L,                         # sorted keys read; leaf keys + run starts written
-        "up": 12 * (L + N) + 12 * N,                    # keys read + parent links; parent keys + starts
+        "runs": 8 * R + 16 * L,                         # sorted keys read; leaf keys, run starts, parent links
+        "up": 24 * N,                                   # heights >= 1: keys read + parent links; parent keys, starts
         "topdown": 48 * (L + N),                        # parent index, parent ref, parent item, own ref
-        "leaf_build": L * (28 + 2 * lw) + (prof.sort_record_bytes + 8) * R,  # old leaf, item, hash, result;
+        "leaf_build": L * (28 + lw) + (prof.sort_record_bytes + 12) * R,  # old ref, range, item, hash, result;
                                                         # per record: its sorted entry, mask and material
-        "leaf_probe": L * (32 + lw),                    # result, item, hash, slot, result
-        "leaf_commit": 28 * L,                          # result, hash, header, slot value
-        "leaf_place": L * (24 + 2 * lw),                # result, slot value, item copy, slot, result
-        "node_build": 100 * N + 20 * (L + N),           # old item, starts, item, hash, result; child entries
-        "node_probe": 68 * N,
-        "node_commit": 28 * N,
-        "node_place": 96 * N,
+        "leaf_probe": 16 * L,                           # result read and written (items probed: not counted)
+        "leaf_commit": 16 * L,                          # result read, slot value written (representatives only)
+        "leaf_place": 8 * L,                            # result read (copies of new items: not counted)
+        "node_build": 64 * N + 20 * (L + N),            # old ref, starts, item, hash, result; child entries
+        "node_probe": 16 * N,
+        "node_commit": 16 * N,
+        "node_place": 8 * N,
     }
 
 

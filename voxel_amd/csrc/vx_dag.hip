@@ -644,28 +644,60 @@ struct RecordStore {
 };
 
 // ---- height structure -------------------------------------------------------
-// Height 0: distinct leaf keys of the sorted records, with record ranges.
-struct RunLoad {
-  using Payload = uint64_t;
+// Sorted records -> height 0 (distinct leaves with record ranges) and height
+// 1 (their distinct parents with leaf ranges) in one pass: a pair count (new
+// leaf, new parent) per record. A record starts a new parent when it starts a
+// new leaf whose parent differs from the previous record's; entries of a
+// scene whose root is at height 0 get no parent.
+struct RunUpPayload {
+  uint64_t key;
+  bool leaf;
+  bool inc;
+};
+
+struct RunUpLoad {
+  using Payload = RunUpPayload;
   SortedRecs recs;
-  __device__ uint32_t operator()(uint32_t i, uint64_t& k) const {
-    k = recs.key(i);
-    return i == 0 || recs.key(i - 1) != k;
+  Scenes sc;  // keys unused: scene from the record key
+  __device__ uint64_t operator()(uint32_t i, RunUpPayload& p) const {
+    p.key = recs.key(i);
+    const uint64_t prev = i ? recs.key(i - 1) : 0;
+    p.leaf = i == 0 || prev != p.key;
+    p.inc = sc.at(sc.codec.scene(p.key)).depth > 0;
+    const bool par = p.leaf && p.inc && (i == 0 || sc.codec.parent(prev) != sc.codec.parent(p.key));
+    return scan::PairSumOp::make(p.leaf, par);
   }
 };
 
-struct RunStore {
+struct RunUpStore {
+  Codec codec;
   uint64_t* keys0;
   uint32_t* first0;
-  uint32_t* n0;
-  __device__ void operator()(uint32_t i, const uint64_t& k, uint32_t f, uint32_t excl, uint32_t incl, uint32_t n) const {
-    if (f) {
-      keys0[excl] = k;
-      first0[excl] = i;
+  uint32_t* parent0;
+  uint64_t* keys1;
+  uint32_t* first1;
+  uint32_t* cnt;  // cnt[1] leaves, cnt[2] height-1 entries
+  uint32_t* root_pos;
+  __device__ void operator()(uint32_t i, const RunUpPayload& p, uint64_t v, uint64_t excl, uint64_t incl,
+                             uint32_t n) const {
+    using PS = scan::PairSumOp;
+    const uint32_t leaves = PS::lo(incl), parents = PS::hi(incl);
+    if (p.leaf) {
+      const uint32_t l = leaves - 1;
+      keys0[l] = p.key;
+      first0[l] = i;
+      parent0[l] = p.inc ? parents - 1 : kNone;
+      if (!p.inc) root_pos[codec.scene(p.key)] = l;
+      if (PS::hi(v)) {
+        keys1[PS::hi(excl)] = codec.parent(p.key);
+        first1[PS::hi(excl)] = l;
+      }
     }
     if (i == n - 1) {
-      *n0 = incl;
-      first0[incl] = i + 1;
+      cnt[1] = leaves;
+      first0[leaves] = i + 1;
+      cnt[2] = parents;
+      first1[parents] = leaves;
     }
   }
 };
@@ -1101,8 +1133,12 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
   const SortedRecs recs{in_alt ? w.rkey_alt : w.rkey, in_alt ? w.rval_alt : w.rval, w.index_bits};
   {
     TimedLaunch tl(VX_K_RUNS, s);
-    st = scan::fused_dev<scan::kSum>(&cnt[0], r_ub, RunLoad{recs}, RunStore{L.keys[0], L.first[0], &cnt[1]}, w.run_ws,
-                                     s, false);
+    Scenes sr = sc;
+    sr.keys = nullptr;
+    st = scan::fused_dev_m<scan::PairSumOp>(
+        &cnt[0], r_ub, RunUpLoad{recs, sr},
+        RunUpStore{p.codec, L.keys[0], L.first[0], L.parent[0], L.keys[1], L.first[1], cnt, w.root_pos}, w.run_ws, s,
+        false);
   }
   if (st != VX_OK) return st;
   // Heights from hu up go to the single-workgroup tail kernels.
@@ -1113,7 +1149,7 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
       break;
     }
   const TailArgs ta{L, sc, p.codec, cnt, w.root_pos, w.items, w.hashes, p.n_scenes, hu, H};
-  for (uint32_t h = 0; h < hu; ++h) {
+  for (uint32_t h = 1; h < hu; ++h) {
     Scenes sh = sc;
     sh.keys = L.keys[h];
     {

@@ -58,6 +58,19 @@ struct SegSumOp {
   __host__ __device__ static T make(uint32_t v, bool head) { return (head ? kHead : 0ull) | v; }
 };
 
+// Two counts in one word: bits 0..29 and 30..59 (each count < 2^30), so u64
+// addition adds them independently; status at bit 62.
+struct PairSumOp {
+  using T = uint64_t;
+  static constexpr uint32_t kStatusShift = 62;
+  static constexpr uint32_t kBits = 30;
+  __host__ __device__ static T identity() { return 0ull; }
+  __device__ static T combine(T a, T b) { return a + b; }
+  __host__ __device__ static T make(uint32_t lo, uint32_t hi) { return uint64_t(lo) | (uint64_t(hi) << kBits); }
+  __host__ __device__ static uint32_t lo(T v) { return uint32_t(v) & ((1u << kBits) - 1); }
+  __host__ __device__ static uint32_t hi(T v) { return uint32_t(v >> kBits) & ((1u << kBits) - 1); }
+};
+
 template <Op OP>
 struct OpOf;
 template <>
