@@ -50,7 +50,7 @@ __device__ __forceinline__ uint32_t count_of(const uint32_t* n_ptr, uint32_t n_a
 }
 
 // ---- pass 0: histograms of every digit position in one read ----------------
-template <class K, uint32_t RB>
+template <class K, uint32_t RB, bool COHERENT>
 __global__ __launch_bounds__(256) void histogram_kernel(const K* __restrict__ keys, const uint32_t* n_ptr,
                                                         uint32_t n_arg, uint32_t begin_bit, uint32_t key_bits,
                                                         uint32_t* __restrict__ hist) {
@@ -61,14 +61,41 @@ __global__ __launch_bounds__(256) void histogram_kernel(const K* __restrict__ ke
   const uint32_t passes = (key_bits + RB - 1) / RB;
   for (uint32_t i = threadIdx.x; i < passes * R; i += blockDim.x) s[i] = 0;
   __syncthreads();
+  // COHERENT (keys that arrive in spatially coherent runs, like the edit's
+  // brick records in brush order): lanes whose digit equals the first active
+  // lane's add it with one atomic (the high digits are then often the same
+  // across a wave, and 64 same-address LDS atomics would serialise); the
+  // others add their own. Random keys skip the extra ballots.
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const K key = keys[i];
+  if constexpr (!COHERENT) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      const K key = keys[i];
+#pragma unroll
+      for (uint32_t p = 0; p < MAXP; ++p) {
+        if (p < passes) {
+          const uint32_t bits = min(RB, key_bits - p * RB);
+          atomicAdd(&s[p * R + digit_of(key, begin_bit + p * RB, (1u << bits) - 1)], 1u);
+        }
+      }
+    }
+  }
+  for (uint32_t i0 = blockIdx.x * blockDim.x; COHERENT && i0 < n; i0 += stride) {
+    const uint32_t i = i0 + threadIdx.x;
+    const bool valid = i < n;
+    const K key = valid ? keys[i] : K(0);
 #pragma unroll
     for (uint32_t p = 0; p < MAXP; ++p) {
       if (p < passes) {
         const uint32_t bits = min(RB, key_bits - p * RB);
-        atomicAdd(&s[p * R + digit_of(key, begin_bit + p * RB, (1u << bits) - 1)], 1u);
+        const uint32_t d = digit_of(key, begin_bit + p * RB, (1u << bits) - 1);
+        const uint64_t act = ballot(valid);
+        const uint32_t lead = __shfl(d, act ? int(__builtin_ctzll(act)) : 0, kWave);
+        const uint64_t same = ballot(valid && d == lead);
+        if (valid && d == lead) {
+          if (lanes_below(same) == 0) atomicAdd(&s[p * R + d], uint32_t(__popcll(same)));
+        } else if (valid) {
+          atomicAdd(&s[p * R + d], 1u);
+        }
       }
     }
   }
@@ -372,7 +399,7 @@ Layout layout(void* ws, uint64_t n, size_t key_bytes, uint32_t key_bits, bool va
 
 // Core driver. Sorts (keys, vals) using (alt_keys, alt_vals) as the second
 // buffer; *in_alt reports where the result ended up.
-template <class K, bool VALS>
+template <class K, bool VALS, bool COHERENT = false>
 vx_status drive(K* keys, uint32_t* vals, K* alt_keys, uint32_t* alt_vals, const uint32_t* n_ptr, uint32_t n,
                 uint32_t begin_bit, uint32_t key_bits, const Layout& lay, const Launcher<K, VALS>& L, const Tuning& tn,
                 bool* in_alt, hipStream_t s) {
@@ -384,9 +411,12 @@ vx_status drive(K* keys, uint32_t* vals, K* alt_keys, uint32_t* alt_vals, const 
   const uint32_t hblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(n, 256 * 16), 2048));
   {
     TimedLaunch tl(VX_K_SORT_HISTOGRAM, s);
-    if (tn.rb == 8) histogram_kernel<K, 8><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
-    else if (tn.rb == 9) histogram_kernel<K, 9><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
-    else histogram_kernel<K, 4><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
+    if (tn.rb == 8)
+      histogram_kernel<K, 8, COHERENT><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
+    else if (tn.rb == 9)
+      histogram_kernel<K, 9, COHERENT><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
+    else
+      histogram_kernel<K, 4, COHERENT><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
   }
   if ((st = last_launch()) != VX_OK) return st;
   {
@@ -529,8 +559,8 @@ vx_status keys_u64_dev(uint64_t* keys, uint64_t* alt_keys, const uint32_t* n_ptr
   if (!pick(L, tn)) return VX_UNSUPPORTED;
   if (n_upper >= (1u << 30) || begin_bit + key_bits > 64) return VX_INVALID_VALUE;
   const Layout lay = layout(ws, n_upper, 8, key_bits, false, tn, L.tile, false);
-  return drive<uint64_t, false>(keys, nullptr, alt_keys, nullptr, n_ptr, uint32_t(n_upper), begin_bit, key_bits, lay,
-                                L, tn, in_alt, s);
+  return drive<uint64_t, false, true>(keys, nullptr, alt_keys, nullptr, n_ptr, uint32_t(n_upper), begin_bit, key_bits,
+                                      lay, L, tn, in_alt, s);
 }
 
 vx_status set_tuning(uint32_t key_bytes, uint32_t rb, uint32_t kpt, uint32_t block) {
