@@ -223,6 +223,10 @@ __global__ __launch_bounds__(kFusedBlock) void fused_dev_kernel(const uint32_t* 
   __shared__ uint32_t s_tile;
   const uint32_t n = min(*n_ptr, n_cap);
   const uint32_t tiles = (n + kFusedTile - 1) / kFusedTile;
+  if (tiles <= 1) {  // one tile: block 0 takes it without a claim or look-back
+    if (tiles == 1 && blockIdx.x == 0) fused_tile<M>(0u, n, false, ld, st, rec, s_wave, &s_prefix);
+    return;
+  }
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
