@@ -178,6 +178,75 @@ def primitives(torch, P):
     return out
 
 
+def ray_leg(torch, scene, world, dev, n=1 << 20):
+    """Ray casting on the edited scene: 2^20 rays from random voxels of the top
+    slab of the world (y in [world - 64, world)) with random integer directions
+    pointing down (dy in [-64, -1], dx, dz in [-64, 64]), unlimited length;
+    rays/s (hipEvents) and the mean number of voxels crossed per hit."""
+    g = torch.Generator(device="cpu").manual_seed(17)
+    rays = torch.zeros((n, 8), dtype=torch.int64)
+    rays[:, 0] = torch.randint(0, world, (n,), generator=g)
+    rays[:, 1] = torch.randint(world - 64, world, (n,), generator=g)
+    rays[:, 2] = torch.randint(0, world, (n,), generator=g)
+    rays[:, 3] = torch.randint(-64, 65, (n,), generator=g)
+    rays[:, 4] = torch.randint(-64, 0, (n,), generator=g)
+    rays[:, 5] = torch.randint(-64, 65, (n,), generator=g)
+    rays[:, 6] = 0xFFFFFFFF
+    rays = rays.to(torch.int32).to(dev)  # two's complement: max_steps 0xFFFFFFFF stays all ones
+    scene.raycast(rays)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    hits = scene.raycast(rays)
+    ev1.record()
+    ev1.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    hit = hits[:, 0] != 0
+    return {"workload": f"{n} rays from the top 64 voxel slab of the {world}^3 scene, downward integer directions",
+            "ms": ms, "rays_per_s": n / ms * 1e3, "hit_fraction": float(hit.float().mean()),
+            "mean_steps_per_hit": float(hits[hit, 1].double().mean()) if bool(hit.any()) else 0.0}
+
+
+def hash_tables(torch, P):
+    """Config 1d: 2^26 unique u64 keys (seed 2) into tables sized for load 0.8
+    (slab set and cuckoo set), then 2^26 lookups, half from the inserted set
+    and half from a disjoint set, shuffled; the hit results are checked
+    exactly. keys/s of insert and lookup."""
+    from voxel_amd import hash as Hh
+    n = 1 << 26
+    keys = P.gen_unique_u64(2, 4, 0, n)
+    other = P.gen_unique_u64(2, 4, n, n // 2)
+    perm = torch.randperm(n, generator=torch.Generator().manual_seed(7)).cuda()
+    queries = torch.cat([keys[: n // 2], other])[perm]
+    expect = (perm < n // 2).to(torch.uint8)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        ev0.record()
+        out = fn()
+        ev1.record()
+        ev1.synchronize()
+        return ev0.elapsed_time(ev1), out
+
+    res = {}
+    slab = Hh.SlabHashSetU64(Hh.slab_buckets_for(n, 0.8), 1 << 22)
+    ms_i, _ = timed(lambda: slab.insert_batch(keys))
+    ms_f, found = timed(lambda: slab.contains_batch(queries))
+    res["slab_set_u64_2^26"] = {"insert_ms": ms_i, "insert_keys_per_s": n / ms_i * 1e3, "lookup_ms": ms_f,
+                                "lookup_keys_per_s": n / ms_f * 1e3, "exact": bool(torch.equal(found, expect)),
+                                "insert_gbs_algorithmic": 16 * n / ms_i / 1e6, "lookup_gbs_algorithmic": 17 * n / ms_f / 1e6}
+    slab.close()
+    cuckoo = Hh.CuckooSetU64(int(n / 0.8) + 1)
+    ms_i, failed = timed(lambda: cuckoo.insert_batch(keys))
+    ms_f, found = timed(lambda: cuckoo.contains_batch(queries))
+    res["cuckoo_set_u64_2^26"] = {"insert_ms": ms_i, "insert_keys_per_s": n / ms_i * 1e3, "lookup_ms": ms_f,
+                                  "lookup_keys_per_s": n / ms_f * 1e3,
+                                  "exact": bool(torch.equal(found, expect)) and int(failed.item()) == 0,
+                                  "insert_gbs_algorithmic": 16 * n / ms_i / 1e6,
+                                  "lookup_gbs_algorithmic": 17 * n / ms_f / 1e6}
+    cuckoo.close()
+    return res
+
+
 def _sample_equal(V, torch, a, b, n=1 << 20, seed=0):
     """Same material at n random voxels of two scenes (a cheap full-size check
     beside the exact canonical-export comparisons of the test suite)."""
@@ -465,10 +534,14 @@ def main():
     bytes_per_launch = kernel_bytes(prof, 16).get(dom, 0) / max(prof.kernel_launches[di], 1)
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
 
+    rays = ray_leg(torch, scene, world, dev)
     scene.close()
     del ws
     torch.cuda.empty_cache()
     prims = {} if args.no_primitives else primitives(torch, P)
+    torch.cuda.empty_cache()
+    if not args.no_primitives:
+        prims.update(hash_tables(torch, P))
     torch.cuda.empty_cache()
     batched = None if args.no_batched else leg_batched(V, torch)
     torch.cuda.empty_cache()
@@ -532,6 +605,7 @@ def main():
         "kernels": kernels,
         "scene": {"build_s": build_s, "node_words": st.node_words_used, "leaf_words": st.leaf_words_used},
         "primitives": prims,
+        "rays": rays,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu_res,

@@ -278,6 +278,30 @@ vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batch, const vx
  * occupancy DAG), 0 if empty or out of the world (query.cuh:12-39). */
 vx_status vx_dag_get_voxels(const vx_dag* dag, const vx_voxel* d_xyz, uint32_t n, uint32_t* d_out, void* stream);
 
+/* Ray traversal (the consumer side of the DAG, next to query.cuh:12-39; the
+ * reference has no GPU ray caster). A ray starts at the centre of voxel
+ * (ox, oy, oz) with the integer direction (dx, dy, dz) and visits voxels in
+ * the order of its boundary crossings: crossing j of axis a happens at
+ * t = (2j + 1) / (2|d_a|); simultaneous crossings are taken x before y
+ * before z. It visits the origin voxel and then at most max_steps further
+ * voxels, and stops at the first non-empty one (material, or 1 in an
+ * occupancy DAG) or when it leaves the world (a miss, as is an origin outside
+ * the world or a zero direction from an empty origin). Exact integer
+ * arithmetic throughout; empty subtrees are crossed in one step. */
+typedef struct vx_ray {
+  int32_t ox, oy, oz;
+  int32_t dx, dy, dz;
+  uint32_t max_steps;
+  uint32_t pad;
+} vx_ray;
+typedef struct vx_hit {
+  uint32_t material; /* 0: miss */
+  uint32_t steps;    /* voxels crossed before the hit voxel */
+  int32_t x, y, z;   /* hit voxel */
+  uint32_t pad;
+} vx_hit;
+vx_status vx_dag_raycast(const vx_dag* dag, const vx_ray* d_rays, uint32_t n, vx_hit* d_hits, void* stream);
+
 /* Canonical depth-first export (host buffer of u32 words):
  *   [magic 0x56584443, inner_depth, leaf_words, root kind (0 EMPTY, 1 FULL,
  *    2 node), node count, leaf count], then per node in first-visit order

@@ -727,6 +727,52 @@ void vxo_dag_box(vxo_dag* d, uint32_t x0, uint32_t y0, uint32_t z0, uint32_t x1,
 
 uint32_t vxo_dag_get(const vxo_dag* d, uint32_t x, uint32_t y, uint32_t z) { return d->get(x, y, z); }
 
+// Voxel-by-voxel walk: the next crossing is the axis with the smallest
+// (2 k_a + 1) / (2 |d_a|) (compared by cross-multiplication), lowest axis on
+// ties; get() at every visited voxel.
+void vxo_dag_raycast(const vxo_dag* d, const int32_t* rays, uint32_t n, int32_t* hits) {
+  const int64_t world = int64_t(d->world);
+  for (uint32_t i = 0; i < n; ++i) {
+    const int32_t* q = rays + 8 * size_t(i);
+    const int64_t o[3] = {q[0], q[1], q[2]};
+    const int64_t dir[3] = {q[3], q[4], q[5]};
+    const uint64_t max_steps = uint32_t(q[6]);
+    uint64_t ad[3], k[3] = {0, 0, 0};
+    int64_t sgn[3];
+    for (int a = 0; a < 3; ++a) {
+      ad[a] = uint64_t(dir[a] < 0 ? -dir[a] : dir[a]);
+      sgn[a] = dir[a] > 0 ? 1 : (dir[a] < 0 ? -1 : 0);
+    }
+    int32_t* h = hits + 6 * size_t(i);
+    for (int c = 0; c < 6; ++c) h[c] = 0;
+    for (uint64_t step = 0;; ++step) {
+      int64_t v[3];
+      bool inside = true;
+      for (int a = 0; a < 3; ++a) {
+        v[a] = o[a] + sgn[a] * int64_t(k[a]);
+        inside = inside && v[a] >= 0 && v[a] < world;
+      }
+      if (!inside || step > max_steps) break;
+      const uint32_t m = d->get(uint32_t(v[0]), uint32_t(v[1]), uint32_t(v[2]));
+      if (m) {
+        h[0] = int32_t(m);
+        h[1] = int32_t(step);
+        h[2] = int32_t(v[0]);
+        h[3] = int32_t(v[1]);
+        h[4] = int32_t(v[2]);
+        break;
+      }
+      int best = -1;
+      for (int a = 0; a < 3; ++a) {
+        if (!ad[a]) continue;
+        if (best < 0 || (2 * k[a] + 1) * ad[best] < (2 * k[best] + 1) * ad[a]) best = a;
+      }
+      if (best < 0) break;
+      ++k[best];
+    }
+  }
+}
+
 uint64_t vxo_dag_export(const vxo_dag* d, uint32_t* out, uint64_t cap) {
   Exporter ex(*d);
   return copy_out(ex.run(), out, cap);

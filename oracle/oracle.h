@@ -108,6 +108,11 @@ void vxo_dag_box(vxo_dag* d, uint32_t x0, uint32_t y0, uint32_t z0,
                  uint32_t x1, uint32_t y1, uint32_t z1, uint32_t material);
 /* 0 if empty, else material (1 in occupancy mode). */
 uint32_t vxo_dag_get(const vxo_dag* d, uint32_t x, uint32_t y, uint32_t z);
+/* Ray walk, one voxel per crossing (definition in include/vx.h, vx_ray):
+ * crossing j of axis a at t = (2j+1)/(2|d_a|), ties x before y before z.
+ * rays: n x {ox, oy, oz, dx, dy, dz, max_steps, pad} (int32 / uint32);
+ * hits: n x {material, steps, x, y, z, pad}. */
+void vxo_dag_raycast(const vxo_dag* d, const int32_t* rays, uint32_t n, int32_t* hits);
 /* Canonical depth-first export (format in the .cpp header comment). Returns
  * the number of u32 words; writes them if out != NULL and cap suffices. */
 uint64_t vxo_dag_export(const vxo_dag* d, uint32_t* out, uint64_t cap);

@@ -84,6 +84,7 @@ def lib():
             "vxo_dag_apply_voxels": (None, [P, P, u32, u32]),
             "vxo_dag_box": (None, [P, u32, u32, u32, u32, u32, u32, u32]),
             "vxo_dag_get": (u32, [P, u32, u32, u32]),
+            "vxo_dag_raycast": (None, [P, P, u32, P]),
             "vxo_dag_export": (u64, [P, P, u64]),
             "vxo_dag_stored_nodes": (u64, [P]),
             "vxo_dag_stored_leaves": (u64, [P]),
@@ -253,6 +254,14 @@ class Dag:
 
     def get(self, x, y, z) -> int:
         return int(lib().vxo_dag_get(self.h, x, y, z))
+
+    def raycast(self, rays: np.ndarray) -> np.ndarray:
+        """rays: (n, 8) int32 {ox, oy, oz, dx, dy, dz, max_steps, 0}; returns
+        (n, 6) int32 {material, steps, x, y, z, 0}."""
+        r = np.ascontiguousarray(rays, np.int32).reshape(-1, 8)
+        out = np.zeros((len(r), 6), np.int32)
+        lib().vxo_dag_raycast(self.h, _ptr(r), len(r), _ptr(out))
+        return out
 
     def export(self) -> np.ndarray:
         n = lib().vxo_dag_export(self.h, None, 0)

@@ -162,3 +162,58 @@ def test_config0_occupancy_matches_reference_golden():
     e = d.export()
     assert (int(e[4]), int(e[5]), "%016x" % O.digest(e)) == (g["edited"]["nodes"], g["edited"]["leaves"],
                                                              g["edited"]["digest"])
+
+
+def test_oracle_raycast_pins():
+    """The oracle's ray walk against what the geometry fixes: a straight
+    segment between two voxel centres crosses exactly |dx|+|dy|+|dz| voxel
+    boundaries, so a ray aimed at the only solid voxel hits it after that many
+    steps; an axis-aligned ray hits the first solid voxel of its column
+    (dense scan); rays that leave the world, start outside it, or cannot move
+    miss."""
+    import numpy as np
+    from oracle import oracle as O
+    rng = np.random.default_rng(5)
+    d = O.Dag(4, True)
+    world = 64
+    tgt = (37, 21, 50)
+    d.box(tgt, tuple(c + 1 for c in tgt), 77)
+    rays = []
+    for _ in range(200):
+        o = rng.integers(0, world, 3)
+        dv = np.array(tgt) - o
+        if not dv.any():
+            continue
+        rays.append([*o, *dv, 10_000, 0])
+    hits = d.raycast(np.array(rays))
+    for r, h in zip(rays, hits):
+        assert h[0] == 77 and tuple(h[2:5]) == tgt
+        assert h[1] == abs(r[3]) + abs(r[4]) + abs(r[5])
+    # max_steps one short of the target: miss
+    short = np.array(rays)
+    short[:, 6] = np.abs(short[:, 3:6]).sum(1) - 1
+    assert (d.raycast(short)[:, 0] == 0).all()
+    # axis-aligned rays in a terrain scene vs a dense column scan
+    t = O.Dag(4, True)
+    t.build_terrain(O.terrain_for_world(world, 9))
+    grid = np.array([[[t.get(x, y, z) for x in range(world)] for y in range(world)] for z in range(world)])
+    for _ in range(100):
+        o = rng.integers(0, world, 3)
+        axis = int(rng.integers(0, 3))
+        sign = int(rng.choice([-1, 1]))
+        dv = [0, 0, 0]
+        dv[axis] = sign * int(rng.integers(1, 5))
+        h = t.raycast(np.array([[*o, *dv, 10_000, 0]]))[0]
+        cells = []
+        p = list(o)
+        while 0 <= p[axis] < world:
+            cells.append(tuple(p))
+            p[axis] += sign
+        solid = [c for c in cells if grid[c[2], c[1], c[0]]]
+        if solid:
+            c = solid[0]
+            assert h[0] == grid[c[2], c[1], c[0]] and tuple(h[2:5]) == c and h[1] == abs(c[axis] - o[axis])
+        else:
+            assert h[0] == 0
+    out = t.raycast(np.array([[-1, 5, 5, 1, 0, 0, 100, 0], [3, 3, 3, 0, 0, 0, 100, 0]]))
+    assert out[0, 0] == 0 and (out[1, 0] == 0) == (t.get(3, 3, 3) == 0)

@@ -1,0 +1,124 @@
+// Ray traversal over the hash DAG: integer DDA with empty-subtree skipping.
+//
+// The voxel order is fixed by exact integer arithmetic (include/vx.h, vx_ray):
+// crossing j of axis a at t = (2j + 1) / (2|d_a|), ties x < y < z. The state
+// is the number of crossings k_a taken on each axis. Each iteration descends
+// the DAG to the current voxel; an EMPTY ref at level l means the whole
+// aligned cube of side world >> l is empty, and the ray leaves it in one
+// step: the first crossing that exits the cube is found per axis, and the
+// crossings of the other axes before it are counted in closed form — the
+// same state a voxel-by-voxel walk would reach.
+#include "vx_dag.hpp"
+
+using namespace vx;
+using namespace vx::dag;
+
+namespace {
+
+struct Ray {
+  int64_t o[3];
+  int32_t sgn[3];
+  uint64_t ad[3];
+  uint64_t k[3];
+};
+
+// Whether crossing (a, ja) comes before crossing (b, jb).
+__device__ __forceinline__ bool before(const Ray& r, int a, uint64_t ja, int b, uint64_t jb) {
+  const uint64_t lhs = (2 * ja + 1) * r.ad[b], rhs = (2 * jb + 1) * r.ad[a];
+  return lhs < rhs || (lhs == rhs && a < b);
+}
+
+// Leave the aligned cube [lo, lo + side) that holds the current voxel: returns
+// false if the ray cannot move (zero direction).
+__device__ __forceinline__ bool exit_cube(Ray& r, const int64_t (&lo)[3], int64_t side) {
+  int best = -1;
+  uint64_t jbest = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!r.ad[a]) continue;
+    const int64_t j = r.sgn[a] > 0 ? lo[a] + side - 1 - r.o[a] : r.o[a] - lo[a];
+    if (best < 0 || before(r, a, uint64_t(j), best, jbest)) {
+      best = a;
+      jbest = uint64_t(j);
+    }
+  }
+  if (best < 0) return false;
+  const uint64_t Y = r.ad[best];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    if (b == best || !r.ad[b]) continue;
+    const uint64_t X = (2 * jbest + 1) * r.ad[b];
+    // crossings (b, j) before (best, jbest): (2j+1)Y < X, or == X when b < best
+    uint64_t cnt;
+    if (b < best) cnt = X >= Y ? (X - Y) / (2 * Y) + 1 : 0;
+    else cnt = X > Y ? (X - Y - 1) / (2 * Y) + 1 : 0;
+    if (cnt > r.k[b]) r.k[b] = cnt;
+  }
+  r.k[best] = jbest + 1;
+  return true;
+}
+
+__global__ void raycast_kernel(Dev d, const vx_ray* __restrict__ rays, uint32_t n, vx_hit* __restrict__ hits) {
+  const int64_t world = int64_t(4) << d.depth;
+  const uint32_t log_world = d.depth + 2;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const vx_ray q = rays[i];
+    Ray r;
+    const int32_t dir[3] = {q.dx, q.dy, q.dz};
+    r.o[0] = q.ox;
+    r.o[1] = q.oy;
+    r.o[2] = q.oz;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      r.sgn[a] = dir[a] > 0 ? 1 : (dir[a] < 0 ? -1 : 0);
+      r.ad[a] = uint64_t(dir[a] < 0 ? -int64_t(dir[a]) : int64_t(dir[a]));
+      r.k[a] = 0;
+    }
+    vx_hit h{0u, 0u, 0, 0, 0, 0u};
+    const uint32_t root = d.ctl[kCtlRoot];
+    for (;;) {
+      int64_t v[3];
+      bool inside = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        v[a] = r.o[a] + int64_t(r.sgn[a]) * int64_t(r.k[a]);
+        inside = inside && v[a] >= 0 && v[a] < world;
+      }
+      const uint64_t steps = r.k[0] + r.k[1] + r.k[2];
+      if (!inside || steps > q.max_steps) break;
+      const uint32_t x = uint32_t(v[0]), y = uint32_t(v[1]), z = uint32_t(v[2]);
+      uint32_t ref = root, lvl = 0;
+      for (; lvl < d.depth && ref > kFull; ++lvl) {
+        const uint32_t sh = log_world - 1 - lvl;
+        ref = child_of(d.node_pool, ref, ((x >> sh) & 1u) | (((y >> sh) & 1u) << 1) | (((z >> sh) & 1u) << 2));
+      }
+      uint32_t m = 0;
+      int64_t side = 1;
+      if (ref == kFull) {
+        m = 1;
+      } else if (ref == kEmpty) {
+        side = world >> lvl;
+      } else {
+        const uint32_t bit = (x & 3u) + 4u * (y & 3u) + 16u * (z & 3u);
+        m = d.leaf_words == 2 ? (d.leaf_pool[ref + (bit >> 5)] >> (bit & 31)) & 1u
+                              : (d.leaf_pool[ref + (bit >> 2)] >> ((bit & 3) * 8)) & 0xffu;
+      }
+      if (m) {
+        h = vx_hit{m, uint32_t(steps), int32_t(x), int32_t(y), int32_t(z), 0u};
+        break;
+      }
+      const int64_t lo[3] = {v[0] & ~(side - 1), v[1] & ~(side - 1), v[2] & ~(side - 1)};
+      if (!exit_cube(r, lo, side)) break;
+    }
+    hits[i] = h;
+  }
+}
+
+}  // namespace
+
+extern "C" vx_status vx_dag_raycast(const vx_dag* g, const vx_ray* rays, uint32_t n, vx_hit* hits, void* stream) {
+  if (!g || ((!rays || !hits) && n)) return VX_INVALID_VALUE;
+  if (n == 0) return VX_OK;
+  raycast_kernel<<<grid_for(n), 256, 0, static_cast<hipStream_t>(stream)>>>(g->dev, rays, n, hits);
+  return last_launch();
+}

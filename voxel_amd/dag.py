@@ -52,6 +52,7 @@ register({
     "vx_dag_brush_workspace_size": (_i32, [_P, _P, _u32, C.POINTER(_sz)]),
     "vx_dag_apply_brushes": (_i32, [_P, _P, _u32, _P, _sz, _P]),
     "vx_dag_get_voxels": (_i32, [_P, _P, _u32, _P, _P]),
+    "vx_dag_raycast": (_i32, [_P, _P, _u32, _P, _P]),
     "vx_dag_export_canonical": (_i32, [_P, _P, _u64, C.POINTER(_u64)]),
     "vx_brush_voxel_ops": (_i32, [_P, _u32, _u32, C.POINTER(_u64)]),
     "vx_gen_brushes": (_i32, [_u64, _u64, _u32, _u32, _u32, _P]),
@@ -220,6 +221,15 @@ class HashDagGpu:
         n = xyz.shape[0]
         out = torch.empty(n, dtype=torch.int32, device=xyz.device)
         check(lib().vx_dag_get_voxels(self.h, _ptr(xyz), n, _ptr(out), _stream(stream)), "vx_dag_get_voxels")
+        return out
+
+    def raycast(self, rays: torch.Tensor, stream=None) -> torch.Tensor:
+        """rays: (n, 8) int32 device tensor {ox, oy, oz, dx, dy, dz, max_steps,
+        0} (vx_ray); returns (n, 6) int32 {material, steps, x, y, z, 0}."""
+        r = rays.contiguous()
+        n = r.shape[0]
+        out = torch.empty((n, 6), dtype=torch.int32, device=r.device)
+        check(lib().vx_dag_raycast(self.h, _ptr(r), n, _ptr(out), _stream(stream)), "vx_dag_raycast")
         return out
 
     def profile(self, enable: bool = True):
