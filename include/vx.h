@@ -309,6 +309,29 @@ vx_status vx_dag_raycast(const vx_dag* dag, const vx_ray* d_rays, uint32_t n, vx
  * *n_words receives the size; words are written only if cap suffices. */
 vx_status vx_dag_export_canonical(const vx_dag* dag, uint32_t* out, uint64_t cap, uint64_t* n_words);
 
+/* Garbage collection of the interned tables: keeps exactly the items
+ * reachable from the root, each once, and rebuilds both slot tables over them
+ * (the GPU counterpart of the CPU HashDAG::collect_garbage,
+ * svt/hash_dag.hpp:133-191; the reference's GPU tables only grow,
+ * cuda/hash_table/bump_allocator.cuh:118-121). The voxel field, the export
+ * and every later edit are unchanged; pool offsets (refs) change. Synchronous
+ * on `stream`; no other call may use the scene meanwhile. Allocates
+ * temporaries (two slot tables and the live pool words); if anything fails
+ * the scene is left as it was. `report` may be NULL. */
+typedef struct vx_gc_report {
+  uint64_t node_words_before, node_words_after;
+  uint64_t leaf_words_before, leaf_words_after;
+  uint64_t live_nodes;  /* live (level, node item) pairs */
+  uint64_t live_leaves; /* live leaf items */
+} vx_gc_report;
+vx_status vx_dag_collect_garbage(vx_dag* dag, vx_gc_report* report, void* stream);
+
+/* Host copy of the used words [0, used) of both pools (inspection and
+ * tests; no counterpart in the reference). *node_n / *leaf_n receive the
+ * sizes; a pool's words are written only if its cap suffices. */
+vx_status vx_dag_read_pools(const vx_dag* dag, uint32_t* node_out, uint64_t node_cap, uint64_t* node_n,
+                            uint32_t* leaf_out, uint64_t leaf_cap, uint64_t* leaf_n);
+
 /* Seeded brush batch (host output). mix 0: spheres r in [4,16], 50% add;
  * mix 1: 50% spheres r in [8,64] / 50% boxes half extents in [4,48], 60% add;
  * added material uniform in [1,255]. Brush i uses draws 16i..16i+8 of the

@@ -1,0 +1,242 @@
+"""Garbage collection of the GPU Hash DAG: the reference's known-answer tests
+for HashDAG::collect_garbage (test/svt/test_svt_hash_dag.cpp:98-135), the
+voxel field unchanged (canonical export and point queries), the pools holding
+exactly the reachable items, each once, and later edits still matching the
+oracle with maximal sharing (the rebuilt slot tables find every live item)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+POOL_START = 16
+
+
+def _dag(depth, materials, **kw):
+    from voxel_amd.dag import HashDagGpu
+    return HashDagGpu(depth, materials, **kw)
+
+
+def _dev_xyz(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, np.uint32).reshape(-1, 3).view(np.int32)).to(dev)
+
+
+def _reachable(d):
+    """Distinct node refs and leaf refs reachable from the root (walk over
+    (level, ref) pairs; a ref at level D - 1 names leaves)."""
+    npool, lpool = d.read_pools()
+    st = d.stats()
+    D = d.inner_depth
+    nodes, leaves = set(), set()
+    if st.root <= 1:
+        return npool, lpool, nodes, leaves
+    frontier = {st.root}
+    for lvl in range(D):
+        nxt = set()
+        for ref in frontier:
+            nodes.add(ref)
+            cm = int(npool[ref]) & 0xFF
+            for k in range(bin(cm).count("1")):
+                nxt.add(int(npool[ref + 1 + k]))
+        if lvl + 1 == D:
+            leaves |= nxt
+        frontier = nxt
+    return npool, lpool, nodes, leaves
+
+
+def _node_width(npool, ref):
+    return 1 + bin(int(npool[ref]) & 0xFF).count("1")
+
+
+def _assert_compact(d):
+    """Live items tile [16, used) of each pool without gaps, and no two hold
+    the same words."""
+    npool, lpool, nodes, leaves = _reachable(d)
+    st = d.stats()
+    lw = st.leaf_words
+    pos = POOL_START
+    seen = set()
+    for ref in sorted(nodes):
+        assert ref == pos, (ref, pos)
+        w = _node_width(npool, ref)
+        item = tuple(npool[ref:ref + w].tolist())
+        assert item not in seen
+        seen.add(item)
+        pos += w
+    assert pos == st.node_words_used
+    pos = POOL_START
+    seen = set()
+    for ref in sorted(leaves):
+        assert ref == pos, (ref, pos)
+        item = tuple(lpool[ref:ref + lw].tolist())
+        assert item not in seen
+        seen.add(item)
+        pos += lw
+    assert pos == st.leaf_words_used
+    return len(nodes), len(leaves)
+
+
+def _assert_items_unique(d):
+    """Every item in the pools (live or not) holds distinct words: what the
+    slot tables guarantee when they see every stored item."""
+    npool, lpool = d.read_pools()
+    lw = d.stats().leaf_words
+    seen = set()
+    k = POOL_START
+    while k < len(npool):
+        w = _node_width(npool, k)
+        item = tuple(npool[k:k + w].tolist())
+        assert item not in seen, k
+        seen.add(item)
+        k += w
+    assert k == len(npool)
+    seen = set()
+    for k in range(POOL_START, len(lpool), lw):
+        item = tuple(lpool[k:k + lw].tolist())
+        assert item not in seen, k
+        seen.add(item)
+
+
+def test_gc_drops_unreachable_entries(gpu):
+    """test_svt_hash_dag.cpp:98-116: set then clear one voxel; GC restores the
+    initial counts and keeps the EMPTY root."""
+    d = _dag(2, False)
+    st0 = d.stats()
+    d.place_voxel_edits(_dev_xyz([[1, 2, 3]], gpu))
+    st1 = d.stats()
+    assert st1.node_words_used > st0.node_words_used and st1.leaf_words_used > st0.leaf_words_used
+    d.destroy_voxel_edits(_dev_xyz([[1, 2, 3]], gpu))
+    assert d.stats().root == 0
+    assert d.stats().node_words_used > st0.node_words_used
+    r = d.collect_garbage()
+    st = d.stats()
+    assert st.root == 0
+    assert (st.node_words_used, st.leaf_words_used) == (st0.node_words_used, st0.leaf_words_used)
+    assert (r.node_words_after, r.leaf_words_after, r.live_nodes, r.live_leaves) == (POOL_START, POOL_START, 0, 0)
+    assert r.node_words_before == st1.node_words_used
+
+
+def test_gc_keeps_root_and_full_subtrees(gpu):
+    """test_svt_hash_dag.cpp:118-135: two voxels survive GC, an empty one
+    stays empty, and a whole-world fill afterwards reads full everywhere."""
+    d = _dag(2, False)
+    d.place_voxel_edits(_dev_xyz([[0, 0, 0], [15, 15, 15]], gpu))
+    d.collect_garbage()
+    q = _dev_xyz([[0, 0, 0], [15, 15, 15], [7, 7, 7]], gpu)
+    assert d.get_voxels(q).cpu().tolist() == [1, 1, 0]
+    _assert_compact(d)
+    world = np.array([[x, y, z] for z in range(16) for y in range(16) for x in range(16)], np.uint32)
+    d.place_voxel_edits(_dev_xyz(world, gpu))
+    assert d.get_voxels(q).cpu().tolist() == [1, 1, 1]
+    assert d.stats().root == 1
+    d.collect_garbage()
+    st = d.stats()
+    assert (st.root, st.node_words_used, st.leaf_words_used) == (1, POOL_START, POOL_START)
+
+
+def test_gc_sentinel_children_only(gpu):
+    """A live node whose children are all FULL / EMPTY (child mask 0): nothing
+    lives below it."""
+    d = _dag(2, False)
+    cube = np.array([[x, y, z] for z in range(8) for y in range(8) for x in range(8)], np.uint32)
+    d.place_voxel_edits(_dev_xyz(cube, gpu))
+    d.destroy_voxel_edits(_dev_xyz([[15, 15, 15]], gpu))
+    d.place_voxel_edits(_dev_xyz([[15, 15, 15]], gpu))
+    d.destroy_voxel_edits(_dev_xyz([[15, 15, 15]], gpu))
+    before = d.export()
+    r = d.collect_garbage()
+    assert np.array_equal(d.export(), before)
+    assert (r.live_nodes, r.live_leaves) == (1, 0)
+    assert d.stats().node_words_used == POOL_START + 1
+    _assert_compact(d)
+
+
+@pytest.mark.parametrize("materials", [False, True])
+def test_gc_after_edits_keeps_field_and_later_edits_match_oracle(gpu, materials):
+    from voxel_amd.dag import terrain_for_world
+    depth = 7
+    world = 4 << depth
+    d = _dag(depth, materials, node_words=1 << 22, leaf_words=1 << 23)
+    ref = O.Dag(depth, materials)
+    d.build_terrain(terrain_for_world(world, 3))
+    ref.build_terrain(O.terrain_for_world(world, 3))
+    for i in range(4):
+        br = O.gen_brushes(40 + i, O.STREAM_BRUSH, 48, world, i % 2)
+        d.apply_brushes(br)
+        ref.apply_brushes(br)
+    before = d.export()
+    assert np.array_equal(before, ref.export())
+    q = _dev_xyz(O.gen_u32(9, O.STREAM_KEYS, 20000 * 3, 0, world - 1), gpu)
+    v0 = d.get_voxels(q).cpu()
+    st0 = d.stats()
+    r = d.collect_garbage()
+    st1 = d.stats()
+    assert np.array_equal(d.export(), before)
+    assert torch.equal(d.get_voxels(q).cpu(), v0)
+    assert st1.node_words_used < st0.node_words_used and st1.leaf_words_used < st0.leaf_words_used
+    assert (r.node_words_before, r.leaf_words_before) == (st0.node_words_used, st0.leaf_words_used)
+    assert (r.node_words_after, r.leaf_words_after) == (st1.node_words_used, st1.leaf_words_used)
+    n_nodes, n_leaves = _assert_compact(d)
+    assert r.live_leaves == n_leaves == before[5]
+    assert r.live_nodes >= n_nodes
+    # Idempotent: over compact pools every level's live list is in pool
+    # order, the rebuild places items in list order, so each ref maps to
+    # itself and the pools come out word for word the same.
+    pools1 = d.read_pools()
+    d.collect_garbage()
+    pools2 = d.read_pools()
+    assert np.array_equal(pools1[0], pools2[0]) and np.array_equal(pools1[1], pools2[1])
+    assert d.stats().root == st1.root
+    # later edits go through the rebuilt tables
+    for i in range(3):
+        br = O.gen_brushes(60 + i, O.STREAM_BRUSH, 48, world, 1)
+        d.apply_brushes(br)
+        ref.apply_brushes(br)
+        assert np.array_equal(d.export(), ref.export()), i
+    _assert_items_unique(d)
+
+
+def test_gc_large_scene_keeps_field(gpu):
+    """2048^3 materials terrain after three brush batches."""
+    from voxel_amd.dag import scene_capacity, terrain_for_world
+    depth = 9
+    world = 4 << depth
+    d = _dag(depth, True, **scene_capacity(depth, True, edit_leaves=8e6))
+    d.build_terrain(terrain_for_world(world, 11))
+    for i in range(3):
+        d.apply_brushes(O.gen_brushes(70 + i, O.STREAM_BRUSH, 400, world, 1))
+    before = d.export()
+    st0 = d.stats()
+    d.collect_garbage()
+    st1 = d.stats()
+    assert np.array_equal(d.export(), before)
+    assert st1.leaf_words_used == POOL_START + 16 * int(before[5])
+    assert st1.node_words_used < st0.node_words_used
+    d.apply_brushes(O.gen_brushes(80, O.STREAM_BRUSH, 500, world, 0))
+    _assert_items_unique(d)
+
+
+def test_batcher_survives_gc(gpu):
+    """GC writes back into the scene's own buffers, so a Batcher captured
+    before it keeps working."""
+    from voxel_amd.dag import Batcher, batch_candidates, terrain_for_world
+    depths = [5, 6]
+    a = [_dag(x, True, node_words=1 << 22, leaf_words=1 << 23) for x in depths]
+    b = [_dag(x, True, node_words=1 << 22, leaf_words=1 << 23) for x in depths]
+    for k, x in enumerate(depths):
+        a[k].build_terrain(terrain_for_world(4 << x, 50 + k))
+        b[k].build_terrain(terrain_for_world(4 << x, 50 + k))
+    rounds = [[(s, O.gen_brushes(300 + 10 * r + i, O.STREAM_BRUSH, 6, 4 << depths[s], 0))
+               for i, s in enumerate([0, 1, 1, 0])] for r in range(3)]
+    cap = max(batch_candidates(a, rb) for rb in rounds)
+    ex = Batcher(a, cap, 64, use_graph=True)
+    for r, rb in enumerate(rounds):
+        ex.run(rb)
+        for s, br in rb:
+            b[s].apply_brushes(br)
+        for k in range(2):
+            assert np.array_equal(a[k].export(), b[k].export()), (r, k)
+        a[r % 2].collect_garbage()
+        assert np.array_equal(a[r % 2].export(), b[r % 2].export())

@@ -171,6 +171,34 @@ struct ItemSet {
   }
 };
 
+// Scratch device buffer owned by a host scope (builders and maintenance calls
+// that size their temporaries on the host).
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <class T>
+  T* get() const {
+    return static_cast<T*>(p);
+  }
+};
+
+inline vx_status dev_alloc(DevBuf& b, size_t bytes) {
+  return hipMalloc(&b.p, bytes > 256 ? bytes : 256) == hipSuccess ? VX_OK : VX_OUT_OF_CAPACITY;
+}
+
+inline uint32_t read_u32(const uint32_t* p, hipStream_t s) {
+  uint32_t v = 0;
+  (void)hipMemcpyAsync(&v, p, 4, hipMemcpyDeviceToHost, s);
+  (void)hipStreamSynchronize(s);
+  return v;
+}
+
 // Probe + commit one level's items into the node (leaves=false) or leaf
 // table. res[i] must be FINAL (sentinel) or PENDING (hashes[i] set) on entry;
 // every entry is FINAL or DUP afterwards, and hashes[] is overwritten. fused_ws holds

@@ -263,28 +263,6 @@ __global__ void root_class_kernel(Pyr p, uint32_t* out) {
   *out = classify(p, 0, 0, &y0);
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() {
-    if (p) hipFree(p);
-  }
-  template <class T>
-  T* get() const {
-    return static_cast<T*>(p);
-  }
-};
-
-vx_status dev_alloc(DevBuf& b, size_t bytes) {
-  return hipMalloc(&b.p, std::max<size_t>(bytes, 256)) == hipSuccess ? VX_OK : VX_OUT_OF_CAPACITY;
-}
-
-uint32_t read_u32(const uint32_t* p, hipStream_t s) {
-  uint32_t v = 0;
-  hipMemcpyAsync(&v, p, 4, hipMemcpyDeviceToHost, s);
-  hipStreamSynchronize(s);
-  return v;
-}
-
 }  // namespace
 
 extern "C" vx_status vx_dag_build_terrain(vx_dag* g, const vx_terrain* t, void* stream) {

@@ -36,6 +36,12 @@ class Stats(C.Structure):
                 ("leaf_words", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class GcReport(C.Structure):
+    _fields_ = [("node_words_before", C.c_uint64), ("node_words_after", C.c_uint64),
+                ("leaf_words_before", C.c_uint64), ("leaf_words_after", C.c_uint64), ("live_nodes", C.c_uint64),
+                ("live_leaves", C.c_uint64)]
+
+
 BRUSH_DTYPE = np.dtype([("cx", "<i4"), ("cy", "<i4"), ("cz", "<i4"), ("hx", "<i4"), ("hy", "<i4"), ("hz", "<i4"),
                         ("kind", "<u4"), ("material", "<u4")])
 
@@ -54,6 +60,8 @@ register({
     "vx_dag_get_voxels": (_i32, [_P, _P, _u32, _P, _P]),
     "vx_dag_raycast": (_i32, [_P, _P, _u32, _P, _P]),
     "vx_dag_export_canonical": (_i32, [_P, _P, _u64, C.POINTER(_u64)]),
+    "vx_dag_collect_garbage": (_i32, [_P, _P, _P]),
+    "vx_dag_read_pools": (_i32, [_P, _P, _u64, C.POINTER(_u64), _P, _u64, C.POINTER(_u64)]),
     "vx_brush_voxel_ops": (_i32, [_P, _u32, _u32, C.POINTER(_u64)]),
     "vx_gen_brushes": (_i32, [_u64, _u64, _u32, _u32, _u32, _P]),
     "vx_gen_sized_batch": (_i32, [_u64, _u64, _u64, _u32, _u32, _u32, _u32, _u32, _P, C.POINTER(_u32),
@@ -244,6 +252,23 @@ class HashDagGpu:
         p = Profile()
         check(lib().vx_dag_profile_read(self.h, C.byref(p)), "vx_dag_profile_read")
         return p
+
+    # -- maintenance ---------------------------------------------------------------
+    def collect_garbage(self, stream=None) -> GcReport:
+        """Keep only the items reachable from the root (HashDAG::collect_garbage,
+        svt/hash_dag.hpp:133-191); the voxel field is unchanged."""
+        r = GcReport()
+        check(lib().vx_dag_collect_garbage(self.h, C.byref(r), _stream(stream)), "vx_dag_collect_garbage")
+        return r
+
+    def read_pools(self) -> tuple[np.ndarray, np.ndarray]:
+        """Host copies of the used words of the node and leaf pools."""
+        nn, ln = C.c_uint64(), C.c_uint64()
+        check(lib().vx_dag_read_pools(self.h, None, 0, C.byref(nn), None, 0, C.byref(ln)), "pool sizes")
+        npool, lpool = np.empty(nn.value, np.uint32), np.empty(ln.value, np.uint32)
+        check(lib().vx_dag_read_pools(self.h, npool.ctypes.data_as(_P), nn.value, C.byref(nn),
+                                      lpool.ctypes.data_as(_P), ln.value, C.byref(ln)), "vx_dag_read_pools")
+        return npool, lpool
 
     def export(self) -> np.ndarray:
         n = C.c_uint64()
