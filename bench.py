@@ -206,6 +206,24 @@ def ray_leg(torch, scene, world, dev, n=1 << 20):
             "mean_steps_per_hit": float(hits[hit, 1].double().mean()) if bool(hit.any()) else 0.0}
 
 
+def gc_leg(torch, scene, world, dev, n=1 << 20):
+    """Garbage collection of the edited scene (a synchronous call: wall
+    clock), with 2^20 random voxels read before and after as a check that the
+    field is unchanged."""
+    g = torch.Generator(device="cpu").manual_seed(23)
+    q = torch.randint(0, world, (n, 3), generator=g, dtype=torch.int64).to(torch.int32).to(dev)
+    v0 = scene.get_voxels(q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = scene.collect_garbage()
+    ms = (time.perf_counter() - t0) * 1e3
+    same = bool(torch.equal(scene.get_voxels(q), v0))
+    return {"workload": f"collect_garbage on the {world}^3 scene after the timed edit steps", "ms": ms,
+            "node_words": [r.node_words_before, r.node_words_after],
+            "leaf_words": [r.leaf_words_before, r.leaf_words_after],
+            "live_node_items": r.live_nodes, "live_leaf_items": r.live_leaves, "sampled_field_unchanged": same}
+
+
 def hash_tables(torch, P):
     """Config 1d: 2^26 unique u64 keys (seed 2) into tables sized for load 0.8
     (slab set and cuckoo set), then 2^26 lookups, half from the inserted set
@@ -535,6 +553,7 @@ def main():
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
 
     rays = ray_leg(torch, scene, world, dev)
+    gc = gc_leg(torch, scene, world, dev)
     scene.close()
     del ws
     torch.cuda.empty_cache()
@@ -606,6 +625,7 @@ def main():
         "scene": {"build_s": build_s, "node_words": st.node_words_used, "leaf_words": st.leaf_words_used},
         "primitives": prims,
         "rays": rays,
+        "gc": gc,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu_res,

@@ -3,11 +3,16 @@ for HashDAG::collect_garbage (test/svt/test_svt_hash_dag.cpp:98-135), the
 voxel field unchanged (canonical export and point queries), the pools holding
 exactly the reachable items, each once, and later edits still matching the
 oracle with maximal sharing (the rebuilt slot tables find every live item)."""
+import json
+import os
+
 import numpy as np
 import pytest
 import torch
 
 from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -136,6 +141,30 @@ def test_gc_keeps_root_and_full_subtrees(gpu):
     assert (st.root, st.node_words_used, st.leaf_words_used) == (1, POOL_START, POOL_START)
 
 
+def test_gc_live_counts_match_reference_hash_dag(gpu):
+    """Box scripts recorded from the reference's own HashDAG: after GC the
+    live (level, node) pairs and live leaves equal its canonical counts (its
+    node_count / leaf_count after collect_garbage less the retained full
+    chain), and the export digest is unchanged."""
+    with open(os.path.join(GOLDEN, "ref_hash_dag.json")) as f:
+        cases = json.load(f)
+    for case in cases:
+        d = _dag(case["depth"], False)
+        for op in case["ops"]:
+            x0, y0, z0, x1, y1, z1, fill = op
+            z, y, x = np.mgrid[z0:z1, y0:y1, x0:x1]
+            vox = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+            if fill:
+                d.place_voxel_edits(_dev_xyz(vox, gpu))
+            else:
+                d.destroy_voxel_edits(_dev_xyz(vox, gpu))
+        assert O.digest(d.export()) == int(case["digest"], 16), case["name"]
+        r = d.collect_garbage()
+        assert (r.live_nodes, r.live_leaves) == (case["nodes"], case["leaves"]), case["name"]
+        assert O.digest(d.export()) == int(case["digest"], 16), case["name"]
+        _assert_compact(d)
+
+
 def test_gc_sentinel_children_only(gpu):
     """A live node whose children are all FULL / EMPTY (child mask 0): nothing
     lives below it."""
@@ -180,7 +209,7 @@ def test_gc_after_edits_keeps_field_and_later_edits_match_oracle(gpu, materials)
     assert (r.node_words_after, r.leaf_words_after) == (st1.node_words_used, st1.leaf_words_used)
     n_nodes, n_leaves = _assert_compact(d)
     assert r.live_leaves == n_leaves == before[5]
-    assert r.live_nodes >= n_nodes
+    assert r.live_nodes == before[4] >= n_nodes
     # Idempotent: over compact pools every level's live list is in pool
     # order, the rebuild places items in list order, so each ref maps to
     # itself and the pools come out word for word the same.
