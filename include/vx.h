@@ -332,6 +332,68 @@ vx_status vx_dag_collect_garbage(vx_dag* dag, vx_gc_report* report, void* stream
 vx_status vx_dag_read_pools(const vx_dag* dag, uint32_t* node_out, uint64_t node_cap, uint64_t* node_n,
                             uint32_t* leaf_out, uint64_t leaf_cap, uint64_t* leaf_n);
 
+/* ---- pointer SVO voxel edits (algo::svt::cuda) -------------------------------
+ * The mutable sparse voxel octree of svt/cuda: 4096^3 world, ten inner
+ * levels of 2x2x2 (config.cuh:16-36); nodes of eight u32 slots {child
+ * index:30 | pointer bit 30 | filled bit 31}, leaves of two u32 masks (bit
+ * x + 4y + 16z of the 4^3 block), LIFO free-index stacks. */
+typedef struct vx_svo vx_svo;
+/* GpuSvoCounters (svt/cuda/device_view.cuh:24-30). */
+typedef struct vx_svo_counters {
+  uint32_t node_count, leaf_count, free_node_count, free_leaf_count;
+} vx_svo_counters;
+/* LeafMask (svt/cuda/voxel/types.cuh:20-25). */
+typedef struct vx_leaf_mask {
+  uint32_t leaf_key, voxel_data_low, voxel_data_high;
+} vx_leaf_mask;
+/* DeviceGpuSvo (svt/cuda/device_view.cuh:34-42): device pointers for kernels
+ * of the caller's own. */
+typedef struct vx_svo_view {
+  uint32_t* nodes; /* 8 slots per node */
+  uint32_t* leaves; /* 2 words per leaf */
+  vx_svo_counters* counters;
+  uint32_t* free_node_indices;
+  uint32_t* free_leaf_indices;
+  uint32_t max_node_count, max_leaf_count;
+} vx_svo_view;
+/* GpuSvo<MaxNodeCount, MaxLeafCount> (svt/cuda/storage.cuh:12-122): capacities
+ * are runtime values here. Allocation only; call vx_svo_reset before use. */
+vx_status vx_svo_create(uint32_t max_node_count, uint32_t max_leaf_count, vx_svo** out);
+vx_status vx_svo_destroy(vx_svo* svo);
+/* reset_svo (svt/cuda/svo.cuh:14-22, detail/reset.cuh:9-21): empty root,
+ * counters {1, 0, 0, 0}. */
+vx_status vx_svo_reset(vx_svo* svo, void* stream);
+/* GpuSvo::view() (svt/cuda/storage.cuh:40-49). */
+vx_status vx_svo_view_get(const vx_svo* svo, vx_svo_view* out);
+/* Host copy of the counters (synchronises the stream). */
+vx_status vx_svo_counters_get(const vx_svo* svo, vx_svo_counters* out, void* stream);
+/* voxel_edits_to_leaf_masks[_workspace_size] (svt/cuda/voxel/edit.cuh:14-41):
+ * one mask per touched leaf, sorted by leaf key; out-of-world edits dropped;
+ * `leaf_masks` holds `count` entries, *leaf_count (device) receives the
+ * number written. */
+vx_status vx_voxel_edits_to_leaf_masks_workspace_size(uint32_t count, size_t* bytes);
+vx_status vx_voxel_edits_to_leaf_masks(const vx_voxel* d_edits, uint32_t count, vx_leaf_mask* d_leaf_masks,
+                                       uint32_t* d_leaf_count, void* workspace, size_t workspace_bytes, void* stream);
+/* apply_voxel_edits_workspace_size / place_voxel_edits / destroy_voxel_edits
+ * (svt/cuda/voxel/edit.cuh:43-92): op 0 Place, 1 Destroy. Paths to touched
+ * leaves are materialised (new slots inherit the replaced slot's fill), the
+ * leaf bits set or cleared, and uniform leaves / nodes on the touched paths
+ * collapsed into filled / empty slots, deepest first, their indices pushed on
+ * the free stacks. Synchronises the stream once (the reference's default
+ * HostLeafCountDispatch does too). If the batch needs more nodes or leaves
+ * than the capacity holds it returns VX_OUT_OF_CAPACITY and changes nothing
+ * (the reference skips the allocations that do not fit). */
+vx_status vx_svo_voxel_edits_workspace_size(uint32_t count, size_t* bytes);
+vx_status vx_svo_apply_voxel_edits(vx_svo* svo, const vx_voxel* d_edits, uint32_t count, uint32_t op,
+                                   void* workspace, size_t workspace_bytes, void* stream);
+/* get_voxel (svt/cuda/query.cuh:12-39), batched: out[i] = 1 if set. */
+vx_status vx_svo_get_voxels(const vx_svo* svo, const vx_voxel* d_xyz, uint32_t n, uint32_t* d_out, void* stream);
+/* Canonical export (host): [0x5658534F, node count, leaf count], 8 slot words
+ * per node in pre-order from the root (a pointer slot holds the child's
+ * canonical id with the slot's own flag bits), then 2 words per leaf in the
+ * order the walk reaches them. *n_words receives the size. */
+vx_status vx_svo_export_canonical(const vx_svo* svo, uint32_t* out, uint64_t cap, uint64_t* n_words);
+
 /* Seeded brush batch (host output). mix 0: spheres r in [4,16], 50% add;
  * mix 1: 50% spheres r in [8,64] / 50% boxes half extents in [4,48], 60% add;
  * added material uniform in [1,255]. Brush i uses draws 16i..16i+8 of the

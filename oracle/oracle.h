@@ -124,6 +124,32 @@ uint64_t vxo_dag_stored_leaves(const vxo_dag* d);
 uint64_t vxo_dense_export(const uint8_t* grid, uint32_t inner_depth,
                           int materials, uint32_t* out, uint64_t cap);
 
+/* ---- CPU pointer SVO (restatement of svt/cuda's voxel edit) -------------
+ * 4096^3 world, ten inner levels, 8-slot nodes (child index | bit 30 pointer
+ * | bit 31 filled), 4^3 leaves as two u32 masks, LIFO free lists. Pinned by
+ * the reference's own SVO tests (the .cuh files under test/cuda/svt, re-expressed on the
+ * CPU) and by invariants: get() equals a set model of the field, live
+ * counters equal the reachable counts, no uniform node on a touched path. */
+typedef struct vxo_svo vxo_svo;
+vxo_svo* vxo_svo_create(uint32_t max_nodes, uint32_t max_leaves);
+void vxo_svo_destroy(vxo_svo* s);
+void vxo_svo_reset(vxo_svo* s);
+/* Leaf masks of a voxel batch, sorted by leaf key: out = 3 words per mask
+ * {leaf_key, voxel_data_low, voxel_data_high}; returns the mask count. */
+uint32_t vxo_svo_leaf_masks(const uint32_t* xyz, uint32_t n, uint32_t* out);
+/* op 0 place, 1 destroy; returns 0, or 2 (out of capacity, nothing changed). */
+int vxo_svo_apply(vxo_svo* s, const uint32_t* xyz, uint32_t n, uint32_t op);
+uint32_t vxo_svo_get(const vxo_svo* s, uint32_t x, uint32_t y, uint32_t z);
+/* {node_count, leaf_count, free_node_count, free_leaf_count} */
+void vxo_svo_counters(const vxo_svo* s, uint32_t* out);
+/* Canonical export: [0x5658534F, nodes, leaves], 8 slot words per node in
+ * pre-order (pointer slots: canonical child id | the slot's flag bits), then
+ * 2 words per leaf in the order reached. */
+uint64_t vxo_svo_export(const vxo_svo* s, uint32_t* out, uint64_t cap);
+uint32_t vxo_svo_leaf_key(uint32_t lx, uint32_t ly, uint32_t lz);
+/* The reference benchmark's random voxel batch (xorshift32 from 0x9e3779b9). */
+void vxo_svo_random_edits(uint32_t count, uint32_t* xyz);
+
 /* FNV-1a 64 over u32 words (digest of a canonical export). */
 uint64_t vxo_digest(const uint32_t* words, uint64_t n);
 

@@ -90,6 +90,16 @@ def lib():
             "vxo_dag_stored_leaves": (u64, [P]),
             "vxo_dense_export": (u64, [P, u32, C.c_int, P, u64]),
             "vxo_digest": (u64, [P, u64]),
+            "vxo_svo_create": (P, [u32, u32]),
+            "vxo_svo_destroy": (None, [P]),
+            "vxo_svo_reset": (None, [P]),
+            "vxo_svo_leaf_masks": (u32, [P, u32, P]),
+            "vxo_svo_apply": (C.c_int, [P, P, u32, u32]),
+            "vxo_svo_get": (u32, [P, u32, u32, u32]),
+            "vxo_svo_counters": (None, [P, P]),
+            "vxo_svo_export": (u64, [P, P, u64]),
+            "vxo_svo_leaf_key": (u32, [u32, u32, u32]),
+            "vxo_svo_random_edits": (None, [u32, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -319,4 +329,73 @@ class RefDag:
         n = ref().vxr_export(self.h, None, 0)
         out = np.empty(n, np.uint32)
         ref().vxr_export(self.h, _ptr(out), n)
+        return out
+
+
+# ---- pointer SVO ----------------------------------------------------------------
+SVO_WORLD = 4096
+SVO_MAGIC = 0x5658534F
+
+
+def svo_leaf_key(lx: int, ly: int, lz: int) -> int:
+    return int(lib().vxo_svo_leaf_key(lx, ly, lz))
+
+
+def svo_random_edits(count: int) -> np.ndarray:
+    out = np.empty((count, 3), np.uint32)
+    lib().vxo_svo_random_edits(count, _ptr(out))
+    return out
+
+
+def svo_leaf_masks(xyz) -> np.ndarray:
+    """(n, 3) u32 voxels -> (m, 3) u32 {leaf_key, low, high}, sorted by key."""
+    a = np.ascontiguousarray(xyz, np.uint32).reshape(-1, 3)
+    out = np.empty((max(len(a), 1), 3), np.uint32)
+    m = lib().vxo_svo_leaf_masks(_ptr(a), len(a), _ptr(out))
+    return out[:m].copy()
+
+
+class Svo:
+    """CPU pointer SVO (restatement of algo::svt::cuda's voxel edit)."""
+
+    def __init__(self, max_nodes: int = 1 << 20, max_leaves: int = 1 << 20):
+        self.h = lib().vxo_svo_create(max_nodes, max_leaves)
+        if not self.h:
+            raise ValueError("bad capacity")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().vxo_svo_destroy(self.h)
+            self.h = None
+
+    def reset(self):
+        lib().vxo_svo_reset(self.h)
+
+    def apply(self, xyz, op: int) -> int:
+        a = np.ascontiguousarray(xyz, np.uint32).reshape(-1, 3)
+        return int(lib().vxo_svo_apply(self.h, _ptr(a), len(a), op))
+
+    def place(self, xyz) -> int:
+        return self.apply(xyz, 0)
+
+    def destroy(self, xyz) -> int:
+        return self.apply(xyz, 1)
+
+    def get(self, x: int, y: int, z: int) -> int:
+        return int(lib().vxo_svo_get(self.h, x, y, z))
+
+    def get_many(self, xyz) -> np.ndarray:
+        a = np.asarray(xyz, np.int64).reshape(-1, 3)
+        return np.array([self.get(int(x) & 0xFFFFFFFF, int(y) & 0xFFFFFFFF, int(z) & 0xFFFFFFFF)
+                         for x, y, z in a], np.uint32)
+
+    def counters(self) -> tuple[int, int, int, int]:
+        out = np.zeros(4, np.uint32)
+        lib().vxo_svo_counters(self.h, _ptr(out))
+        return tuple(int(v) for v in out)
+
+    def export(self) -> np.ndarray:
+        n = lib().vxo_svo_export(self.h, None, 0)
+        out = np.empty(n, np.uint32)
+        lib().vxo_svo_export(self.h, _ptr(out), n)
         return out

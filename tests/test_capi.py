@@ -16,6 +16,7 @@ def test_every_declared_entry_point_has_a_python_signature():
     import voxel_amd.dag  # noqa: F401
     import voxel_amd.hash  # noqa: F401
     import voxel_amd.primitives  # noqa: F401
+    import voxel_amd.svo  # noqa: F401
     names = set(_lib.header_functions())
     assert names <= set(_lib.SIGNATURES), sorted(names - set(_lib.SIGNATURES))
 
