@@ -68,12 +68,15 @@ def parse():
     ap.add_argument("--no-primitives", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-svo", action="store_true", help="skip the pointer-SVO leg")
     ap.add_argument("--cpu-brushes", type=int, default=1024, help="brushes in the CPU baseline sample")
     ap.add_argument("--no-parity", action="store_true",
                     help="skip the full-size check (CPU oracle finishes batch 0; canonical digests compared)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarise two rocprofv3 --pmc counter CSVs of this benchmark (FETCH_SIZE, WRITE_SIZE) "
                          "into the per-kernel traffic file used for roofline.traffic, then exit")
+    ap.add_argument("--kernel-trace", metavar="TRACE_CSV",
+                    help="summarise a rocprofv3 --kernel-trace CSV of this benchmark per edit kernel kind and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="finish with one 1 GiB read + 1 GiB write elementwise kernel (bitwise_not on int32), "
                          "the known-size reference for hardware byte counters")
@@ -222,6 +225,48 @@ def gc_leg(torch, scene, world, dev, n=1 << 20):
             "node_words": [r.node_words_before, r.node_words_after],
             "leaf_words": [r.leaf_words_before, r.leaf_words_after],
             "live_node_items": r.live_nodes, "live_leaf_items": r.live_leaves, "sampled_field_unchanged": same}
+
+
+# The reference's published pointer-SVO figures (algorithm/README.md:43-48, best
+# allocation policy CompactAllDepth, HostLeafCountDispatch; RTX 3070 — other
+# hardware, context only).
+SVO_REFERENCE_MS = {"sphere_r128": 5.00, "random_2^24": 43.55}
+
+
+def svo_leg(torch, reps=5):
+    """Pointer-SVO voxel edits (svt/cuda API) on the reference benchmark's two
+    workloads (bench/bench_cuda_svt.cu:46-112, GpuSvo<2^26, 2^24>): a radius-128
+    sphere at the world centre (8.78M voxels, 144k leaves) and 2^24 xorshift
+    voxels (15.8M leaves), each placed on a freshly reset SVO; hipEvent time of
+    the place call (its one host synchronisation included), median of `reps`
+    after one warm-up."""
+    from voxel_amd import svo as S
+    dev = torch.device("cuda:0")
+    s = S.GpuSvo(1 << 26, 1 << 24)
+    out = {}
+    for name, xyz in (("sphere_r128", S.sphere_edits(2048, 2048, 2048, 128)),
+                      ("random_2^24", S.random_edits(1 << 24))):
+        e = S._edits(xyz, dev)
+        n = int(e.shape[0])
+        ws = torch.empty(S.GpuSvo.apply_voxel_edits_workspace_size(n), dtype=torch.uint8, device=dev)
+        times = []
+        for r in range(reps + 1):
+            s.reset()
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            s.place_voxel_edits(e, workspace=ws)
+            b.record()
+            b.synchronize()
+            if r:
+                times.append(a.elapsed_time(b))
+        ms = sorted(times)[len(times) // 2]
+        c = s.counters()
+        out[name] = {"voxels": n, "ms": ms, "voxels_per_s": n / ms * 1e3, "node_count": c[0], "leaf_count": c[1],
+                     "reference_rtx3070_ms": SVO_REFERENCE_MS[name]}
+        del e, ws
+    s.close()
+    return out
 
 
 def hash_tables(torch, P):
@@ -396,7 +441,7 @@ def _kind_of(name, last_probe):
     """Edit kernel kind of a device symbol (see KERNEL_SYMBOL)."""
     table = [("BrushLoad", "records"), ("VoxelLoad", "records"), ("histogram_kernel", "sort_histogram"),
              ("hist_scan_kernel", "sort_scan"), ("partition_kernel", "sort_partition"), ("copy_kernel", "sort_copy"),
-             ("RunLoad", "runs"), ("UpLoad", "up"), ("topdown_kernel", "topdown"), ("leaf_build_kernel", "leaf_build"),
+             ("RunUpLoad", "runs"), ("UpLoad", "up"), ("topdown_kernel", "topdown"), ("leaf_build_kernel", "leaf_build"),
              ("node_build_kernel", "node_build"), ("publish_kernel", "publish"), ("fill_u32_kernel", "fill"),
              ("begin_kernel", "other")]
     for key, kind in (("tail_up_kernel", "tail_up"), ("tail_topdown_kernel", "tail_topdown"),
@@ -413,6 +458,58 @@ def _kind_of(name, last_probe):
         if key in name:
             return kind
     return None
+
+
+def kernel_trace_summary(trace_csv):
+    """Per-kind table of a rocprofv3 --kernel-trace CSV of this benchmark:
+    an edit runs from its begin kernel to its publish kernel, the first edit
+    is dropped as warm-up, and each kind gets launches per edit, the average
+    launch duration and the time per edit; the per-launch durations of
+    node_build (one per height) are listed too. Prints markdown rows."""
+    import csv
+    rows = list(csv.DictReader(open(trace_csv)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    rows = [(r["Kernel_Name"], (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3) for r in rows]
+    edits, cur = [], None
+    for name, us in rows:
+        if "begin_kernel" in name:
+            cur = []
+        if cur is not None:
+            cur.append((name, us))
+            if "publish_kernel" in name:
+                edits.append(cur)
+                cur = None
+    edits = edits[1:]
+    kinds, order = {}, []
+    per_height = []
+    for e in edits:
+        last_probe, heights = None, []
+        for name, us in e:
+            kind = _kind_of(name, last_probe)
+            if kind in ("leaf_probe", "node_probe"):
+                last_probe = kind
+            if kind is None:
+                continue
+            if kind not in kinds:
+                kinds[kind] = [0, 0.0, name]
+                order.append(kind)
+            kinds[kind][0] += 1
+            kinds[kind][1] += us
+            if kind == "node_build":
+                heights.append(us)
+        per_height.append(heights)
+    ne = max(len(edits), 1)
+    print(f"edits: {len(edits)} (after the warm-up)")
+    print("| kind | launches/edit | avg us | ms/edit | device symbol |")
+    print("|---|---|---|---|---|")
+    for k in sorted(order, key=lambda k: -kinds[k][1]):
+        n, us, name = kinds[k]
+        print(f"| {k} | {n / ne:g} | {us / max(n, 1):.1f} | {us / ne / 1e3:.3f} | `{name[:90]}` |")
+    total = sum(v[1] for v in kinds.values()) / ne / 1e3
+    print(f"\nkernel time per edit: {total:.3f} ms")
+    if per_height and per_height[0]:
+        avg = [sum(h[i] for h in per_height if len(h) > i) / len(per_height) for i in range(len(per_height[0]))]
+        print("node_build per launch (height 1 first), us: " + ", ".join(f"{x:.1f}" for x in avg))
 
 
 def traffic_summary(fetch_csv, write_csv, out_path):
@@ -469,6 +566,9 @@ def main():
     args = parse()
     if args.traffic:
         traffic_summary(args.traffic[0], args.traffic[1], TRAFFIC_FILE)
+        return
+    if args.kernel_trace:
+        kernel_trace_summary(args.kernel_trace)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: every box here has one GPU, so --gpus {args.gpus} is not run "
@@ -557,6 +657,8 @@ def main():
     scene.close()
     del ws
     torch.cuda.empty_cache()
+    svo = None if args.no_svo else svo_leg(torch)
+    torch.cuda.empty_cache()
     prims = {} if args.no_primitives else primitives(torch, P)
     torch.cuda.empty_cache()
     if not args.no_primitives:
@@ -626,6 +728,7 @@ def main():
         "primitives": prims,
         "rays": rays,
         "gc": gc,
+        "svo": svo,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu_res,

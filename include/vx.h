@@ -409,6 +409,11 @@ vx_status vx_gen_sized_batch(uint64_t seed, uint64_t stream_id, uint64_t index, 
                              uint32_t max_brushes, uint32_t world, uint32_t n_scenes, vx_brush* out, uint32_t* n_out,
                              uint32_t* scene_out);
 
+/* Random voxel edits of the pointer-SVO benchmark (make_random_edits,
+ * bench/bench_cuda_svt.cu:90-112): xorshift32 (13, 17, 5) from 0x9e3779b9,
+ * three draws per voxel, each mod `world` (host output). */
+vx_status vx_gen_xorshift_voxels(uint32_t count, uint32_t world, vx_voxel* out);
+
 /* Exact number of voxel operations of a brush batch: sum over brushes of the
  * voxels each covers inside the world (overlaps count once per brush). */
 vx_status vx_brush_voxel_ops(const vx_brush* h_brushes, uint32_t n, uint32_t world, uint64_t* ops);

@@ -96,3 +96,13 @@ def test_capacity_overflow_changes_nothing():
     e, c = s.export(), s.counters()
     assert s.place([(4 * i, 0, 0) for i in range(1, 8)]) == 2
     assert np.array_equal(s.export(), e) and s.counters() == c
+
+
+def test_product_workload_generators():
+    """The library's generators of the reference benchmark workloads (used by
+    the benchmark) equal the oracle's and the reference's stated sizes."""
+    from voxel_amd import svo
+    assert np.array_equal(svo.random_edits(1 << 16), O.svo_random_edits(1 << 16))
+    sph = svo.sphere_edits(2048, 2048, 2048, 128)
+    assert len(sph) == 8782785
+    assert np.array_equal(sph, S.sphere_edits(2048, 2048, 2048, 128))

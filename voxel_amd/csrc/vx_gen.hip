@@ -177,3 +177,25 @@ extern "C" vx_status vx_gen_sized_batch(uint64_t seed, uint64_t stream_id, uint6
   *n_out = n;
   return VX_OK;
 }
+
+// Random voxel edits of the pointer-SVO benchmark (bench/bench_cuda_svt.cu:
+// 90-112): one xorshift32 state (13, 17, 5) seeded 0x9e3779b9; voxel i takes
+// three consecutive draws, each reduced mod `world`, as x, y, z (host output;
+// the stream is sequential by construction).
+extern "C" vx_status vx_gen_xorshift_voxels(uint32_t count, uint32_t world, vx_voxel* out) {
+  if ((!out && count) || world == 0) return VX_INVALID_VALUE;
+  uint32_t s = 0x9e3779b9u;
+  auto next = [&s]() {
+    s ^= s << 13;
+    s ^= s >> 17;
+    s ^= s << 5;
+    return s;
+  };
+  for (uint32_t i = 0; i < count; ++i) {
+    const uint32_t x = next() % world;
+    const uint32_t y = next() % world;
+    const uint32_t z = next() % world;
+    out[i] = vx_voxel{x, y, z};
+  }
+  return VX_OK;
+}

@@ -49,6 +49,7 @@ register({
     "vx_svo_apply_voxel_edits": (_i32, [_P, _P, _u32, _u32, _P, _sz, _P]),
     "vx_svo_get_voxels": (_i32, [_P, _P, _u32, _P, _P]),
     "vx_svo_export_canonical": (_i32, [_P, _P, _u64, C.POINTER(_u64)]),
+    "vx_gen_xorshift_voxels": (_i32, [_u32, _u32, _P]),
 })
 
 
@@ -58,6 +59,23 @@ def _edits(xyz, device="cuda") -> torch.Tensor:
         return xyz.to(device=device, dtype=torch.int32).contiguous()
     a = np.ascontiguousarray(np.asarray(xyz, np.int64).reshape(-1, 3) & 0xFFFFFFFF, np.uint32)
     return torch.from_numpy(a.view(np.int32)).to(device)
+
+
+def sphere_edits(cx: int, cy: int, cz: int, r: int) -> np.ndarray:
+    """(n, 3) u32 voxels of the solid sphere |v - c|^2 <= r^2 in z, y, x order
+    (make_voxel_sphere_edits, bench/bench_cuda_svt.cu:56-80)."""
+    d = np.arange(-r, r + 1, dtype=np.int64)
+    dz, dy, dx = np.meshgrid(d, d, d, indexing="ij")
+    inside = dx * dx + dy * dy + dz * dz <= r * r
+    return np.stack([(cx + dx)[inside], (cy + dy)[inside], (cz + dz)[inside]], 1).astype(np.uint32)
+
+
+def random_edits(count: int, world: int = WORLD) -> np.ndarray:
+    """(count, 3) u32 voxels of the reference SVO benchmark's random workload
+    (make_random_edits, bench/bench_cuda_svt.cu:90-112)."""
+    out = np.empty((count, 3), np.uint32)
+    check(lib().vx_gen_xorshift_voxels(count, world, out.ctypes.data_as(_P)), "vx_gen_xorshift_voxels")
+    return out
 
 
 def voxel_edits_to_leaf_masks_workspace_size(count: int) -> int:
