@@ -75,8 +75,9 @@ def parse():
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarise two rocprofv3 --pmc counter CSVs of this benchmark (FETCH_SIZE, WRITE_SIZE) "
                          "into the per-kernel traffic file used for roofline.traffic, then exit")
-    ap.add_argument("--kernel-trace", metavar="TRACE_CSV",
-                    help="summarise a rocprofv3 --kernel-trace CSV of this benchmark per edit kernel kind and exit")
+    ap.add_argument("--kernel-trace", metavar="TRACE",
+                    help="summarise a rocprofv3 --kernel-trace output (CSV or rocpd .db) of this benchmark per edit "
+                         "kernel kind and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="finish with one 1 GiB read + 1 GiB write elementwise kernel (bitwise_not on int32), "
                          "the known-size reference for hardware byte counters")
@@ -461,15 +462,21 @@ def _kind_of(name, last_probe):
 
 
 def kernel_trace_summary(trace_csv):
-    """Per-kind table of a rocprofv3 --kernel-trace CSV of this benchmark:
+    """Per-kind table of a rocprofv3 --kernel-trace output (CSV, or the rocpd
+    SQLite database) of this benchmark:
     an edit runs from its begin kernel to its publish kernel, the first edit
     is dropped as warm-up, and each kind gets launches per edit, the average
     launch duration and the time per edit; the per-launch durations of
     node_build (one per height) are listed too. Prints markdown rows."""
-    import csv
-    rows = list(csv.DictReader(open(trace_csv)))
-    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
-    rows = [(r["Kernel_Name"], (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3) for r in rows]
+    if trace_csv.endswith(".db"):  # rocprofv3's default rocpd (SQLite) output
+        import sqlite3
+        con = sqlite3.connect(trace_csv)
+        rows = [(n, (e - b) * 1e-3) for n, b, e in con.execute("select name, start, end from kernels order by start")]
+    else:
+        import csv
+        rows = list(csv.DictReader(open(trace_csv)))
+        rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+        rows = [(r["Kernel_Name"], (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3) for r in rows]
     edits, cur = [], None
     for name, us in rows:
         if "begin_kernel" in name:

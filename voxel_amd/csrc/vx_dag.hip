@@ -974,7 +974,7 @@ struct TailArgs {
 };
 
 __device__ __forceinline__ uint32_t tail_count(const TailArgs& a, uint32_t h) {
-  return min(a.cnt[1 + h], uint32_t(scan::kFusedTile));
+  return min(a.cnt[1 + h], uint32_t(kTailCap));
 }
 
 // Parents of heights h_lo..h_hi (the UpLoad / UpStore scan, one tile each).
@@ -984,7 +984,7 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_up_kernel(TailArgs a) 
   for (uint32_t h = a.h_lo; h <= a.h_hi; ++h) {
     Scenes sh = a.sc;
     sh.keys = a.hs.keys[h];
-    scan::fused_tile<scan::SumOp>(0u, tail_count(a, h), false, UpLoad{a.hs.keys[h], sh, h},
+    scan::fused_tile<scan::SumOp, 1>(0u, tail_count(a, h), false, UpLoad{a.hs.keys[h], sh, h},
                                   UpStore{a.codec, a.hs.keys[h + 1], a.hs.first[h + 1], a.hs.parent[h], &a.cnt[2 + h],
                                           a.root_pos},
                                   nullptr, s_wave, &s_prefix);
@@ -1023,7 +1023,7 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) probe_entry<kNodeStride>(set, a.hashes, res, sh, false, i);
     __syncthreads();
-    scan::fused_tile<scan::SegSumOp>(0u, n, false, CommitLoad{res, a.hashes, &a.cnt[1 + h], set, sh, false},
+    scan::fused_tile<scan::SegSumOp, 1>(0u, n, false, CommitLoad{res, a.hashes, &a.cnt[1 + h], set, sh, false},
                                      CommitStore{res, a.hashes, sh, false}, nullptr, s_wave, &s_prefix);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) place_entry<kNodeStride>(set, a.hashes, res, sh, false, i);

@@ -8,7 +8,8 @@
 //
 // Layout: a tile is kBlock threads x 16 elements. Each wave owns a contiguous
 // 1024-element segment that it reads as 4 rounds of 64 lanes x uint4, so every
-// load instruction moves 1 KiB of consecutive bytes. A tile publishes its
+// load instruction moves 1 KiB of consecutive bytes; loads and stores are
+// streaming (nontemporal). A tile publishes its
 // aggregate, then wave 0 inspects up to 64 predecessor records per step
 // (ballot over the 64 lanes) instead of walking back one tile at a time.
 #include "vx_scan.hpp"
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void scan_vec_kernel(uint32_t* __restrict__
   for (int k = 0; k < K; ++k) {
     const uint64_t i = seg + k * 256u + lane * 4u;
     if (i + 3 < n) {
-      v[k] = *reinterpret_cast<const uint4*>(data + i);
+      v[k] = ld_stream(data + i);
     } else {
       v[k].x = i < n ? data[i] : 0u;
       v[k].y = i + 1 < n ? data[i + 1] : 0u;
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void scan_vec_kernel(uint32_t* __restrict__
       o.w = a;
     }
     if (i + 3 < n) {
-      *reinterpret_cast<uint4*>(data + i) = o;
+      st_stream(data + i, o);
     } else {
       if (i < n) data[i] = o.x;
       if (i + 1 < n) data[i + 1] = o.y;

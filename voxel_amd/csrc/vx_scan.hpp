@@ -142,22 +142,24 @@ __device__ typename M::T lookback_window(const uint64_t* rec, uint32_t tile) {
   return excl;
 }
 
-// One tile: kFusedBlock threads x kFusedItems elements, wave-segment order
-// (lane-contiguous per round, so every round of loads is coalesced).
-template <class M, class Load, class Store>
+// One tile: kFusedBlock threads x ITEMS elements, wave-segment order
+// (lane-contiguous per round, so every round of loads is coalesced). Grid
+// kernels use kFusedItems; a single-workgroup pass over at most kFusedBlock
+// elements uses one item per thread (a quarter of the registers).
+template <class M, uint32_t ITEMS = kFusedItems, class Load, class Store>
 __device__ __forceinline__ void fused_tile(uint32_t tile, uint32_t n, bool multi, const Load& ld, const Store& st,
                                            uint64_t* rec, typename M::T* s_wave, typename M::T* s_prefix) {
   using T = typename M::T;
   constexpr uint32_t kWaves = kFusedBlock / kWave;
   using P = typename Load::Payload;
   const uint32_t lane = lane_id(), wave = wave_id();
-  const uint32_t seg = tile * kFusedTile + wave * (kWave * kFusedItems);
-  T v[kFusedItems], ex[kFusedItems];
-  P pl[kFusedItems];
+  const uint32_t seg = tile * (kFusedBlock * ITEMS) + wave * (kWave * ITEMS);
+  T v[ITEMS], ex[ITEMS];
+  P pl[ITEMS];
   T carry = M::identity();
   Load lc = ld;  // per-thread copy: a Load may keep state from call to call
 #pragma unroll
-  for (uint32_t k = 0; k < kFusedItems; ++k) {
+  for (uint32_t k = 0; k < ITEMS; ++k) {
     const uint32_t i = seg + k * kWave + lane;
     v[k] = i < n ? T(lc(i, pl[k])) : M::identity();
     const T inc = wave_inclusive<M>(v[k]);
@@ -191,7 +193,7 @@ __device__ __forceinline__ void fused_tile(uint32_t tile, uint32_t n, bool multi
   __syncthreads();
   const T prefix = M::combine(*s_prefix, before);
 #pragma unroll
-  for (uint32_t k = 0; k < kFusedItems; ++k) {
+  for (uint32_t k = 0; k < ITEMS; ++k) {
     const uint32_t i = seg + k * kWave + lane;
     if (i < n) {
       const T e = M::combine(prefix, ex[k]);
@@ -227,6 +229,9 @@ __global__ __launch_bounds__(kFusedBlock) void fused_dev_kernel(const uint32_t* 
     if (tiles == 1 && blockIdx.x == 0) fused_tile<M>(0u, n, false, ld, st, rec, s_wave, &s_prefix);
     return;
   }
+  // Blocks beyond the tile count leave at once: they would only add
+  // contended claims on the counter (the grid is sized for the upper bound).
+  if (blockIdx.x >= tiles) return;
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
