@@ -46,27 +46,75 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 template <class T>
 __device__ __forceinline__ T shfl(T v, int src) { return __shfl(v, src, kWave); }
 
+// ---- wave64 scans on DPP --------------------------------------------------
+// Cross-lane moves through DPP (data-parallel primitives: the VALU reads a
+// neighbour lane's register in the same instruction) instead of ds_bpermute
+// round trips through LDS. Controls (GFX9 encoding): row_shr:k shifts within
+// each 16-lane row, row_bcast:15 / row_bcast:31 carry lane 15 of a row (lane
+// 31 of the wave) into the next row(s), wave_shr:1 shifts the whole wave by
+// one lane. Lanes without a source read 0, the identity of every monoid
+// scanned here (sum, max, segmented sum, pair sum).
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), CTRL, ROW_MASK, 0xf, true));
+}
+template <int CTRL, int ROW_MASK = 0xf, class T>
+__device__ __forceinline__ T dpp_move(T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit lanes");
+  if constexpr (sizeof(T) == 4) {
+    return T(dpp_u32<CTRL, ROW_MASK>(uint32_t(v)));
+  } else {
+    const uint64_t u = uint64_t(v);
+    const uint32_t lo = dpp_u32<CTRL, ROW_MASK>(uint32_t(u));
+    const uint32_t hi = dpp_u32<CTRL, ROW_MASK>(uint32_t(u >> 32));
+    return T((uint64_t(hi) << 32) | lo);
+  }
+}
+
+// Inclusive scan over the 64 lanes of a wave for a monoid whose identity is
+// zero: combine(earlier, later).
+template <class T, class F>
+__device__ __forceinline__ T wave_inclusive_dpp(T v, F combine) {
+  v = combine(dpp_move<kDppRowShr1>(v), v);
+  v = combine(dpp_move<kDppRowShr2>(v), v);
+  v = combine(dpp_move<kDppRowShr4>(v), v);
+  v = combine(dpp_move<kDppRowShr8>(v), v);
+  v = combine(dpp_move<kDppRowBcast15, 0xa>(v), v);
+  v = combine(dpp_move<kDppRowBcast31, 0xc>(v), v);
+  return v;
+}
+
+// The value of the lane below (0 in lane 0): inclusive -> exclusive.
+template <class T>
+__device__ __forceinline__ T wave_shift_up1(T v) {
+  return dpp_move<kDppWaveShr1>(v);
+}
+
+// Lane 63's value in every lane.
+template <class T>
+__device__ __forceinline__ T wave_last(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return T(__builtin_amdgcn_readlane(int(v), 63));
+  } else {
+    const uint64_t u = uint64_t(v);
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), 63));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u >> 32)), 63));
+    return T((uint64_t(hi) << 32) | lo);
+  }
+}
+
 // Inclusive wave scan (sum) over 64 lanes.
 template <class T>
 __device__ __forceinline__ T wave_inclusive_sum(T v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const T o = __shfl_up(v, d, kWave);
-    if (lane >= uint32_t(d)) v += o;
-  }
-  return v;
+  return wave_inclusive_dpp(v, [](T a, T b) { return T(a + b); });
 }
 
 template <class T>
 __device__ __forceinline__ T wave_inclusive_max(T v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const T o = __shfl_up(v, d, kWave);
-    if (lane >= uint32_t(d)) v = o > v ? o : v;
-  }
-  return v;
+  return wave_inclusive_dpp(v, [](T a, T b) { return a > b ? a : b; });
 }
 
 // Streaming (nontemporal) 16-byte loads and stores for data touched once:

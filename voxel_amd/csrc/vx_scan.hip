@@ -64,10 +64,8 @@ struct TileScan {
     for (int k = 0; k < K; ++k) {
       const uint32_t s = combine<OP>(combine<OP>(v[k].x, v[k].y), combine<OP>(v[k].z, v[k].w));
       const uint32_t inc = wave_incl<OP>(s);
-      uint32_t ex = __shfl_up(inc, 1, kWave);
-      if (lane == 0) ex = 0;
-      excl[k] = combine<OP>(carry, ex);
-      carry = combine<OP>(carry, shfl(inc, 63));
+      excl[k] = combine<OP>(carry, wave_shift_up1(inc));
+      carry = combine<OP>(carry, wave_last(inc));
     }
     if (lane == 0) s_wave[wave] = carry;
     __syncthreads();

@@ -101,13 +101,7 @@ inline size_t fused_dev_workspace_bytes(uint64_t n_upper) {
 
 template <class M>
 __device__ __forceinline__ typename M::T wave_inclusive(typename M::T v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const typename M::T o = __shfl_up(v, d, kWave);
-    if (lane >= uint32_t(d)) v = M::combine(o, v);
-  }
-  return v;
+  return wave_inclusive_dpp(v, [](typename M::T a, typename M::T b) { return M::combine(a, b); });
 }
 
 // Decoupled look-back executed by one full wave: lane i reads the record of
@@ -134,7 +128,7 @@ __device__ typename M::T lookback_window(const uint64_t* rec, uint32_t tile) {
     // Reverse lane order (oldest first), then an inclusive scan: lane 63 holds
     // rec[base-63] (+) ... (+) rec[base].
     const T rev = __shfl(mine, int(63 - lane), kWave);
-    const T win = shfl(wave_inclusive<M>(rev), 63);
+    const T win = wave_last(wave_inclusive<M>(rev));
     excl = M::combine(win, excl);
     if (first < 64) break;
     base -= 64;
@@ -163,10 +157,8 @@ __device__ __forceinline__ void fused_tile(uint32_t tile, uint32_t n, bool multi
     const uint32_t i = seg + k * kWave + lane;
     v[k] = i < n ? T(lc(i, pl[k])) : M::identity();
     const T inc = wave_inclusive<M>(v[k]);
-    T e = __shfl_up(inc, 1, kWave);
-    if (lane == 0) e = M::identity();
-    ex[k] = M::combine(carry, e);
-    carry = M::combine(carry, shfl(inc, 63));
+    ex[k] = M::combine(carry, wave_shift_up1(inc));
+    carry = M::combine(carry, wave_last(inc));
   }
   if (lane == 0) s_wave[wave] = carry;
   __syncthreads();
