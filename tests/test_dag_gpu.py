@@ -116,6 +116,27 @@ def test_brushes_on_empty_scene_match_oracle(gpu, materials, depth, n, mix, seed
 
 
 @pytest.mark.parametrize("materials", [False, True])
+def test_sphere_radii_across_arithmetic_paths_match_oracle(gpu, materials):
+    """Sphere masks switch from packed 16-bit to 32-bit arithmetic above radius
+    100: radii 95..106 and 126 at unaligned centres (some clipped by the world
+    boundary), mixed add / remove, on a 512^3 world."""
+    depth = 7
+    rng = np.random.default_rng(7)
+    radii = list(range(95, 107)) + [126, 1, 2, 3]
+    br = np.zeros(len(radii), O.BRUSH_DTYPE)
+    for i, r in enumerate(radii):
+        br[i]["cx"], br[i]["cy"], br[i]["cz"] = rng.integers(-20, 532, 3)
+        br[i]["hx"] = r
+        br[i]["kind"] = 0
+        br[i]["material"] = 0 if i % 4 == 3 else int(rng.integers(1, 256))
+    d = _dag(depth, materials)
+    d.apply_brushes(br)
+    ref = O.Dag(depth, materials)
+    ref.apply_brushes(br)
+    assert np.array_equal(d.export(), ref.export())
+
+
+@pytest.mark.parametrize("materials", [False, True])
 def test_successive_batches_match_oracle(gpu, materials):
     depth = 7
     d = _dag(depth, materials)
