@@ -40,6 +40,20 @@ KERNEL_SYMBOL = {
     "node_build": "node_build_kernel",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_edit_16384.json")
+VENDOR_SORT_BIN = os.path.join(ROOT, "voxel_amd", "_build", "bench_vendor_sort")
+
+
+def vendor_sort():
+    """rocPRIM's radix_sort_pairs on the sort rows' shapes (a comparison, run
+    as a child process of the compiled bench_vendor_sort.hip)."""
+    import subprocess
+    if not os.path.exists(VENDOR_SORT_BIN):
+        return {"error": "not built (python bench.py --build-vendor-sort)"}
+    r = subprocess.run([VENDOR_SORT_BIN], capture_output=True, text=True, timeout=300)
+    try:
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return {"error": f"exit {r.returncode}: {r.stderr[-300:]}"}
 
 
 def traffic_for(kernel):
@@ -80,6 +94,10 @@ def parse():
     ap.add_argument("--kernel-trace", metavar="TRACE",
                     help="summarise a rocprofv3 --kernel-trace output (CSV or rocpd .db) of this benchmark per edit "
                          "kernel kind and exit")
+    ap.add_argument("--build-vendor-sort", action="store_true",
+                    help="compile bench_vendor_sort.hip (rocPRIM radix sort on the sort rows' shapes) and exit")
+    ap.add_argument("--vendor-sort", action="store_true",
+                    help="run the compiled rocPRIM comparison and add its figures to the JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="finish with one 1 GiB read + 1 GiB write elementwise kernel (bitwise_not on int32), "
                          "the known-size reference for hardware byte counters")
@@ -590,6 +608,13 @@ def main():
     if args.kernel_trace:
         kernel_trace_summary(args.kernel_trace)
         return
+    if args.build_vendor_sort:
+        import subprocess
+        os.makedirs(os.path.dirname(VENDOR_SORT_BIN), exist_ok=True)
+        subprocess.check_call(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", "-Wno-unused-value",
+                               "-Wno-unused-result",
+                               os.path.join(ROOT, "bench_vendor_sort.hip"), "-o", VENDOR_SORT_BIN])
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: every box here has one GPU, so --gpus {args.gpus} is not run "
               "(the multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
@@ -683,6 +708,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_primitives:
         prims.update(hash_tables(torch, P))
+    if args.vendor_sort:
+        prims["vendor"] = vendor_sort()
     torch.cuda.empty_cache()
     batched = None if args.no_batched else leg_batched(V, torch)
     torch.cuda.empty_cache()
