@@ -96,8 +96,8 @@ def parse():
                          "kernel kind and exit")
     ap.add_argument("--build-vendor-sort", action="store_true",
                     help="compile bench_vendor_sort.hip (rocPRIM radix sort on the sort rows' shapes) and exit")
-    ap.add_argument("--vendor-sort", action="store_true",
-                    help="run the compiled rocPRIM comparison and add its figures to the JSON line")
+    ap.add_argument("--no-vendor-sort", action="store_true",
+                    help="skip the rocPRIM comparison (run with the primitives when it has been built)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="finish with one 1 GiB read + 1 GiB write elementwise kernel (bitwise_not on int32), "
                          "the known-size reference for hardware byte counters")
@@ -708,7 +708,7 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_primitives:
         prims.update(hash_tables(torch, P))
-    if args.vendor_sort:
+    if not args.no_primitives and not args.no_vendor_sort and os.path.exists(VENDOR_SORT_BIN):
         prims["vendor"] = vendor_sort()
     torch.cuda.empty_cache()
     batched = None if args.no_batched else leg_batched(V, torch)

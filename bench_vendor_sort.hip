@@ -2,8 +2,9 @@
 // radix_sort_pairs on the same shapes (2^28 u32 keys + u32 index payload,
 // 2^27 u64 keys of 42 bits + u32 payload), hipEvent median of 5 after one
 // warm-up. Keys are uniform random (std::mt19937_64, seed 2); the output is
-// checked sorted. Built by `python bench.py --build-vendor-sort`, run by
-// `python bench.py --vendor-sort`; it prints one JSON object.
+// checked sorted. Built by `python bench.py --build-vendor-sort` (the package
+// build runs it too), run by `python bench.py` with the primitives; it prints
+// one JSON object.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
