@@ -124,6 +124,7 @@ def ref():
             "vxr_create": (P, [u32]),
             "vxr_destroy": (None, [P]),
             "vxr_box": (C.c_int, [P, u32, u32, u32, u32, u32, u32, C.c_int]),
+            "vxr_boxes": (C.c_int, [P, P, u64]),
             "vxr_set_voxel": (C.c_int, [P, u32, u32, u32, C.c_int]),
             "vxr_get_voxel": (C.c_int, [P, u32, u32, u32]),
             "vxr_node_count": (u64, [P]),
@@ -292,6 +293,45 @@ def dense_export(grid: np.ndarray, inner_depth: int, materials: bool) -> np.ndar
     return out
 
 
+def terrain_column_boxes(terrain, world: int) -> np.ndarray:
+    """The heightfield as one solid column box per (x, z): (x, 0, z) to
+    (x + 1, min(world, h + 1), z + 1), z-major (the reference has no terrain
+    builder; this is how its HashDAG is fed the config-0 scene)."""
+    ops = np.empty((world * world, 7), np.uint32)
+    k = 0
+    for z in range(world):
+        for x in range(world):
+            h = terrain_height(terrain, x, z)
+            ops[k] = (x, 0, z, x + 1, min(world, h + 1), z + 1, 1)
+            k += 1
+    return ops
+
+
+def sphere_row_boxes(brushes, world: int) -> np.ndarray:
+    """Sphere brushes as x-row boxes in brush order (fill = add), clipped to
+    the world: for each (y, z) row, x in [cx - s, cx + s] with
+    s = isqrt(r^2 - dy^2 - dz^2)."""
+    import math
+    ops = []
+    for b in np.asarray(brushes):
+        r, cx, cy, cz = int(b["hx"]), int(b["cx"]), int(b["cy"]), int(b["cz"])
+        fill = 1 if int(b["material"]) else 0
+        for dz in range(-r, r + 1):
+            z = cz + dz
+            if not 0 <= z < world:
+                continue
+            for dy in range(-r, r + 1):
+                y = cy + dy
+                rem = r * r - dy * dy - dz * dz
+                if rem < 0 or not 0 <= y < world:
+                    continue
+                s = math.isqrt(rem)
+                x0, x1 = max(0, cx - s), min(world, cx + s + 1)
+                if x0 < x1:
+                    ops.append((x0, y, z, x1, y + 1, z + 1, fill))
+    return np.array(ops, np.uint32).reshape(-1, 7)
+
+
 class RefDag:
     """The reference's own HashDAG (occupancy only), from oracle/_ref."""
 
@@ -309,6 +349,12 @@ class RefDag:
     def box(self, lo, hi, fill: bool):
         rc = ref().vxr_box(self.h, *lo, *hi, int(fill))
         if rc:
+            raise RuntimeError("reference box edit failed")
+
+    def boxes(self, ops: np.ndarray):
+        """(n, 7) u32 half-open boxes (x0 y0 z0 x1 y1 z1 fill), applied in order."""
+        o = np.ascontiguousarray(ops, np.uint32).reshape(-1, 7)
+        if ref().vxr_boxes(self.h, _ptr(o), o.shape[0]):
             raise RuntimeError("reference box edit failed")
 
     def set_voxel(self, x, y, z, value: bool):

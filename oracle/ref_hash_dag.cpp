@@ -102,6 +102,21 @@ int vxr_box(void* h, uint32_t x0, uint32_t y0, uint32_t z0, uint32_t x1, uint32_
   } catch (...) { return -1; }
 }
 
+// `n` boxes of 7 words each (x0 y0 z0 x1 y1 z1 fill) in order, in one call
+// (no per-box call overhead when the reference is timed).
+int vxr_boxes(void* h, const uint32_t* ops, uint64_t n) {
+  try {
+    HashDAG* d = static_cast<HashDAG*>(h);
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint32_t* o = ops + 7 * i;
+      const HashDAG::Box b{o[0], o[1], o[2], o[3], o[4], o[5]};
+      if (o[6]) d->fill_box(b);
+      else d->clear_box(b);
+    }
+    return 0;
+  } catch (...) { return -1; }
+}
+
 int vxr_set_voxel(void* h, uint32_t x, uint32_t y, uint32_t z, int value) {
   try { static_cast<HashDAG*>(h)->set_voxel(x, y, z, value != 0); return 0; } catch (...) { return -1; }
 }

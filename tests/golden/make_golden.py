@@ -10,7 +10,6 @@ plus the full export for small cases). The 512^3 case applies the config-0
 heightfield as column boxes and its 64 spheres as x-row boxes, in brush order.
 """
 import json
-import math
 import os
 import sys
 
@@ -19,23 +18,6 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 from oracle import oracle as O  # noqa: E402
-
-
-def sphere_rows(b, world):
-    r = int(b["hx"])
-    for dz in range(-r, r + 1):
-        z = int(b["cz"]) + dz
-        if not 0 <= z < world:
-            continue
-        for dy in range(-r, r + 1):
-            y = int(b["cy"]) + dy
-            rem = r * r - dy * dy - dz * dz
-            if rem < 0 or not 0 <= y < world:
-                continue
-            s = math.isqrt(rem)
-            x0, x1 = max(0, int(b["cx"]) - s), min(world, int(b["cx"]) + s + 1)
-            if x0 < x1:
-                yield (x0, y, z, x1, y + 1, z + 1)
 
 
 def run_ref(depth, ops):
@@ -80,17 +62,11 @@ def main():
 
     # Config 0 (512^3, inner depth 7), occupancy: terrain columns + 64 spheres.
     depth, world = 7, 512
-    t = O.terrain_for_world(world, 1)
     d = O.RefDag(depth)
-    for z in range(world):
-        for x in range(world):
-            h = O.terrain_height(t, x, z)
-            d.box((x, 0, z), (x + 1, min(world, h + 1), z + 1), True)
+    d.boxes(O.terrain_column_boxes(O.terrain_for_world(world, 1), world))
     e_terrain = d.export()
     brushes = O.gen_brushes(1, O.STREAM_BRUSH, 64, world, 0)
-    for b in brushes:
-        for row in sphere_rows(b, world):
-            d.box(row[0:3], row[3:6], bool(b["material"]))
+    d.boxes(O.sphere_row_boxes(brushes, world))
     e = d.export()
     c0 = {"name": "config0_occupancy", "depth": depth, "seed": 1, "brushes": 64, "mix": 0,
           "terrain": {"nodes": int(e_terrain[4]), "leaves": int(e_terrain[5]),

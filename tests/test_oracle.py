@@ -164,6 +164,23 @@ def test_config0_occupancy_matches_reference_golden():
                                                              g["edited"]["digest"])
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="reference not built here")
+def test_reference_box_scripts_reproduce_config0_golden():
+    """The column / row box scripts the benchmark feeds the compiled reference
+    HashDAG (one native call per script) reproduce the committed golden."""
+    with open(os.path.join(GOLDEN, "ref_config0.json")) as f:
+        g = json.load(f)
+    world = 4 << g["depth"]
+    r = O.RefDag(g["depth"])
+    r.boxes(O.terrain_column_boxes(O.terrain_for_world(world, g["seed"]), world))
+    e = r.export()
+    assert "%016x" % O.digest(e) == g["terrain"]["digest"]
+    r.boxes(O.sphere_row_boxes(O.gen_brushes(g["seed"], O.STREAM_BRUSH, g["brushes"], world, g["mix"]), world))
+    e = r.export()
+    assert (int(e[4]), int(e[5]), "%016x" % O.digest(e)) == (g["edited"]["nodes"], g["edited"]["leaves"],
+                                                             g["edited"]["digest"])
+
+
 def test_oracle_raycast_pins():
     """The oracle's ray walk against what the geometry fixes: a straight
     segment between two voxel centres crosses exactly |dx|+|dy|+|dz| voxel
