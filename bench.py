@@ -118,6 +118,7 @@ class CpuBaseline(threading.Thread):
         self.result = None
         self.digest = None
         self.error = None
+        self.primitives = None
 
     def run(self):
         try:
@@ -140,8 +141,47 @@ class CpuBaseline(threading.Thread):
             if self.parity is not None:
                 d.apply_brushes(self.parity[self.brushes:])
                 self.digest = O.digest(d.export())
+            del d
+            self.primitives = cpu_primitives(O)
         except Exception as e:  # reported, never fatal for the GPU figure
             self.error = repr(e)
+
+
+def cpu_primitives(O):
+    """The primitives on the CPU (the oracle's std::stable_sort of (key,
+    index) pairs, std::exclusive_scan, std::unordered_set), 1 core, on bounded
+    samples of the config-1 inputs: 2^24 u32 pairs, 2^23 42-bit u64 pairs,
+    2^26-element scan, 2^22 inserts + 2^22 lookups (50% hits)."""
+    import numpy as np
+    out = {}
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    n = 1 << 24
+    k = O.gen_u32(2, O.STREAM_KEYS, n, 0, 0xFFFFFFFF)
+    v = np.arange(n, dtype=np.uint32)
+    dt = timed(lambda: O.sort_pairs(k, v, 32))
+    out["sort_u32_pairs"] = {"sample": "2^24 pairs", "keys_per_s": n / dt, "cores": 1, "kind": "port"}
+    n = 1 << 23
+    k = O.gen_morton(2, O.STREAM_KEYS, n, 14)
+    v = np.arange(n, dtype=np.uint32)
+    dt = timed(lambda: O.sort_pairs(k, v, 42))
+    out["sort_u64_morton42_pairs"] = {"sample": "2^23 pairs", "keys_per_s": n / dt, "cores": 1, "kind": "port"}
+    n = 1 << 26
+    a = O.gen_u32(2, O.STREAM_SCAN, n, 0, 15)
+    dt = timed(lambda: O.exclusive_sum(a))
+    out["exclusive_scan_u32"] = {"sample": "2^26 elements", "elements_per_s": n / dt, "gbs": 8 * n / dt / 1e9,
+                                 "cores": 1, "kind": "port"}
+    n = 1 << 22
+    keys = O.gen_unique_u64(2, 4, n)
+    q = np.concatenate([keys[: n // 2], O.gen_unique_u64(3, 4, n // 2)])
+    dt = timed(lambda: O.set_contains(keys, q))
+    out["unordered_set_u64"] = {"sample": "2^22 inserts + 2^22 lookups", "ops_per_s": 2 * n / dt, "cores": 1,
+                                "kind": "port"}
+    return out
 
 
 class RefConfig0(threading.Thread):
@@ -829,6 +869,8 @@ def main():
     if cpu is not None:
         cpu.join()
         cpu_res = cpu.result if cpu.result else {"error": cpu.error}
+        if cpu.primitives:
+            prims["cpu_baseline"] = cpu.primitives
         if check:
             parity = {"check": f"canonical DAG digest after batch 0 ({args.brushes} brushes) on the {world}^3 "
                                "scene: GPU export vs CPU oracle",

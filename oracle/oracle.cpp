@@ -608,20 +608,21 @@ void vxo_inclusive_max_u32(const uint32_t* in, uint64_t n, uint32_t* out) {
 
 }  // extern "C"
 
+// std::stable_sort of (key, value) pairs by the low `bits` key bits
+// (sort.cuh semantics: stable, ordered by the masked key).
 template <class K>
 static void sort_pairs(K* keys, uint32_t* vals, uint64_t n, uint32_t bits) {
   const K mask = bits >= sizeof(K) * 8 ? ~K(0) : ((K(1) << bits) - 1);
-  std::vector<uint64_t> idx(n);
-  std::iota(idx.begin(), idx.end(), 0);
-  std::stable_sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) { return (keys[a] & mask) < (keys[b] & mask); });
-  std::vector<K> k2(n);
-  for (uint64_t i = 0; i < n; ++i) k2[i] = keys[idx[i]];
-  std::memcpy(keys, k2.data(), n * sizeof(K));
-  if (vals) {
-    std::vector<uint32_t> v2(n);
-    for (uint64_t i = 0; i < n; ++i) v2[i] = vals[idx[i]];
-    std::memcpy(vals, v2.data(), n * 4);
-  }
+  struct Pair {
+    K key;
+    uint32_t val;
+  };
+  std::vector<Pair> p(n);
+  for (uint64_t i = 0; i < n; ++i) p[i] = Pair{keys[i], vals ? vals[i] : 0u};
+  std::stable_sort(p.begin(), p.end(), [mask](const Pair& a, const Pair& b) { return (a.key & mask) < (b.key & mask); });
+  for (uint64_t i = 0; i < n; ++i) keys[i] = p[i].key;
+  if (vals)
+    for (uint64_t i = 0; i < n; ++i) vals[i] = p[i].val;
 }
 
 extern "C" {
