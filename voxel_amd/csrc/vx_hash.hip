@@ -2,12 +2,11 @@
 // and cuckoo hash set (u64 keys) for gfx950.
 //
 // Slab tables follow algo::cuda slab_hash (slab_hash/set/device.cuh:32-316,
-// slab_hash/map/device.cuh:49-393): a bucket is a chain of slabs, one wave
-// works on one key at a time from a ballot work queue, every lane of the wave
-// reads one slot, and ballots find matches and empty slots. The slab is sized
-// to the 64-lane wavefront: 64 u64 words = 512 bytes, lanes 0..62 hold slots
-// and lane 63 holds the next-slab link (the reference's 32-lane slab holds 31
-// u32 slots + next). Status codes are the reference's
+// slab_hash/map/device.cuh:49-393): a bucket is a chain of slabs, keys come
+// from a ballot work queue, and ballots find matches and empty slots. A slab
+// is 64 u64 words = 512 bytes, words 0..62 hold slots and word 63 the
+// next-slab link (the reference's 32-lane slab holds 31 u32 slots + next);
+// a wave walks four keys at once, 16 lanes per key. Status codes are the reference's
 // (slab_hash/common.cuh:18-25). Slabs beyond the bucket heads come from a
 // bump allocator (hash_table/bump_allocator.cuh:84-121).
 //
@@ -56,27 +55,97 @@ __device__ __forceinline__ uint32_t alloc_slab(const SlabTable& t) {
 constexpr uint32_t kLinkLock = 0xfffffffeu;  // a wave is allocating the next slab
 constexpr uint32_t kLinkFull = 0xfffffffdu;  // the allocator was exhausted here
 
-// Called by the whole wave at a slab whose link is not a slab id. Lane 0 locks
-// the link, allocates, then publishes the id, so no slab is ever allocated
-// and lost to a race; a wave that finds the link locked waits for the locking
-// wave (another wave, which needs nothing from it to finish). Returns the next
-// slab or kNull when the allocator is exhausted.
-__device__ uint32_t link_next(const SlabTable& t, uint64_t* s) {
+// ---- wave groups -------------------------------------------------------------
+// A wave works on four keys at once: a group of 16 lanes per key, each lane
+// holding words gl, gl+16, gl+32, gl+48 of the key's current slab (four
+// independent loads, one 128-byte line per load instruction across the
+// group). A slab walk is a chain of dependent memory round trips, so four
+// walks in flight per wave quadruple the keys per round trip of the
+// reference's one-key-per-warp queue (slab_hash/set/device.cuh:32-316).
+constexpr uint32_t kGroupLanes = 16;
+constexpr uint32_t kGroupsPerWave = kWave / kGroupLanes;  // keys in flight per wave
+constexpr uint32_t kLaneWords = kWave / kGroupLanes;      // slab words per lane
+
+struct Group {
+  uint32_t g, gl, first;  // group of this lane, lane within the group, group's first lane
+};
+
+__device__ __forceinline__ Group my_group() {
+  const uint32_t l = lane_id();
+  return Group{l / kGroupLanes, l % kGroupLanes, (l / kGroupLanes) * kGroupLanes};
+}
+
+// Lowest slab word index (0..62) for which the group's predicate holds, or 64.
+// Word (lane gl, part q) is gl + 16 q. Executed by every lane of the wave.
+__device__ __forceinline__ uint32_t group_first(const Group& G, const bool (&p)[kLaneWords]) {
+  uint32_t best = 64;
+#pragma unroll
+  for (int q = kLaneWords - 1; q >= 0; --q) {
+    const uint32_t bits = uint32_t(ballot(p[q]) >> G.first) & ((1u << kGroupLanes) - 1u);
+    if (bits) best = kGroupLanes * uint32_t(q) + uint32_t(__builtin_ctz(bits));
+  }
+  return best;
+}
+
+// Lane src_j of the wave's j-th pending key (64 if none): the same for every
+// lane, computed from the wave-uniform queue.
+__device__ __forceinline__ uint32_t pending_lane(uint64_t queue, uint32_t j) {
+  uint32_t src = 64;
+#pragma unroll
+  for (uint32_t x = 0; x < kGroupsPerWave; ++x) {
+    const uint32_t c = queue ? uint32_t(__builtin_ctzll(queue)) : 64u;
+    if (x == j) src = c;
+    if (queue) queue &= queue - 1;
+  }
+  return src;
+}
+
+// Called by a whole group at a slab whose link is not a slab id. The group's
+// first lane locks the link, allocates and publishes the id in one step, so no
+// slab is ever allocated and lost to a race. A group that finds the link
+// locked gets kLinkLock back and re-reads the slab instead of waiting (the
+// holder may be another group of the same wave, which can only proceed once
+// this one yields). Returns a slab id, kLinkLock, or kNull when the allocator
+// is exhausted.
+__device__ uint32_t group_link(const SlabTable& t, uint64_t* s, const Group& G) {
   uint32_t link = kNull;
-  if (lane_id() == 0) {
-    uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s[63]), (unsigned long long)kEmpty64,
-                              (unsigned long long)(0xffffffff00000000ull | kLinkLock));
+  if (G.gl == 0) {
+    const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s[63]), (unsigned long long)kEmpty64,
+                                    (unsigned long long)(0xffffffff00000000ull | kLinkLock));
     if (prev == kEmpty64) {
       const uint32_t id = alloc_slab(t);
       link = id == kNull ? kLinkFull : id;
       st_agent(&s[63], id == kNull ? uint64_t(0xffffffff00000000ull | kLinkFull) : uint64_t(id));
     } else {
-      while (uint32_t(prev) == kLinkLock) prev = ld_agent(&s[63]);
       link = uint32_t(prev);
     }
   }
-  link = shfl(link, 0);
-  return link >= kLinkFull ? kNull : link;
+  link = __shfl(link, int(G.first), kWave);
+  return (link == kLinkFull || link == kNull) ? kNull : link;
+}
+
+// Bound on the rounds of one walk. A round either moves to the next slab,
+// ends the walk, or re-reads the slab because another walker changed it (won
+// a slot, linked a slab): every such retry is progress of some other key, and
+// at most 63 slots per slab can be won, so the walks of one launch end well
+// within this bound; it only guards against a corrupted chain.
+__device__ __forceinline__ uint64_t walk_guard(const SlabTable& t) { return (uint64_t(t.capacity) + 1) * 256 + 1024; }
+
+// The group's words of slab `s` (kEmpty64 for a group with nothing to do).
+__device__ __forceinline__ void group_load(const uint64_t* s, const Group& G, bool idle, uint64_t (&v)[kLaneWords]) {
+#pragma unroll
+  for (uint32_t q = 0; q < kLaneWords; ++q) v[q] = idle ? kEmpty64 : ld_agent(&s[G.gl + kGroupLanes * q]);
+}
+
+// This lane's copy of word part q = w / kGroupLanes (the owner lane of word w
+// holds word w there).
+__device__ __forceinline__ uint64_t own_word(const uint64_t (&v)[kLaneWords], uint32_t w) {
+  const uint32_t q = w / kGroupLanes;
+  uint64_t r = v[0];
+#pragma unroll
+  for (uint32_t x = 1; x < kLaneWords; ++x)
+    if (q == x) r = v[x];
+  return r;
 }
 
 // ---- set --------------------------------------------------------------------
@@ -86,6 +155,7 @@ template <SetOp OP>
 __global__ void slab_set_kernel(SlabTable t, const uint64_t* __restrict__ keys, uint32_t n, uint8_t* found,
                                 uint32_t* status) {
   const uint32_t lane = lane_id();
+  const Group G = my_group();
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
   const uint32_t wave_global = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   for (uint32_t base = wave_global * kWave; base < n; base += waves * kWave) {
@@ -99,62 +169,85 @@ __global__ void slab_set_kernel(SlabTable t, const uint64_t* __restrict__ keys, 
     }
     uint64_t queue = ballot(active);
     while (queue) {
-      const uint32_t src = uint32_t(__builtin_ctzll(queue));
-      const uint64_t k = shfl(key, int(src));
-      uint32_t slab = bucket_of(k, t.buckets);
+      const uint32_t src = pending_lane(queue, G.g);
+      bool done = src == 64;
+      const uint64_t k = shfl(key, done ? 0 : int(src));
+      uint32_t slab = done ? 0u : bucket_of(k, t.buckets);
       uint32_t result = kOverflow;
-      for (uint32_t guard = 0; guard < t.capacity * 4 + 64; ++guard) {
+      for (uint64_t guard = 0, limit = walk_guard(t); guard < limit && ballot(!done); ++guard) {
         uint64_t* s = t.slabs + size_t(slab) * kWave;
-        const uint64_t v = ld_agent(&s[lane]);
-        const uint64_t match = ballot(lane < kSlotLanes && v == k);
-        if (match) {
+        uint64_t v[kLaneWords];
+        group_load(s, G, done, v);
+        bool pm[kLaneWords], pe[kLaneWords];
+#pragma unroll
+        for (uint32_t q = 0; q < kLaneWords; ++q) {
+          const bool slot = !done && G.gl + kGroupLanes * q < kSlotLanes;
+          pm[q] = slot && v[q] == k;
+          pe[q] = slot && v[q] == kEmpty64;
+        }
+        const uint32_t m = group_first(G, pm);
+        const uint32_t e = OP == SetOp::kInsert ? group_first(G, pe) : 64u;
+        const uint32_t link = uint32_t(__shfl(v[kLaneWords - 1], int(G.first + kGroupLanes - 1), kWave));
+        if (done) continue;
+        if (m < 64) {
           if constexpr (OP == SetOp::kInsert) {
             result = kAlreadyPresent;
+            done = true;
           } else if constexpr (OP == SetOp::kContains) {
             result = kFound;
+            done = true;
           } else {
+            const uint32_t owner = G.first + m % kGroupLanes;
             uint32_t ok = 0;
-            if (lane == uint32_t(__builtin_ctzll(match)))
-              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[lane]), (unsigned long long)k,
+            if (lane == owner)
+              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[m]), (unsigned long long)k,
                              (unsigned long long)kTomb64) == k;
-            if (!shfl(ok, int(__builtin_ctzll(match)))) continue;  // raced: re-read this slab
-            result = kErased;
+            if (__shfl(ok, int(owner), kWave)) {
+              result = kErased;
+              done = true;
+            }  // else raced: re-read this slab
           }
-          break;
+          continue;
         }
         if constexpr (OP == SetOp::kInsert) {
-          const uint64_t empty = ballot(lane < kSlotLanes && v == kEmpty64);
-          if (empty) {
-            const uint32_t dst = uint32_t(__builtin_ctzll(empty));
+          if (e < 64) {
+            const uint32_t owner = G.first + e % kGroupLanes;
             uint32_t ok = 0;
-            if (lane == dst)
-              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[lane]), (unsigned long long)kEmpty64,
+            if (lane == owner)
+              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[e]), (unsigned long long)kEmpty64,
                              (unsigned long long)k) == kEmpty64;
-            if (shfl(ok, int(dst))) {
+            if (__shfl(ok, int(owner), kWave)) {
               result = kInserted;
-              break;
-            }
-            continue;  // lost the slot: re-read (the winner may hold the same key)
+              done = true;
+            }  // else lost the slot: re-read (the winner may hold the same key)
+            continue;
           }
         }
-        uint32_t next = uint32_t(shfl(v, 63));
+        uint32_t next = link;
         if (next >= kLinkFull) {  // null, being linked, or exhausted
           if constexpr (OP != SetOp::kInsert) {
             result = kNotFound;
-            break;
+            done = true;
+            continue;
           } else {
-            next = next == kLinkFull ? kNull : link_next(t, s);
+            next = next == kLinkFull ? kNull : group_link(t, s, G);
             if (next == kNull) {
               result = kOverflow;
-              break;
+              done = true;
+              continue;
             }
+            if (next == kLinkLock) continue;  // being linked elsewhere: re-read
           }
         }
         slab = next;
       }
-      if (lane == src) {
-        st = result;
-        active = false;
+#pragma unroll
+      for (uint32_t j = 0; j < kGroupsPerWave; ++j) {
+        const uint32_t r = __shfl(result, int(j * kGroupLanes), kWave);
+        if (lane == pending_lane(queue, j)) {
+          st = r;
+          active = false;
+        }
       }
       queue = ballot(active);
     }
@@ -175,6 +268,7 @@ template <MapOp OP>
 __global__ void slab_map_kernel(SlabTable t, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals_in,
                                 uint32_t* vals_out, uint32_t n, uint32_t* status) {
   const uint32_t lane = lane_id();
+  const Group G = my_group();
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
   const uint32_t wave_global = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   constexpr uint32_t kEmptyKey = 0xffffffffu, kTombKey = 0xfffffffeu;
@@ -190,67 +284,89 @@ __global__ void slab_map_kernel(SlabTable t, const uint32_t* __restrict__ keys, 
     }
     uint64_t queue = ballot(active);
     while (queue) {
-      const uint32_t src = uint32_t(__builtin_ctzll(queue));
-      const uint32_t k = shfl(key, int(src));
-      const uint32_t kv = shfl(val, int(src));
+      const uint32_t src = pending_lane(queue, G.g);
+      bool done = src == 64;
+      const uint32_t k = shfl(key, done ? 0 : int(src));
+      const uint32_t kv = shfl(val, done ? 0 : int(src));
       const uint64_t packed = (uint64_t(k) << 32) | kv;
-      uint32_t slab = bucket_of(k, t.buckets);
+      uint32_t slab = done ? 0u : bucket_of(k, t.buckets);
       uint32_t result = kOverflow, found_val = 0xffffffffu;
-      for (uint32_t guard = 0; guard < t.capacity * 4 + 64; ++guard) {
+      for (uint64_t guard = 0, limit = walk_guard(t); guard < limit && ballot(!done); ++guard) {
         uint64_t* s = t.slabs + size_t(slab) * kWave;
-        const uint64_t v = ld_agent(&s[lane]);
-        const uint64_t match = ballot(lane < kSlotLanes && slot_key(v) == k);
-        if (match) {
-          const uint32_t at = uint32_t(__builtin_ctzll(match));
+        uint64_t v[kLaneWords];
+        group_load(s, G, done, v);
+        bool pm[kLaneWords], pe[kLaneWords];
+#pragma unroll
+        for (uint32_t q = 0; q < kLaneWords; ++q) {
+          const bool slot = !done && G.gl + kGroupLanes * q < kSlotLanes;
+          pm[q] = slot && slot_key(v[q]) == k;
+          pe[q] = slot && v[q] == kEmpty64;
+        }
+        const uint32_t m = group_first(G, pm);
+        const uint32_t e = OP == MapOp::kInsertOrReplace ? group_first(G, pe) : 64u;
+        const uint32_t link = uint32_t(__shfl(v[kLaneWords - 1], int(G.first + kGroupLanes - 1), kWave));
+        if (done) continue;
+        if (m < 64) {
+          const uint32_t owner = G.first + m % kGroupLanes;
+          const uint64_t mine = own_word(v, m);
           if constexpr (OP == MapOp::kFind) {
-            found_val = uint32_t(shfl(v, int(at)));
+            found_val = uint32_t(__shfl(mine, int(owner), kWave));
             result = kFound;
-            break;
+            done = true;
           } else {
             const uint64_t repl = OP == MapOp::kErase ? ((uint64_t(kTombKey) << 32) | 0xffffffffu) : packed;
             uint32_t ok = 0;
-            if (lane == at)
-              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[lane]), (unsigned long long)v,
-                             (unsigned long long)repl) == v;
-            if (!shfl(ok, int(at))) continue;
-            result = OP == MapOp::kErase ? kErased : kReplaced;
-            break;
+            if (lane == owner)
+              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[m]), (unsigned long long)mine,
+                             (unsigned long long)repl) == mine;
+            if (__shfl(ok, int(owner), kWave)) {
+              result = OP == MapOp::kErase ? kErased : kReplaced;
+              done = true;
+            }
           }
+          continue;
         }
         if constexpr (OP == MapOp::kInsertOrReplace) {
-          const uint64_t empty = ballot(lane < kSlotLanes && v == kEmpty64);
-          if (empty) {
-            const uint32_t dst = uint32_t(__builtin_ctzll(empty));
+          if (e < 64) {
+            const uint32_t owner = G.first + e % kGroupLanes;
             uint32_t ok = 0;
-            if (lane == dst)
-              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[lane]), (unsigned long long)kEmpty64,
+            if (lane == owner)
+              ok = atomicCAS(reinterpret_cast<unsigned long long*>(&s[e]), (unsigned long long)kEmpty64,
                              (unsigned long long)packed) == kEmpty64;
-            if (shfl(ok, int(dst))) {
+            if (__shfl(ok, int(owner), kWave)) {
               result = kInserted;
-              break;
+              done = true;
             }
             continue;
           }
         }
-        uint32_t next = uint32_t(shfl(v, 63));
+        uint32_t next = link;
         if (next >= kLinkFull) {  // null, being linked, or exhausted
           if constexpr (OP != MapOp::kInsertOrReplace) {
             result = kNotFound;
-            break;
+            done = true;
+            continue;
           } else {
-            next = next == kLinkFull ? kNull : link_next(t, s);
+            next = next == kLinkFull ? kNull : group_link(t, s, G);
             if (next == kNull) {
               result = kOverflow;
-              break;
+              done = true;
+              continue;
             }
+            if (next == kLinkLock) continue;
           }
         }
         slab = next;
       }
-      if (lane == src) {
-        st = result;
-        outv = found_val;
-        active = false;
+#pragma unroll
+      for (uint32_t j = 0; j < kGroupsPerWave; ++j) {
+        const uint32_t r = __shfl(result, int(j * kGroupLanes), kWave);
+        const uint32_t fv = __shfl(found_val, int(j * kGroupLanes), kWave);
+        if (lane == pending_lane(queue, j)) {
+          st = r;
+          outv = fv;
+          active = false;
+        }
       }
       queue = ballot(active);
     }
