@@ -3,7 +3,9 @@
 // The voxel order is fixed by exact integer arithmetic (include/vx.h, vx_ray):
 // crossing j of axis a at t = (2j + 1) / (2|d_a|), ties x < y < z. The state
 // is the number of crossings k_a taken on each axis. Each iteration descends
-// the DAG to the current voxel; an EMPTY ref at level l means the whole
+// the DAG to the current voxel, restarting below the deepest node it shares
+// with the previous voxel (the path is kept in LDS); an EMPTY ref at level l
+// means the whole
 // aligned cube of side world >> l is empty, and the ray leaves it in one
 // step: the first crossing that exits the cube is found per axis, and the
 // crossings of the other axes before it are counted in closed form — the
@@ -58,7 +60,16 @@ __device__ __forceinline__ bool exit_cube(Ray& r, const int64_t (&lo)[3], int64_
   return true;
 }
 
-__global__ void raycast_kernel(Dev d, const vx_ray* __restrict__ rays, uint32_t n, vx_hit* __restrict__ hits) {
+// Refs of the current descent path, one column per thread (level-major, so
+// the threads of a wave hit distinct banks): path[l] is the node at level l
+// that holds the last voxel visited.
+constexpr uint32_t kRayBlock = 256;
+constexpr uint32_t kMaxLevels = 16;
+
+__global__ __launch_bounds__(kRayBlock) void raycast_kernel(Dev d, const vx_ray* __restrict__ rays, uint32_t n,
+                                                            vx_hit* __restrict__ hits) {
+  __shared__ uint32_t s_path[kMaxLevels][kRayBlock];
+  uint32_t* path = &s_path[0][threadIdx.x];  // path[l * kRayBlock]
   const int64_t world = int64_t(4) << d.depth;
   const uint32_t log_world = d.depth + 2;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -75,7 +86,9 @@ __global__ void raycast_kernel(Dev d, const vx_ray* __restrict__ rays, uint32_t 
       r.k[a] = 0;
     }
     vx_hit h{0u, 0u, 0, 0, 0, 0u};
-    const uint32_t root = d.ctl[kCtlRoot];
+    path[0] = d.ctl[kCtlRoot];
+    uint32_t plen = 0;                 // path[0..plen] hold the last descent
+    uint32_t px = 0, py = 0, pz = 0;   // the last voxel visited
     for (;;) {
       int64_t v[3];
       bool inside = true;
@@ -87,11 +100,25 @@ __global__ void raycast_kernel(Dev d, const vx_ray* __restrict__ rays, uint32_t 
       const uint64_t steps = r.k[0] + r.k[1] + r.k[2];
       if (!inside || steps > q.max_steps) break;
       const uint32_t x = uint32_t(v[0]), y = uint32_t(v[1]), z = uint32_t(v[2]);
-      uint32_t ref = root, lvl = 0;
+      // The new voxel shares the nodes of levels below the first differing
+      // coordinate bit with the last one: the descent restarts there, not at
+      // the root (the same nodes a descent from the root would read).
+      uint32_t lvl = 0;
+      const uint32_t diff = (x ^ px) | (y ^ py) | (z ^ pz);
+      if (plen > 0 && diff != 0) {
+        const uint32_t common = log_world - 1 - (31u - uint32_t(__builtin_clz(diff)));
+        lvl = common < plen ? common : plen;
+      }
+      uint32_t ref = path[lvl * kRayBlock];
       for (; lvl < d.depth && ref > kFull; ++lvl) {
         const uint32_t sh = log_world - 1 - lvl;
         ref = child_of(d.node_pool, ref, ((x >> sh) & 1u) | (((y >> sh) & 1u) << 1) | (((z >> sh) & 1u) << 2));
+        path[(lvl + 1) * kRayBlock] = ref;
       }
+      plen = lvl;
+      px = x;
+      py = y;
+      pz = z;
       uint32_t m = 0;
       int64_t side = 1;
       if (ref == kFull) {
@@ -119,6 +146,7 @@ __global__ void raycast_kernel(Dev d, const vx_ray* __restrict__ rays, uint32_t 
 extern "C" vx_status vx_dag_raycast(const vx_dag* g, const vx_ray* rays, uint32_t n, vx_hit* hits, void* stream) {
   if (!g || ((!rays || !hits) && n)) return VX_INVALID_VALUE;
   if (n == 0) return VX_OK;
-  raycast_kernel<<<grid_for(n), 256, 0, static_cast<hipStream_t>(stream)>>>(g->dev, rays, n, hits);
+  if (g->depth + 1 > kMaxLevels) return VX_INVALID_VALUE;
+  raycast_kernel<<<grid_for(n), kRayBlock, 0, static_cast<hipStream_t>(stream)>>>(g->dev, rays, n, hits);
   return last_launch();
 }
