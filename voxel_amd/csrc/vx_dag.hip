@@ -320,8 +320,14 @@ vx_status intern_width(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet
     TimedLaunch tl(leaves ? VX_K_LEAF_COMMIT : VX_K_NODE_COMMIT, s);
     const CommitLoad ld{res, hashes, n_ptr, set, sc, leaves};
     const CommitStore sto{res, hashes, sc, leaves};
-    st = n_scenes == 1 ? scan::fused_dev_m<scan::SumOp>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws)
-                       : scan::fused_dev_m<scan::SegSumOp>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws);
+    // Leaves: one pass over millions of entries; node heights: small counts.
+    constexpr uint32_t kBig = scan::kFusedItems, kSmall = scan::kFusedItemsSmall;
+    if (n_scenes == 1)
+      st = leaves ? scan::fused_dev_m<scan::SumOp, kBig>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws)
+                  : scan::fused_dev_m<scan::SumOp, kSmall>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws);
+    else
+      st = leaves ? scan::fused_dev_m<scan::SegSumOp, kBig>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws)
+                  : scan::fused_dev_m<scan::SegSumOp, kSmall>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws);
   }
   if (st != VX_OK) return st;
   {
@@ -1200,7 +1206,7 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
     sh.keys = L.keys[h];
     {
       TimedLaunch tl(VX_K_UP, s);
-      st = scan::fused_dev<scan::kSum>(&cnt[1 + h], L.cap[h], UpLoad{L.keys[h], sh, h},
+      st = scan::fused_dev<scan::kSum, scan::kFusedItemsSmall>(&cnt[1 + h], L.cap[h], UpLoad{L.keys[h], sh, h},
                                        UpStore{p.codec, L.keys[h + 1], L.first[h + 1], L.parent[h], &cnt[2 + h],
                                                w.root_pos},
                                        w.up_ws[h], s, false);
