@@ -557,9 +557,10 @@ def kernel_bytes(prof, leaf_words):
         "sort_partition": 2 * prof.sort_record_bytes * R * P,  # per pass: each record read and written
         "runs": 8 * R + 16 * L,                         # sorted keys read; leaf keys, run starts, parent links
         "up": 24 * N,                                   # heights >= 1: keys read + parent links; parent keys, starts
-        "topdown": 48 * (L + N),                        # parent index, parent ref, parent item, own ref
-        "leaf_build": L * (28 + lw) + (prof.sort_record_bytes + 12) * R,  # old ref, range, item, hash, result;
-                                                        # per record: its sorted entry, mask and material
+        "topdown": 48 * N,                              # parent index, parent ref, parent item, own ref
+        "leaf_build": L * (40 + lw) + (prof.sort_record_bytes + 12) * R,  # parent index, parent ref, parent
+                                                        # item word, range, item, hash, result; per record: its
+                                                        # sorted entry, mask and material
         "leaf_probe": 16 * L,                           # result read and written (items probed: not counted)
         "leaf_commit": 16 * L,                          # result read, slot value written (representatives only)
         "leaf_place": 8 * L,                            # result read (copies of new items: not counted)

@@ -815,15 +815,21 @@ __global__ void topdown_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t*
 }
 
 // ---- rebuild ---------------------------------------------------------------
+// The leaf's old ref is found here (its parent's old node, one child read)
+// instead of in a separate top-down pass over the leaves: this kernel is
+// bandwidth-bound with many entries in flight, so the dependent reads hide
+// behind the streaming work.
 template <uint32_t LW>
-__global__ void leaf_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ old,
-                                  const uint32_t* __restrict__ first, SortedRecs recs,
-                                  const uint64_t* __restrict__ rmask, const uint32_t* __restrict__ rmat,
-                                  uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
-                                  uint64_t* __restrict__ res) {
+__global__ void leaf_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ parent,
+                                  const uint32_t* __restrict__ old_up, const uint32_t* __restrict__ first,
+                                  SortedRecs recs, const uint64_t* __restrict__ rmask,
+                                  const uint32_t* __restrict__ rmat, uint32_t* __restrict__ items,
+                                  uint64_t* __restrict__ hashes, uint64_t* __restrict__ res) {
   const uint32_t n = *n_ptr;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    const uint32_t ref = old[j];
+    const Dev& d = sc.of(j);
+    const uint32_t ref =
+        d.depth == 0 ? d.ctl[kCtlRoot] : child_of(d.node_pool, old_up[parent[j]], uint32_t(sc.keys[j] & 7u));
     uint32_t w[LW];
     if (ref == kEmpty) {
 #pragma unroll
@@ -1224,7 +1230,7 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
     tail_topdown_kernel<<<1, scan::kFusedBlock, 0, s>>>(ta);
   }
   if ((st = last_launch()) != VX_OK) return st;
-  for (uint32_t h = hu; h-- > 0;) {
+  for (uint32_t h = hu; h-- > 1;) {  // the leaves' old refs are read by the leaf build
     Scenes sh = sc;
     sh.keys = L.keys[h];
     {
@@ -1240,11 +1246,11 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
     {
       TimedLaunch tl(VX_K_LEAF_BUILD, s);
       if (p.leaf_words == 2)
-        leaf_build_kernel<2><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.old[0], L.first[0], recs, w.rmask,
-                                                                w.rmat, w.items, w.hashes, L.res[0]);
+        leaf_build_kernel<2><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.parent[0], L.old[1], L.first[0], recs,
+                                                                w.rmask, w.rmat, w.items, w.hashes, L.res[0]);
       else
-        leaf_build_kernel<16><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.old[0], L.first[0], recs, w.rmask,
-                                                                 w.rmat, w.items, w.hashes, L.res[0]);
+        leaf_build_kernel<16><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.parent[0], L.old[1], L.first[0],
+                                                                 recs, w.rmask, w.rmat, w.items, w.hashes, L.res[0]);
     }
     if ((st = last_launch()) != VX_OK) return st;
     if ((st = intern_scenes(sh, p.n_scenes, true, ItemSet{w.items, p.leaf_words, p.leaf_words}, w.hashes, &cnt[1],
