@@ -1061,30 +1061,149 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_topdown_kernel(TailArg
   }
 }
 
-// Node items of heights h_lo..h_hi: build, probe, commit, place.
+// Node items of heights h_lo..h_hi: build, probe, commit, place. Each entry
+// stays with its thread from build to place (its item, hash and result in
+// registers), the height's items sit in LDS for the tagged-slot compares, and
+// the results of one height stay in LDS as the child refs of the next: the
+// only global round trips left per height are the old node, the level links,
+// the slot probe and the stores.
+struct TailRegLoad {
+  using Payload = scan::NoPayload;
+  uint64_t v;
+  __device__ uint64_t operator()(uint32_t, scan::NoPayload&) const { return v; }
+};
+struct TailRegStore {
+  uint32_t* incl;
+  __device__ void operator()(uint32_t, const scan::NoPayload&, uint64_t, uint64_t, uint64_t in, uint32_t) const {
+    *incl = uint32_t(in);
+  }
+};
+
 __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs a) {
   __shared__ uint64_t s_wave[scan::kFusedBlock / kWave];
   __shared__ uint64_t s_prefix;
-  const ItemSet set{a.items, kNodeStride, 0};
+  __shared__ uint32_t s_item[kTailCap * kNodeStride];
+  __shared__ uint64_t s_res[2][kTailCap];
+  const uint32_t t = threadIdx.x;
   for (uint32_t h = a.h_lo; h <= a.h_hi; ++h) {
     Scenes sh = a.sc;
     sh.keys = a.hs.keys[h];
-    uint64_t* res = a.hs.res[h];
     const uint32_t n = tail_count(a, h);
-    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x)
-      node_build_entry(sh, a.hs.old[h], a.hs.first[h], a.hs.keys[h - 1], a.hs.parent[h - 1], a.hs.res[h - 1], a.items,
-                       a.hashes, res, j);
-    for (uint32_t sidx = threadIdx.x; sidx < a.n_scenes; sidx += blockDim.x) {
-      const Table t = table_of(sh.at(sidx), false);
-      *t.base = *t.used;
+    const bool act = t < n;
+    const bool lds_children = h > a.h_lo;  // the height below was built here
+    const uint64_t* cres_lds = s_res[(h - 1) & 1];
+    const uint64_t* cres_glb = a.hs.res[h - 1];
+    // Build.
+    uint32_t it[kNodeStride] = {};
+    uint32_t w = 0;
+    uint64_t r = 0, hash = 0;
+    if (act) {
+      uint32_t ch[8];
+      children_of(sh.of(t).node_pool, a.hs.old[h][t], ch);
+      const uint32_t k1 = a.hs.first[h][t + 1];
+      for (uint32_t k = a.hs.first[h][t]; k < k1; ++k) {
+        if (a.hs.parent[h - 1][k] != t) continue;
+        uint64_t cr = lds_children ? cres_lds[k] : cres_glb[k];
+        if (res_kind(cr) == kResDup) cr = lds_children ? cres_lds[uint32_t(cr)] : cres_glb[uint32_t(cr)];
+        ch[uint32_t(a.hs.keys[h - 1][k] & 7u)] = uint32_t(cr);
+      }
+      uint32_t ref = 0;
+      w = make_node_item(ch, it, &ref);
+      if (!w) {
+        r = res_make(kResFinal, ref);
+      } else {
+        hash = hash_words(it, w);
+        r = res_make(kResPending, 0);
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < kNodeStride; ++q) s_item[t * kNodeStride + q] = it[q];
+    }
+    for (uint32_t sidx = t; sidx < a.n_scenes; sidx += blockDim.x) {
+      const Table tb = table_of(sh.at(sidx), false);
+      *tb.base = *tb.used;
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) probe_entry<kNodeStride>(set, a.hashes, res, sh, false, i);
-    __syncthreads();
-    scan::fused_tile<scan::SegSumOp, 1>(0u, n, false, CommitLoad{res, a.hashes, &a.cnt[1 + h], set, sh, false},
-                                     CommitStore{res, a.hashes, sh, false}, nullptr, s_wave, &s_prefix);
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) place_entry<kNodeStride>(set, a.hashes, res, sh, false, i);
+    // Probe: an equal item in the table or earlier in this height, else claim
+    // an empty slot (the same protocol as probe_entry, with the item held in
+    // registers and this height's tagged items read from LDS).
+    if (act && res_kind(r) == kResPending) {
+      const Dev& d = sh.of(t);
+      const Table tb = table_of(d, false);
+      const uint32_t fp = uint32_t(hash >> 32);
+      uint64_t idx = hash & tb.mask;
+      const uint64_t limit = tb.mask < kMaxProbe ? tb.mask + 1 : kMaxProbe;
+      uint64_t out = res_make(kResFinal, kEmpty);
+      bool done = false;
+      for (uint64_t probe = 0; probe < limit && !done; ++probe) {
+        uint64_t sv = tb.slots[idx];
+        if (sv == 0) {
+          const uint64_t tagged = (uint64_t(fp) << 32) | kTag | t;
+          const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&tb.slots[idx]), 0ull,
+                                          static_cast<unsigned long long>(tagged));
+          if (prev == 0) {
+            out = res_make(kResRep, uint32_t(idx));
+            done = true;
+            break;
+          }
+          sv = prev;
+        }
+        if (uint32_t(sv >> 32) == fp) {
+          const uint32_t v = uint32_t(sv);
+          if (v & kTag) {
+            const uint32_t j = v & ~kTag;
+            uint32_t diff = 0;
+            for (uint32_t q = 0; q < w; ++q) diff |= it[q] ^ s_item[j * kNodeStride + q];
+            if (!diff) {
+              out = res_make(kResDup, j);
+              done = true;
+              break;
+            }
+          } else if (v >= kPoolStart) {
+            if (uint64_t(v) + w <= tb.cap && item_equal<kNodeStride>(it, tb.pool + v, w)) {
+              out = res_make(kResFinal, v);
+              done = true;
+              break;
+            }
+          }
+        }
+        idx = (idx + 1) & tb.mask;
+      }
+      if (!done) set_status(d.ctl, kStatusOverflow);
+      r = out;
+    }
+    __syncthreads();  // every claim of this height is in the table before pool offsets are handed out
+    // Commit: pool offsets of the representatives (a segmented sum of their
+    // widths per scene), then place: copy, publish the slot, become final.
+    const uint32_t wr = (act && res_kind(r) == kResRep) ? w : 0u;
+    const uint32_t id = act ? sh.id(t) : 0u;
+    const bool head = act && (t == 0 || sh.id(t - 1) != id);
+    uint32_t incl = 0;
+    scan::fused_tile<scan::SegSumOp, 1>(0u, n, false, TailRegLoad{scan::SegSumOp::make(wr, head)},
+                                        TailRegStore{&incl}, nullptr, s_wave, &s_prefix);
+    if (act) {
+      const Dev& d = sh.of(t);
+      const Table tb = table_of(d, false);
+      const uint32_t base = *tb.base;
+      if (wr) {
+        const uint64_t off = uint64_t(base) + incl - wr;
+        const uint64_t fp_hi = uint64_t(uint32_t(hash >> 32)) << 32;
+        if (off + wr > tb.cap || off >= kTag) {
+          set_status(d.ctl, kStatusOverflow);
+          st_agent(&tb.slots[uint32_t(r)], fp_hi | kTomb);
+          r = res_make(kResFinal, kEmpty);
+        } else {
+          store_item<kNodeStride>(tb.pool + off, wr, it);
+          st_agent(&tb.slots[uint32_t(r)], fp_hi | uint32_t(off));
+          r = res_make(kResFinal, uint32_t(off));
+        }
+      }
+      if (t == n - 1 || sh.id(t + 1) != id) {
+        const uint64_t used = uint64_t(base) + incl;
+        *tb.used = used > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(used);
+      }
+      s_res[h & 1][t] = r;
+      a.hs.res[h][t] = r;
+    }
     __syncthreads();
   }
 }
