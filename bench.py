@@ -83,6 +83,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-svo", action="store_true", help="skip the pointer-SVO leg")
+    ap.add_argument("--no-config0", action="store_true", help="skip the config-0 leg (GPU and reference CPU)")
     ap.add_argument("--node-slots-log2", type=int, default=27, help="log2 of the scene's node slot-table size")
     ap.add_argument("--leaf-slots-log2", type=int, default=27, help="log2 of the scene's leaf slot-table size")
     ap.add_argument("--cpu-brushes", type=int, default=1024, help="brushes in the CPU baseline sample")
@@ -574,7 +575,7 @@ def kernel_bytes(prof, leaf_words):
 PMC_COMMAND = ("cd /tmp && export TMPDIR=/tmp && cd <repo> && "
                "rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmc_{tag} -o {tag} -- "
                "python3 bench.py --warmup 1 --profile-steps 0 --steps 1 --no-cpu --no-primitives --no-batched "
-               "--no-serving --copy-calibration")
+               "--no-serving --no-svo --no-config0 --copy-calibration")
 
 
 def _kind_of(name, last_probe):
@@ -747,8 +748,9 @@ def main():
     if not args.no_cpu:
         cpu = CpuBaseline(args.inner_depth, args.seed, args.cpu_brushes, batches[0] if check else None)
         cpu.start()
-        ref0 = RefConfig0()
-        ref0.start()
+        if not args.no_config0:
+            ref0 = RefConfig0()
+            ref0.start()
 
     torch.cuda.init()
     dev = torch.device("cuda:0")
@@ -834,7 +836,7 @@ def main():
     scene.close()
     del ws
     torch.cuda.empty_cache()
-    c0, c0_export = config0_leg(V, torch)
+    c0, c0_export = (None, None) if args.no_config0 else config0_leg(V, torch)
     torch.cuda.empty_cache()
     svo = None if args.no_svo else svo_leg(torch)
     torch.cuda.empty_cache()
@@ -856,7 +858,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    if ref0 is not None:
+    if ref0 is not None and c0 is not None:
         ref0.join()
         c0["reference_cpu"] = ref0.result if ref0.result else {"error": ref0.error}
         if ref0.digest is not None:
