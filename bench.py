@@ -815,10 +815,13 @@ def main():
     scene.profile_kernels([dom])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    step_ends = []
     for k in range(t_first, t_first + args.steps):
-        scene.apply_brushes(batches[k], workspace=ws, stream=stream)
+        scene.apply_brushes(batches[k], workspace=ws, stream=stream)  # returns after its stream synchronises
+        step_ends.append(time.perf_counter())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    step_ms = [(b - a) * 1e3 for a, b in zip([t0] + step_ends[:-1], step_ends)]
     prof = scene.profile_read()
     scene.profile(False)
     timed_ops = sum(ops[t_first:])
@@ -917,6 +920,7 @@ def main():
             "traffic": traffic_for(dom),
         },
         "stages_ms": stages,
+        "step_wall_ms": [round(x, 3) for x in step_ms],
         "kernels": kernels,
         "dominance_check_ms_per_edit": confirm,
         "scene": {"build_s": build_s, "node_words": st.node_words_used, "leaf_words": st.leaf_words_used},

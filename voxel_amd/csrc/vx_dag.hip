@@ -199,6 +199,31 @@ __device__ __forceinline__ void probe_entry(const ItemSet& set, const uint64_t* 
   }
 }
 
+// Read-only lookup of an item among the items interned by earlier edits, for
+// a build step: before a level's probe the table holds no tagged slot, so a
+// walk that ends at an empty slot proves the item is not interned yet. An
+// entry found here becomes FINAL at once and its item is never written; 0
+// (walk ended at an empty or tagged slot, or after kFindProbes slots) leaves
+// the decision to the probe.
+constexpr uint32_t kFindProbes = 64;
+
+template <uint32_t kMax>
+__device__ __forceinline__ uint32_t find_interned(const Table& t, const uint32_t (&w)[kMax], uint32_t n, uint64_t h) {
+  const uint32_t fp = uint32_t(h >> 32);
+  uint64_t idx = h & t.mask;
+  for (uint32_t p = 0; p < kFindProbes; ++p) {
+    const uint64_t s = t.slots[idx];
+    if (s == 0) return 0;
+    if (uint32_t(s >> 32) == fp) {
+      const uint32_t v = uint32_t(s);
+      if (v & kTag) return 0;
+      if (v >= kPoolStart && uint64_t(v) + n <= t.cap && item_equal<kMax>(w, t.pool + v, n)) return v;
+    }
+    idx = (idx + 1) & t.mask;
+  }
+  return 0;
+}
+
 template <uint32_t kMax>
 __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, const uint32_t* n_ptr, uint64_t* res,
                              Scenes sc, uint32_t n_scenes, bool leaves) {
@@ -874,11 +899,17 @@ __global__ void leaf_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32
     } else if (LW == 2 && all == 0xffffffffu) {
       res[j] = res_make(kResFinal, kFull);
     } else {
-      uint32_t* it = items + size_t(j) * LW;
+      const uint64_t h = hash_words(w, LW);
+      const uint32_t v = find_interned<LW>(table_of(d, true), w, LW, h);
+      if (v) {
+        res[j] = res_make(kResFinal, v);
+      } else {
+        uint32_t* it = items + size_t(j) * LW;
 #pragma unroll
-      for (uint32_t k = 0; k < LW; ++k) it[k] = w[k];
-      hashes[j] = hash_words(w, LW);
-      res[j] = res_make(kResPending, 0);
+        for (uint32_t k = 0; k < LW; ++k) it[k] = w[k];
+        hashes[j] = h;
+        res[j] = res_make(kResPending, 0);
+      }
     }
   }
 }
@@ -1005,6 +1036,7 @@ struct EditWs {
   void* commit_ws[kMaxHeights + 1];
   BrushJob* jobs;
   Dev* scenes;
+  size_t scenes_off, jobs_off;  // byte offsets from counts (the workspace base)
   size_t bytes;
 };
 
@@ -1231,7 +1263,12 @@ Plan make_plan(const std::vector<const vx_dag*>& dags) {
 EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t candidates) {
   EditWs w{};
   Carver c(base);
+  // The host uploads [counts, scene table, brush jobs] as one block.
   w.counts = c.take<uint32_t>(64);
+  w.scenes_off = align_up(c.off, 256);
+  w.scenes = c.take<Dev>(p.n_scenes);
+  w.jobs_off = align_up(c.off, 256);
+  w.jobs = c.take<BrushJob>(std::max<uint32_t>(1, n_jobs));
   w.root_pos = c.take<uint32_t>(p.n_scenes);
   const uint64_t R = std::max<uint64_t>(1, r_ub);
   // Packed records when the index of the last record fits below the key
@@ -1280,8 +1317,6 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
     w.fused_all = reinterpret_cast<uint32_t*>(start);
     w.fused_bytes = size_t(end - start);
   }
-  w.jobs = c.take<BrushJob>(std::max<uint32_t>(1, n_jobs));
-  w.scenes = c.take<Dev>(p.n_scenes);
   w.bytes = c.size();
   return w;
 }
@@ -1444,14 +1479,18 @@ uint64_t make_jobs(const vx_brush* b, uint32_t n, uint32_t world, uint32_t scene
   return total - start;
 }
 
-vx_status finish(vx_dag* g, const std::vector<vx_dag*>& dags, const EditWs& w, const Plan& p, hipStream_t s) {
-  std::vector<uint32_t> status(dags.size(), 0);
-  uint32_t counts[64];
+// Reads back every scene's status (and the counts when profiling) into the
+// pinned read-back words `upload` reserved, then synchronises the stream.
+vx_status finish(vx_dag* g, const std::vector<vx_dag*>& dags, const EditWs& w, const Plan& p, HostStage& hs,
+                 hipStream_t s) {
+  uint32_t* const rb = reinterpret_cast<uint32_t*>(static_cast<char*>(hs.p) + hs.readback);
+  uint32_t* const status = rb + 64;
+  uint32_t* const counts = rb;
   for (size_t i = 0; i < dags.size(); ++i)
     if (hipMemcpyAsync(&status[i], dags[i]->ctl + kCtlStatus, 4, hipMemcpyDeviceToHost, s) != hipSuccess)
       return VX_DEVICE_ERROR;
   const bool prof = g && g->profile;
-  if (prof && hipMemcpyAsync(counts, w.counts, sizeof(counts), hipMemcpyDeviceToHost, s) != hipSuccess)
+  if (prof && hipMemcpyAsync(counts, w.counts, 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess)
     return VX_DEVICE_ERROR;
   if (hipStreamSynchronize(s) != hipSuccess) return VX_DEVICE_ERROR;
   if (prof) {
@@ -1477,29 +1516,50 @@ vx_status finish(vx_dag* g, const std::vector<vx_dag*>& dags, const EditWs& w, c
       t.used = 0;
     }
   }
-  for (uint32_t v : status)
-    if (v != kStatusOk) return VX_OUT_OF_CAPACITY;
+  for (size_t i = 0; i < dags.size(); ++i)
+    if (status[i] != kStatusOk) return VX_OUT_OF_CAPACITY;
   return VX_OK;
+}
+
+vx_status HostStage::reserve(size_t want) {
+  if (want <= bytes) return VX_OK;
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  bytes = 0;
+  const size_t size = align_up(std::max<size_t>(want, size_t(1) << 16), size_t(1) << 16);
+  if (hipHostMalloc(&p, size, hipHostMallocDefault) != hipSuccess) {
+    p = nullptr;
+    return VX_DEVICE_ERROR;
+  }
+  bytes = size;
+  return VX_OK;
+}
+
+HostStage::~HostStage() {
+  if (p) (void)hipHostFree(p);
 }
 
 // Counts words the host uploads before the device pipeline runs.
 constexpr uint32_t kCntCandidates = 62;
 constexpr uint32_t kCntJobs = 63;
 
-// Host -> device inputs of one edit: scene table, brush jobs, candidate and
-// job counts. `staging` must stay alive until the stream synchronises.
+// Host -> device inputs of one edit (scene table, brush jobs, candidate and
+// job counts) in one copy from pinned memory laid out like the workspace's
+// upload block; reserves the read-back words `finish` uses after it.
 vx_status upload(EditWs& w, const std::vector<Dev>& devs, const std::vector<BrushJob>* jobs, uint32_t n_cand,
-                 uint32_t* staging, hipStream_t s) {
-  if (hipMemcpyAsync(w.scenes, devs.data(), devs.size() * sizeof(Dev), hipMemcpyHostToDevice, s) != hipSuccess)
-    return VX_DEVICE_ERROR;
-  if (jobs && !jobs->empty() &&
-      hipMemcpyAsync(w.jobs, jobs->data(), jobs->size() * sizeof(BrushJob), hipMemcpyHostToDevice, s) != hipSuccess)
-    return VX_DEVICE_ERROR;
-  staging[0] = n_cand;
-  staging[1] = jobs ? uint32_t(jobs->size()) : 0u;
-  if (hipMemcpyAsync(&w.counts[kCntCandidates], staging, 8, hipMemcpyHostToDevice, s) != hipSuccess)
-    return VX_DEVICE_ERROR;
-  return VX_OK;
+                 HostStage& hs, hipStream_t s) {
+  const size_t n_jobs = jobs ? jobs->size() : 0;
+  const size_t bytes = w.jobs_off + n_jobs * sizeof(BrushJob);
+  hs.readback = align_up(bytes, 256);
+  if (hs.reserve(hs.readback + (devs.size() + 64) * sizeof(uint32_t)) != VX_OK) return VX_DEVICE_ERROR;
+  char* h = static_cast<char*>(hs.p);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(h);
+  std::memset(cnt, 0, 64 * sizeof(uint32_t));
+  cnt[kCntCandidates] = n_cand;
+  cnt[kCntJobs] = uint32_t(n_jobs);
+  std::memcpy(h + w.scenes_off, devs.data(), devs.size() * sizeof(Dev));
+  if (n_jobs) std::memcpy(h + w.jobs_off, jobs->data(), n_jobs * sizeof(BrushJob));
+  return from_hip(hipMemcpyAsync(w.counts, h, bytes, hipMemcpyHostToDevice, s));
 }
 
 // The record writer that matches each candidate kind.
@@ -1544,8 +1604,8 @@ vx_status run_edit(const std::vector<vx_dag*>& dags, vx_dag* prof_dag, const Pla
   vx_status st;
   std::vector<Dev> devs;
   for (vx_dag* g : dags) devs.push_back(g->dev);
-  uint32_t staging[2];
-  if ((st = upload(w, devs, jobs, n_cand, staging, s)) != VX_OK) return st;
+  HostStage& hs = dags[0]->stage;
+  if ((st = upload(w, devs, jobs, n_cand, hs, s)) != VX_OK) return st;
   const Scenes sc{p.n_scenes > 1 ? w.scenes : nullptr, nullptr, p.codec, devs[0]};
   {
     // Per-launch timing of this call's kernels while the dag is profiled.
@@ -1562,7 +1622,7 @@ vx_status run_edit(const std::vector<vx_dag*>& dags, vx_dag* prof_dag, const Pla
     } install(prof_dag && prof_dag->profile ? prof_dag->timer : nullptr);
     if ((st = device_pipeline(prof_dag, p, sc, w, n_cand, load, s)) != VX_OK) return st;
   }
-  return finish(prof_dag, dags, w, p, s);
+  return finish(prof_dag, dags, w, p, hs, s);
 }
 
 // ---- queries -------------------------------------------------------------------
@@ -1984,7 +2044,7 @@ struct vx_batcher {
   hipGraphExec_t exec = nullptr;
   std::vector<Dev> devs;
   std::vector<BrushJob> jobs;
-  uint32_t staging[2] = {0, 0};
+  HostStage stage;
 };
 
 extern "C" vx_status vx_batcher_destroy(vx_batcher* b) {
@@ -2058,7 +2118,7 @@ extern "C" vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batc
   if (cand == 0) return VX_OK;
   hipStream_t s = b->stream;
   vx_status st;
-  if ((st = upload(b->w, b->devs, &b->jobs, uint32_t(cand), b->staging, s)) != VX_OK) return st;
+  if ((st = upload(b->w, b->devs, &b->jobs, uint32_t(cand), b->stage, s)) != VX_OK) return st;
   const Scenes sc{b->plan.n_scenes > 1 ? b->w.scenes : nullptr, nullptr, b->plan.codec, b->devs[0]};
   const BrushLoad load{BrushCands{b->w.jobs, &b->w.counts[kCntJobs], b->scenes[0]->materials, b->plan.codec}};
   if (b->use_graph) {
@@ -2082,5 +2142,5 @@ extern "C" vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batc
   } else {
     if ((st = device_pipeline(nullptr, b->plan, sc, b->w, b->cap_cand, load, s)) != VX_OK) return st;
   }
-  return finish(nullptr, b->scenes, b->w, b->plan, s);
+  return finish(nullptr, b->scenes, b->w, b->plan, b->stage, s);
 }

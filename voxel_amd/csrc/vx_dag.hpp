@@ -206,6 +206,21 @@ inline uint32_t read_u32(const uint32_t* p, hipStream_t s) {
 vx_status intern(const Dev& d, bool leaves, ItemSet set, uint64_t* hashes, const uint32_t* n_ptr, uint64_t n_ub,
                  uint64_t* res, void* fused_ws, hipStream_t s);
 
+// Pinned host memory for an edit's uploads (counts, scene table, brush jobs)
+// and its status read-back, grown on demand and reused: every edit call
+// synchronises its stream before it returns, so no copy from the buffer is in
+// flight when the next call writes it.
+struct HostStage {
+  void* p = nullptr;
+  size_t bytes = 0;
+  size_t readback = 0;  // offset of the read-back words of the current call
+  vx_status reserve(size_t want);
+  HostStage() = default;
+  HostStage(const HostStage&) = delete;
+  HostStage& operator=(const HostStage&) = delete;
+  ~HostStage();
+};
+
 }  // namespace dag
 }  // namespace vx
 
@@ -226,4 +241,5 @@ struct vx_dag {
   hipEvent_t ev[VX_DAG_STAGES + 1] = {};
   vx::KernelTimer* timer = nullptr;
   vx_dag_profile prof{};
+  vx::dag::HostStage stage;
 };

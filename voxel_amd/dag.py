@@ -217,9 +217,10 @@ class HashDagGpu:
 
     def apply_brushes(self, brushes: np.ndarray, workspace=None, stream=None):
         b = np.ascontiguousarray(brushes, BRUSH_DTYPE)
-        need = self.brush_workspace_size(b)
-        if workspace is None and need:
-            workspace = torch.empty(need, dtype=torch.uint8, device="cuda")
+        if workspace is None:  # a caller's workspace is checked against the need by the library
+            need = self.brush_workspace_size(b)
+            if need:
+                workspace = torch.empty(need, dtype=torch.uint8, device="cuda")
         wsb = 0 if workspace is None else workspace.numel()
         check(lib().vx_dag_apply_brushes(self.h, b.ctypes.data_as(_P), len(b), _ptr(workspace), wsb,
                                          _stream(stream)), "vx_dag_apply_brushes")
