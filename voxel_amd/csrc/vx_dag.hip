@@ -246,28 +246,35 @@ __global__ void probe_kernel(ItemSet set, const uint64_t* __restrict__ hashes, c
 // are all in flight together), so the store half only writes.
 struct CommitPayload {
   uint64_t r;
+  uint32_t i;     // the entry
   uint32_t fp;
   uint32_t base;  // the scene's pool words in use before this level
   bool last;      // last entry of its scene's segment
 };
 
+// Position t of the scan is entry list[t] (entry t without a list); a list
+// is in entry order, so scene segments stay contiguous.
 struct CommitLoad {
   using Payload = CommitPayload;
   const uint64_t* res;
   const uint64_t* hashes;
   const uint32_t* n_ptr;
+  const uint32_t* list;
   ItemSet set;
   Scenes sc;
   bool leaves;
-  __device__ uint64_t operator()(uint32_t i, CommitPayload& p) const {
+  __device__ uint32_t entry(uint32_t t) const { return list ? list[t] : t; }
+  __device__ uint64_t operator()(uint32_t t, CommitPayload& p) const {
+    const uint32_t i = entry(t);
+    p.i = i;
     p.r = res[i];
     const bool rep = res_kind(p.r) == kResRep;
     p.fp = rep ? uint32_t(hashes[i] >> 32) : 0u;
     const uint32_t w = rep ? set.width(set.items + size_t(i) * set.stride) : 0u;
     const uint32_t id = sc.id(i);
     p.base = *table_of(sc.of(i), leaves).base;
-    p.last = i == *n_ptr - 1 || sc.id(i + 1) != id;
-    const bool head = i == 0 || sc.id(i - 1) != id;
+    p.last = t == *n_ptr - 1 || sc.id(entry(t + 1)) != id;
+    const bool head = t == 0 || sc.id(entry(t - 1)) != id;
     return scan::SegSumOp::make(w, head);
   }
 };
@@ -277,7 +284,8 @@ struct CommitStore {
   uint64_t* slot_vals;  // the hashes array, reused
   Scenes sc;
   bool leaves;
-  __device__ void operator()(uint32_t i, const CommitPayload& p, uint64_t v, uint64_t, uint64_t incl, uint32_t) const {
+  __device__ void operator()(uint32_t, const CommitPayload& p, uint64_t v, uint64_t, uint64_t incl, uint32_t) const {
+    const uint32_t i = p.i;
     const Dev& d = sc.of(i);
     const Table t = table_of(d, leaves);
     const uint32_t base = p.base;
@@ -322,30 +330,27 @@ __device__ __forceinline__ void place_entry(const ItemSet& set, const uint64_t* 
 
 template <uint32_t kMax>
 __global__ void place_kernel(ItemSet set, const uint64_t* __restrict__ slot_vals, const uint32_t* n_ptr,
-                             uint64_t* res, Scenes sc, bool leaves) {
+                             const uint32_t* __restrict__ list, uint64_t* res, Scenes sc, bool leaves) {
   const uint32_t n = *n_ptr;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    place_entry<kMax>(set, slot_vals, res, sc, leaves, i);
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    place_entry<kMax>(set, slot_vals, res, sc, leaves, list ? list[t] : t);
 }
 
+// Commit and place of the entries list[0 .. *n_ptr) (0 .. *n_ptr without a
+// list) after their probe.
 template <uint32_t kMax>
-vx_status intern_width(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet set, uint64_t* hashes,
-                       const uint32_t* n_ptr, uint64_t n_ub, uint64_t* res, void* fused_ws, hipStream_t s,
-                       bool clear_ws) {
-  {
-    TimedLaunch tl(leaves ? VX_K_LEAF_PROBE : VX_K_NODE_PROBE, s);
-    probe_kernel<kMax><<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, n_scenes, leaves);
-  }
-  vx_status st = last_launch();
-  if (st != VX_OK) return st;
+vx_status commit_place(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet set, uint64_t* hashes,
+                       const uint32_t* n_ptr, const uint32_t* list, uint64_t n_ub, uint64_t* res, void* fused_ws,
+                       hipStream_t s, bool clear_ws) {
+  vx_status st;
   {
     // One scene has one segment: a plain 32-bit sum (CommitLoad's value
     // truncated to its low word, the width) replaces the 64-bit segmented sum,
     // halving the scan's cross-lane work and registers.
     TimedLaunch tl(leaves ? VX_K_LEAF_COMMIT : VX_K_NODE_COMMIT, s);
-    const CommitLoad ld{res, hashes, n_ptr, set, sc, leaves};
+    const CommitLoad ld{res, hashes, n_ptr, list, set, sc, leaves};
     const CommitStore sto{res, hashes, sc, leaves};
-    // Leaves: one pass over millions of entries; node heights: small counts.
+    // Leaves: one pass over many entries; node heights: small counts.
     constexpr uint32_t kBig = scan::kFusedItems, kSmall = scan::kFusedItemsSmall;
     if (n_scenes == 1)
       st = leaves ? scan::fused_dev_m<scan::SumOp, kBig>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws)
@@ -357,9 +362,22 @@ vx_status intern_width(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet
   if (st != VX_OK) return st;
   {
     TimedLaunch tl(leaves ? VX_K_LEAF_PLACE : VX_K_NODE_PLACE, s);
-    place_kernel<kMax><<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, leaves);
+    place_kernel<kMax><<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, list, res, sc, leaves);
   }
   return last_launch();
+}
+
+template <uint32_t kMax>
+vx_status intern_width(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet set, uint64_t* hashes,
+                       const uint32_t* n_ptr, uint64_t n_ub, uint64_t* res, void* fused_ws, hipStream_t s,
+                       bool clear_ws) {
+  {
+    TimedLaunch tl(leaves ? VX_K_LEAF_PROBE : VX_K_NODE_PROBE, s);
+    probe_kernel<kMax><<<grid_for(n_ub), 256, 0, s>>>(set, hashes, n_ptr, res, sc, n_scenes, leaves);
+  }
+  const vx_status st = last_launch();
+  if (st != VX_OK) return st;
+  return commit_place<kMax>(sc, n_scenes, leaves, set, hashes, n_ptr, nullptr, n_ub, res, fused_ws, s, clear_ws);
 }
 
 vx_status intern_scenes(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet set, uint64_t* hashes,
@@ -844,98 +862,161 @@ __global__ void topdown_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t*
 // instead of in a separate top-down pass over the leaves: this kernel is
 // bandwidth-bound with many entries in flight, so the dependent reads hide
 // behind the streaming work.
+// The leaves the build leaves PENDING, listed in entry order without global
+// atomics (one counter hit by every wave of the chip serialises) and without
+// giving up the build's grid-stride order (the whole chip works through one
+// window of entries, whose parents and old nodes then stay in L2): slice s =
+// entries [64 s, 64 s + 64), the span one wave covers in one step, keeps its
+// pending entries, in order, in pend[64 s ...] and their number in count[s];
+// a fused scan over the slice counts gives each slice's base in the list,
+// and the leaf probe walks the slices, writing the list as it goes.
+constexpr uint32_t kLeafBlock = 256;
+
+struct LeafSlices {
+  uint32_t* pend;      // 64 per slice
+  uint32_t* count;     // per slice
+  uint32_t* base;      // per slice: exclusive prefix of count
+  uint32_t* list;      // the pending entries, contiguous, in entry order
+  uint32_t* n_slices;  // ceil(leaves / 64), written by the build
+  uint32_t* total;     // pending entries
+};
+
+struct SliceLoad {
+  using Payload = scan::NoPayload;
+  const uint32_t* count;
+  __device__ uint32_t operator()(uint32_t s, Payload&) const { return count[s]; }
+};
+struct SliceStore {
+  uint32_t* base;
+  uint32_t* total;
+  __device__ void operator()(uint32_t s, const scan::NoPayload&, uint32_t, uint32_t excl, uint32_t incl,
+                             uint32_t n) const {
+    base[s] = excl;
+    if (s == n - 1) *total = incl;
+  }
+};
+
 template <uint32_t LW>
-__global__ void leaf_build_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ parent,
-                                  const uint32_t* __restrict__ old_up, const uint32_t* __restrict__ first,
-                                  SortedRecs recs, const uint64_t* __restrict__ rmask,
-                                  const uint32_t* __restrict__ rmat, uint32_t* __restrict__ items,
-                                  uint64_t* __restrict__ hashes, uint64_t* __restrict__ res) {
+__global__ __launch_bounds__(kLeafBlock, 8) void leaf_build_kernel(
+    Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ parent, const uint32_t* __restrict__ old_up,
+    const uint32_t* __restrict__ first, SortedRecs recs, const uint64_t* __restrict__ rmask,
+    const uint32_t* __restrict__ rmat, uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
+    uint64_t* __restrict__ res, LeafSlices ls) {
   const uint32_t n = *n_ptr;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    const Dev& d = sc.of(j);
-    const uint32_t ref =
-        d.depth == 0 ? d.ctl[kCtlRoot] : child_of(d.node_pool, old_up[parent[j]], uint32_t(sc.keys[j] & 7u));
-    uint32_t w[LW];
-    if (ref == kEmpty) {
+  const uint32_t slices = (n + kWave - 1) / kWave;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ls.n_slices = slices;
+  const uint32_t waves = gridDim.x * (kLeafBlock / kWave);
+  for (uint32_t sl = blockIdx.x * (kLeafBlock / kWave) + wave_id(); sl < slices; sl += waves) {
+    const uint32_t j = sl * kWave + lane_id();
+    bool pending = false;
+    if (j < n) {
+      const Dev& d = sc.of(j);
+      const uint32_t ref =
+          d.depth == 0 ? d.ctl[kCtlRoot] : child_of(d.node_pool, old_up[parent[j]], uint32_t(sc.keys[j] & 7u));
+      uint32_t w[LW];
+      if (ref == kEmpty) {
 #pragma unroll
-      for (uint32_t k = 0; k < LW; ++k) w[k] = 0;
-    } else if (ref == kFull) {
+        for (uint32_t k = 0; k < LW; ++k) w[k] = 0;
+      } else if (ref == kFull) {
 #pragma unroll
-      for (uint32_t k = 0; k < LW; ++k) w[k] = 0xffffffffu;
-    } else {
-      load_item<LW>(sc.of(j).leaf_pool + ref, LW, w);
-    }
-    const uint32_t r1 = first[j + 1];
-    for (uint32_t r = first[j]; r < r1; ++r) {
-      const uint32_t idx = recs.index(r);
-      const uint64_t m = rmask[idx];
-      const uint32_t mat = rmat[idx];
-      if constexpr (LW == 2) {
-        const uint32_t lo = uint32_t(m), hi = uint32_t(m >> 32);
-        if (mat) {
-          w[0] |= lo;
-          w[1] |= hi;
+        for (uint32_t k = 0; k < LW; ++k) w[k] = 0xffffffffu;
+      } else {
+        load_item<LW>(sc.of(j).leaf_pool + ref, LW, w);
+      }
+      const uint32_t r1 = first[j + 1];
+      for (uint32_t r = first[j]; r < r1; ++r) {
+        const uint32_t idx = recs.index(r);
+        const uint64_t m = rmask[idx];
+        const uint32_t mat = rmat[idx];
+        if constexpr (LW == 2) {
+          const uint32_t lo = uint32_t(m), hi = uint32_t(m >> 32);
+          if (mat) {
+            w[0] |= lo;
+            w[1] |= hi;
+          } else {
+            w[0] &= ~lo;
+            w[1] &= ~hi;
+          }
         } else {
-          w[0] &= ~lo;
-          w[1] &= ~hi;
-        }
-      } else {
-        const uint32_t fill = mat * 0x01010101u;
+          const uint32_t fill = mat * 0x01010101u;
 #pragma unroll
-        for (uint32_t k = 0; k < LW; ++k) {
-          const uint32_t bm = nibble_bytes(uint32_t(m >> (4 * k)) & 15u);
-          w[k] = (w[k] & ~bm) | (fill & bm);
+          for (uint32_t k = 0; k < LW; ++k) {
+            const uint32_t bm = nibble_bytes(uint32_t(m >> (4 * k)) & 15u);
+            w[k] = (w[k] & ~bm) | (fill & bm);
+          }
+        }
+      }
+      uint32_t any = 0, all = 0xffffffffu;
+#pragma unroll
+      for (uint32_t k = 0; k < LW; ++k) {
+        any |= w[k];
+        all &= w[k];
+      }
+      if (!any) {
+        res[j] = res_make(kResFinal, kEmpty);
+      } else if (LW == 2 && all == 0xffffffffu) {
+        res[j] = res_make(kResFinal, kFull);
+      } else {
+        const uint64_t h = hash_words(w, LW);
+        const uint32_t v = find_interned<LW>(table_of(d, true), w, LW, h);
+        if (v) {
+          res[j] = res_make(kResFinal, v);
+        } else {
+          uint32_t* it = items + size_t(j) * LW;
+#pragma unroll
+          for (uint32_t k = 0; k < LW; ++k) it[k] = w[k];
+          hashes[j] = h;
+          res[j] = res_make(kResPending, 0);
+          pending = true;
         }
       }
     }
-    uint32_t any = 0, all = 0xffffffffu;
-#pragma unroll
-    for (uint32_t k = 0; k < LW; ++k) {
-      any |= w[k];
-      all &= w[k];
-    }
-    if (!any) {
-      res[j] = res_make(kResFinal, kEmpty);
-    } else if (LW == 2 && all == 0xffffffffu) {
-      res[j] = res_make(kResFinal, kFull);
-    } else {
-      const uint64_t h = hash_words(w, LW);
-      const uint32_t v = find_interned<LW>(table_of(d, true), w, LW, h);
-      if (v) {
-        res[j] = res_make(kResFinal, v);
-      } else {
-        uint32_t* it = items + size_t(j) * LW;
-#pragma unroll
-        for (uint32_t k = 0; k < LW; ++k) it[k] = w[k];
-        hashes[j] = h;
-        res[j] = res_make(kResPending, 0);
-      }
-    }
+    const uint64_t pm = ballot(pending);
+    if (pending) ls.pend[sl * kWave + lanes_below(pm)] = j;
+    if (lane_id() == 0) ls.count[sl] = uint32_t(__popcll(pm));
   }
 }
 
-// Node at height h: old item plus the changed children (entries of height
-// h - 1 whose parent is this entry).
-// Node item of entry j: the old node's children, overwritten by the new refs
-// of the edited children (the entries of the height below whose parent is j).
-__device__ __forceinline__ void node_build_entry(const Scenes& sc, const uint32_t* __restrict__ old,
-                                                 const uint32_t* __restrict__ first, const uint64_t* __restrict__ ckeys,
-                                                 const uint32_t* __restrict__ cparent, const uint64_t* __restrict__ cres,
-                                                 uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
-                                                 uint64_t* __restrict__ res, uint32_t j) {
-  uint32_t ch[8];
-  children_of(sc.of(j).node_pool, old[j], ch);
-  const uint32_t k1 = first[j + 1];
-  for (uint32_t k = first[j]; k < k1; ++k)
-    if (cparent[k] == j) ch[uint32_t(ckeys[k] & 7u)] = resolve(cres, k);
-  uint32_t* it = items + size_t(j) * kNodeStride;
-  uint32_t ref = 0;
-  const uint32_t w = make_node_item(ch, it, &ref);
-  if (!w) {
-    res[j] = res_make(kResFinal, ref);
-  } else {
-    hashes[j] = hash_words(it, w);
-    res[j] = res_make(kResPending, 0);
+// Leaf probe over the slices: a wave takes kProbeSlices slices at a time and
+// spreads their pending entries over its lanes (a slice holds a few pending
+// entries on average, so a wave per slice would leave most lanes idle). The
+// bases are an exclusive scan in slice order, so entry e of the group (in
+// order) goes to list position base[first slice] + e.
+constexpr uint32_t kProbeSlices = 16;
+
+template <uint32_t kMax>
+__global__ void probe_slices_kernel(ItemSet set, const uint64_t* __restrict__ hashes, LeafSlices ls, uint64_t* res,
+                                    Scenes sc, uint32_t n_scenes) {
+  if (blockIdx.x == 0)
+    for (uint32_t s = threadIdx.x; s < n_scenes; s += blockDim.x) {
+      const Table t = table_of(sc.at(s), true);
+      *t.base = *t.used;
+    }
+  const uint32_t slices = *ls.n_slices;
+  const uint32_t lane = lane_id();
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t s0 = (blockIdx.x * (blockDim.x / kWave) + wave_id()) * kProbeSlices; s0 < slices;
+       s0 += waves * kProbeSlices) {
+    const uint32_t c = lane < kProbeSlices && s0 + lane < slices ? ls.count[s0 + lane] : 0u;
+    const uint32_t incl = wave_inclusive_sum(c);  // lanes >= kProbeSlices carry the group total
+    const uint32_t total = wave_last(incl);
+    const uint32_t b0 = ls.base[s0];
+    for (uint32_t e = lane; e < total; e += kWave) {
+      // Slice q of the group holding entry e: the number of slices whose
+      // inclusive count is <= e; ex = the entries of the slices before it.
+      uint32_t q = 0, ex = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kProbeSlices; ++k) {
+        const uint32_t ik = uint32_t(__builtin_amdgcn_readlane(int(incl), int(k)));
+        if (ik <= e) {
+          q = k + 1;
+          ex = ik;
+        }
+      }
+      const uint32_t i = ls.pend[(s0 + q) * kWave + (e - ex)];
+      ls.list[b0 + e] = i;
+      probe_entry<kMax>(set, hashes, res, sc, true, i);
+    }
   }
 }
 
@@ -1016,8 +1097,10 @@ struct Heights {
   uint64_t cap[kMaxHeights + 1];
 };
 
+constexpr uint32_t kCntSlices = 59, kCntPending = 60;
+
 struct EditWs {
-  uint32_t* counts;  // [0] records, [1 + h] entries at height h
+  uint32_t* counts;  // [0] records, [1 + h] entries at height h, [kCntPending] pending leaves
   uint32_t* root_pos;
   uint64_t *rkey, *rkey_alt, *rmask;
   uint32_t *rval, *rval_alt, *rmat;
@@ -1034,6 +1117,8 @@ struct EditWs {
   void* run_ws;
   void* up_ws[kMaxHeights + 1];
   void* commit_ws[kMaxHeights + 1];
+  uint32_t *pend, *pend_list, *slice_count, *slice_base;  // pending leaves (LeafSlices)
+  void* slice_ws;
   BrushJob* jobs;
   Dev* scenes;
   size_t scenes_off, jobs_off;  // byte offsets from counts (the workspace base)
@@ -1304,11 +1389,17 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
     max_items = std::max<uint64_t>(max_items, cap * (h == 0 ? p.leaf_words : kNodeStride));
   }
   w.items = c.take<uint32_t>(max_items);
+  const uint64_t slices = (w.hs.cap[0] + kWave - 1) / kWave;
+  w.pend = c.take<uint32_t>(slices * kWave);
+  w.pend_list = c.take<uint32_t>(w.hs.cap[0]);
+  w.slice_count = c.take<uint32_t>(slices);
+  w.slice_base = c.take<uint32_t>(slices);
   w.hashes = c.take<uint64_t>(max_cap);
   {
     char* start = c.take<char>(0);
     w.rec_ws = c.take<char>(scan::fused_dev_workspace_bytes(std::max<uint64_t>(candidates, 1)));
     w.run_ws = c.take<char>(scan::fused_dev_workspace_bytes(R));
+    w.slice_ws = c.take<char>(scan::fused_dev_workspace_bytes((w.hs.cap[0] + kWave - 1) / kWave));
     for (uint32_t h = 0; h <= H; ++h) {
       w.up_ws[h] = c.take<char>(scan::fused_dev_workspace_bytes(w.hs.cap[h]));
       w.commit_ws[h] = c.take<char>(scan::fused_dev_workspace_bytes(w.hs.cap[h]));
@@ -1397,19 +1488,40 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
   {
     Scenes sh = sc;
     sh.keys = L.keys[0];
+    const LeafSlices ls{w.pend, w.slice_count, w.slice_base, w.pend_list, &cnt[kCntSlices], &cnt[kCntPending]};
     {
       TimedLaunch tl(VX_K_LEAF_BUILD, s);
       if (p.leaf_words == 2)
-        leaf_build_kernel<2><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.parent[0], L.old[1], L.first[0], recs,
-                                                                w.rmask, w.rmat, w.items, w.hashes, L.res[0]);
+        leaf_build_kernel<2><<<grid_for(L.cap[0]), kLeafBlock, 0, s>>>(sh, &cnt[1], L.parent[0], L.old[1],
+                                                                       L.first[0], recs, w.rmask, w.rmat, w.items,
+                                                                       w.hashes, L.res[0], ls);
       else
-        leaf_build_kernel<16><<<grid_for(L.cap[0]), 256, 0, s>>>(sh, &cnt[1], L.parent[0], L.old[1], L.first[0],
-                                                                 recs, w.rmask, w.rmat, w.items, w.hashes, L.res[0]);
+        leaf_build_kernel<16><<<grid_for(L.cap[0]), kLeafBlock, 0, s>>>(sh, &cnt[1], L.parent[0], L.old[1],
+                                                                        L.first[0], recs, w.rmask, w.rmat, w.items,
+                                                                        w.hashes, L.res[0], ls);
     }
     if ((st = last_launch()) != VX_OK) return st;
-    if ((st = intern_scenes(sh, p.n_scenes, true, ItemSet{w.items, p.leaf_words, p.leaf_words}, w.hashes, &cnt[1],
-                            L.cap[0], L.res[0], w.commit_ws[0], s, false)) != VX_OK)
-      return st;
+    {
+      // The slice scan is timed with the probe (one kind).
+      TimedLaunch tl(VX_K_LEAF_PROBE, s);
+      const uint64_t slices_ub = (L.cap[0] + kWave - 1) / kWave;
+      if ((st = scan::fused_dev_m<scan::SumOp>(ls.n_slices, slices_ub, SliceLoad{ls.count},
+                                               SliceStore{ls.base, ls.total}, w.slice_ws, s, false)) != VX_OK)
+        return st;
+      if (p.leaf_words == 2)
+        probe_slices_kernel<2><<<grid_for(L.cap[0]), 256, 0, s>>>(ItemSet{w.items, 2, 2}, w.hashes, ls, L.res[0], sh,
+                                                                  p.n_scenes);
+      else
+        probe_slices_kernel<16><<<grid_for(L.cap[0]), 256, 0, s>>>(ItemSet{w.items, 16, 16}, w.hashes, ls, L.res[0],
+                                                                   sh, p.n_scenes);
+    }
+    if ((st = last_launch()) != VX_OK) return st;
+    const ItemSet leaf_set{w.items, p.leaf_words, p.leaf_words};
+    st = p.leaf_words == 2 ? commit_place<2>(sh, p.n_scenes, true, leaf_set, w.hashes, ls.total, ls.list, L.cap[0],
+                                             L.res[0], w.commit_ws[0], s, false)
+                           : commit_place<16>(sh, p.n_scenes, true, leaf_set, w.hashes, ls.total, ls.list,
+                                              L.cap[0], L.res[0], w.commit_ws[0], s, false);
+    if (st != VX_OK) return st;
   }
   mark(g, 5, s);
   for (uint32_t h = 1; h < hu; ++h) {
