@@ -551,6 +551,7 @@ def kernel_bytes(prof, leaf_words):
     where every entry does it; random 4..8-byte reads count their own size
     (the line-granular cost is what the hardware byte counters report)."""
     R, L, N, P = prof.records, prof.leaves, prof.nodes, prof.sort_passes
+    Q = prof.pending_leaves  # leaves the build's lookup did not find: the probe, commit and place run over these
     lw = 4 * leaf_words
     return {
         "records": (prof.sort_record_bytes + 12) * R,   # sort record + mask 8 + material 4 written
@@ -559,12 +560,15 @@ def kernel_bytes(prof, leaf_words):
         "runs": 8 * R + 16 * L,                         # sorted keys read; leaf keys, run starts, parent links
         "up": 24 * N,                                   # heights >= 1: keys read + parent links; parent keys, starts
         "topdown": 48 * N,                              # parent index, parent ref, parent item, own ref
-        "leaf_build": L * (40 + lw) + (prof.sort_record_bytes + 12) * R,  # parent index, parent ref, parent
-                                                        # item word, range, item, hash, result; per record: its
-                                                        # sorted entry, mask and material
-        "leaf_probe": 16 * L,                           # result read and written (items probed: not counted)
-        "leaf_commit": 16 * L,                          # result read, slot value written (representatives only)
-        "leaf_place": 8 * L,                            # result read (copies of new items: not counted)
+        "leaf_build": L * (48 + lw) + (prof.sort_record_bytes + 12) * R,  # parent index, parent ref, parent
+                                                        # item word, range, slot, result, and the item (written
+                                                        # when new, else the interned one compared); per record:
+                                                        # its sorted entry, mask and material
+        "leaf_probe": (40 + lw) * Q + L // 16,          # pending entry, list entry, item, hash, result read and
+                                                        # written; slice counts and bases
+        "leaf_commit": 28 * Q,                          # list entry, result, hash read; slot value written
+        "leaf_place": 20 * Q,                           # list entry, result, slot value (copies of new items:
+                                                        # not counted)
         "node_build": 64 * N + 20 * (L + N),            # old ref, starts, item, hash, result; child entries
         "node_probe": 16 * N,
         "node_commit": 16 * N,

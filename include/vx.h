@@ -461,6 +461,7 @@ typedef struct vx_dag_profile {
   uint64_t sort_record_bytes; /* bytes one record occupies in the sort: 12 (key, index) or 8 (packed) */
   uint64_t kernel_launches[VX_DAG_KERNELS];
   double kernel_ms[VX_DAG_KERNELS];
+  uint64_t pending_leaves; /* leaves not found among the interned items by the build (sum): the probe's work */
 } vx_dag_profile;
 vx_status vx_dag_profile_enable(vx_dag* dag, int32_t enable);
 vx_status vx_dag_profile_read(const vx_dag* dag, vx_dag_profile* out);

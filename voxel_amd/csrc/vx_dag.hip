@@ -1613,6 +1613,7 @@ vx_status finish(vx_dag* g, const std::vector<vx_dag*>& dags, const EditWs& w, c
     g->prof.edits += 1;
     g->prof.records += counts[0];
     g->prof.leaves += counts[1];
+    g->prof.pending_leaves += counts[kCntPending];
     for (uint32_t h = 1; h <= p.max_depth; ++h) g->prof.nodes += counts[1 + h];
     const uint32_t kb = std::max(p.key_bits, 1u);
     g->prof.sort_passes = w.index_bits ? sort::keys_u64_dev_passes(kb) : sort::pairs_u64_dev_passes(kb);

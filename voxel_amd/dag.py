@@ -88,7 +88,8 @@ KERNELS = ("records", "sort_histogram", "sort_scan", "sort_partition", "sort_cop
 class Profile(C.Structure):
     _fields_ = [("ms", C.c_double * 7), ("edits", C.c_uint64), ("records", C.c_uint64), ("leaves", C.c_uint64),
                 ("nodes", C.c_uint64), ("sort_passes", C.c_uint64), ("sort_record_bytes", C.c_uint64),
-                ("kernel_launches", C.c_uint64 * len(KERNELS)), ("kernel_ms", C.c_double * len(KERNELS))]
+                ("kernel_launches", C.c_uint64 * len(KERNELS)), ("kernel_ms", C.c_double * len(KERNELS)),
+                ("pending_leaves", C.c_uint64)]
 
 
 STREAM_BRUSH = 5
