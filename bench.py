@@ -910,6 +910,8 @@ def main():
             "brick_records_per_step": prof.records // max(prof.edits, 1),
             "leaves_per_step": prof.leaves // max(prof.edits, 1),
             "nodes_per_step": prof.nodes // max(prof.edits, 1),
+            "pending_leaves_per_step": prof.pending_leaves // max(prof.edits, 1),
+            "entries_per_height": [int(x) // max(prof.edits, 1) for x in prof.height_entries if x],
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -462,6 +462,7 @@ typedef struct vx_dag_profile {
   uint64_t kernel_launches[VX_DAG_KERNELS];
   double kernel_ms[VX_DAG_KERNELS];
   uint64_t pending_leaves; /* leaves not found among the interned items by the build (sum): the probe's work */
+  uint64_t height_entries[16]; /* entries per height (sum; [0] leaves, [h] inner nodes at height h) */
 } vx_dag_profile;
 vx_status vx_dag_profile_enable(vx_dag* dag, int32_t enable);
 vx_status vx_dag_profile_read(const vx_dag* dag, vx_dag_profile* out);

@@ -1082,9 +1082,18 @@ __global__ void publish_kernel(Scenes sc, uint32_t n_scenes, const uint32_t* __r
   }
 }
 
-__global__ void begin_kernel(Scenes sc, uint32_t n_scenes) {
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n_scenes; s += gridDim.x * blockDim.x)
-    sc.at(s).ctl[kCtlStatus] = kStatusOk;
+// Start of an edit, one launch (a fill per buffer would cost a launch each):
+// every scene's status OK, the first `n_counts` count words zero, no root
+// position yet, and the fused scans' claim counters and look-back rows zero.
+__global__ void begin_kernel(Scenes sc, uint32_t n_scenes, uint32_t* counts, uint32_t n_counts, uint32_t* root_pos,
+                             uint32_t* fused, uint64_t fused_words) {
+  const uint64_t t0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x, stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t s = t0; s < n_scenes; s += stride) {
+    sc.at(uint32_t(s)).ctl[kCtlStatus] = kStatusOk;
+    root_pos[s] = kNone;
+  }
+  for (uint64_t i = t0; i < n_counts; i += stride) counts[i] = 0;
+  for (uint64_t i = t0; i < fused_words; i += stride) fused[i] = 0;
 }
 
 // ---- workspace ---------------------------------------------------------------
@@ -1614,6 +1623,7 @@ vx_status finish(vx_dag* g, const std::vector<vx_dag*>& dags, const EditWs& w, c
     g->prof.records += counts[0];
     g->prof.leaves += counts[1];
     g->prof.pending_leaves += counts[kCntPending];
+    for (uint32_t h = 0; h <= p.max_depth && h < 16; ++h) g->prof.height_entries[h] += counts[1 + h];
     for (uint32_t h = 1; h <= p.max_depth; ++h) g->prof.nodes += counts[1 + h];
     const uint32_t kb = std::max(p.key_bits, 1u);
     g->prof.sort_passes = w.index_bits ? sort::keys_u64_dev_passes(kb) : sort::pairs_u64_dev_passes(kb);
@@ -1693,12 +1703,11 @@ template <class Load>
 vx_status device_pipeline(vx_dag* prof_dag, const Plan& p, const Scenes& sc, EditWs& w, uint64_t cand_cap, Load load,
                           hipStream_t s) {
   vx_status st;
-  if ((st = fill_u32(w.counts, kCntCandidates, 0, s)) != VX_OK) return st;
-  if ((st = fill_u32(w.root_pos, p.n_scenes, kNone, s)) != VX_OK) return st;
-  if ((st = fill_u32(w.fused_all, w.fused_bytes / 4, 0, s)) != VX_OK) return st;
   {
     TimedLaunch tl(VX_K_OTHER, s);
-    begin_kernel<<<grid_for(p.n_scenes), 256, 0, s>>>(sc, p.n_scenes);
+    const uint64_t fused_words = w.fused_bytes / 4;
+    begin_kernel<<<grid_for(std::max<uint64_t>(fused_words, p.n_scenes)), 256, 0, s>>>(
+        sc, p.n_scenes, w.counts, kCntCandidates, w.root_pos, w.fused_all, fused_words);
   }
   if ((st = last_launch()) != VX_OK) return st;
   mark(prof_dag, 0, s);

@@ -89,7 +89,7 @@ class Profile(C.Structure):
     _fields_ = [("ms", C.c_double * 7), ("edits", C.c_uint64), ("records", C.c_uint64), ("leaves", C.c_uint64),
                 ("nodes", C.c_uint64), ("sort_passes", C.c_uint64), ("sort_record_bytes", C.c_uint64),
                 ("kernel_launches", C.c_uint64 * len(KERNELS)), ("kernel_ms", C.c_double * len(KERNELS)),
-                ("pending_leaves", C.c_uint64)]
+                ("pending_leaves", C.c_uint64), ("height_entries", C.c_uint64 * 16)]
 
 
 STREAM_BRUSH = 5
