@@ -814,9 +814,16 @@ def main():
     # Timed steps: stage events plus events around the dominant kernel's
     # launches only (its average launch duration is measured over these steps).
     t_first = args.warmup + n_prof + n_conf
+    # Every event costs the stream a few microseconds, so the timed steps carry
+    # only what the roofline needs: with a profiled pass before them (stage
+    # times and launches per edit come from it), no stage events and one
+    # launch of the dominant kind timed per edit, rotating over its launches.
+    lpe = int(round(kernels[dom]["launches_per_edit"])) if dom in kernels else 1
     scene.profile(False)
-    scene.profile(True)
+    scene.profile(True, stages=not n_prof)
     scene.profile_kernels([dom])
+    if n_prof and lpe > 1:
+        scene.profile_period(lpe)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     step_ends = []
@@ -832,10 +839,13 @@ def main():
     value = timed_ops / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
 
-    stages = {name: prof.ms[i] / max(prof.edits, 1) for i, name in enumerate(V.STAGES)}
+    sp = pp if n_prof else prof
+    stages = {name: sp.ms[i] / max(sp.edits, 1) for i, name in enumerate(V.STAGES)}
     di = V.KERNELS.index(dom)
-    avg_us = prof.kernel_ms[di] * 1e3 / max(prof.kernel_launches[di], 1)
-    bytes_per_launch = kernel_bytes(prof, 16).get(dom, 0) / max(prof.kernel_launches[di], 1)
+    avg_us = prof.kernel_ms[di] * 1e3 / max(prof.kernel_launches[di], 1)  # over the timed launches
+    # Algorithmic bytes of the timed edits over all their launches of the kind.
+    launches_all = lpe * prof.edits if (n_prof and lpe > 1) else prof.kernel_launches[di]
+    bytes_per_launch = kernel_bytes(prof, 16).get(dom, 0) / max(launches_all, 1)
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
 
     rays = ray_leg(torch, scene, world, dev)

@@ -464,12 +464,23 @@ typedef struct vx_dag_profile {
   uint64_t pending_leaves; /* leaves not found among the interned items by the build (sum): the probe's work */
   uint64_t height_entries[16]; /* entries per height (sum; [0] leaves, [h] inner nodes at height h) */
 } vx_dag_profile;
+/* enable: 0 off, VX_PROFILE_ON per-stage and per-kernel events; with
+ * VX_PROFILE_NO_STAGES added, per-kernel events only (each stage event costs
+ * the stream a few microseconds). Clears the accumulated profile and the
+ * kernel period. */
+#define VX_PROFILE_ON 1
+#define VX_PROFILE_NO_STAGES 2
 vx_status vx_dag_profile_enable(vx_dag* dag, int32_t enable);
 vx_status vx_dag_profile_read(const vx_dag* dag, vx_dag_profile* out);
 /* Which kernel kinds get per-launch events while profiled (bit k = VX_K_k;
  * default all): timing only the kernel under study keeps the event overhead
  * out of the other launches. */
 vx_status vx_dag_profile_kernel_mask(vx_dag* dag, uint64_t mask);
+/* period > 1: of the launches of a timed kind within one call, only the one at
+ * index (call number mod period) gets events; over `period` calls each index
+ * is timed once (one event pair per call for a kind launched `period` times
+ * per call). */
+vx_status vx_dag_profile_kernel_period(vx_dag* dag, uint32_t period);
 
 #ifdef __cplusplus
 }

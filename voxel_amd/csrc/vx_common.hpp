@@ -172,6 +172,10 @@ __host__ __device__ __forceinline__ uint32_t uniform_u32(uint64_t r, uint32_t lo
 // host thread, every launch bracketed by a TimedLaunch gets a hipEvent pair
 // on its stream, tagged with its kernel kind (VX_K_*). Scopes do not nest:
 // an inner TimedLaunch inside an outer one is a no-op.
+// With period p > 1, of the launches of a timed kind within one call only
+// the one at index (phase mod p) gets events, phase advancing by one per
+// call: over p calls every position is timed once, with one event pair per
+// call instead of one per launch.
 struct KernelTimer {
   static constexpr uint32_t kEvents = 1024;
   hipEvent_t ev[kEvents] = {};
@@ -179,6 +183,9 @@ struct KernelTimer {
   uint32_t used = 0;
   uint32_t depth = 0;
   uint64_t mask = ~0ull;  // kinds that are timed
+  uint32_t period = 1;
+  uint32_t phase = 0;
+  uint32_t seen[64] = {};  // launches of each kind in the current call
 };
 KernelTimer*& current_timer();
 
@@ -189,6 +196,7 @@ struct TimedLaunch {
   TimedLaunch(uint32_t kind, hipStream_t stream) : s(stream) {
     KernelTimer* k = current_timer();
     if (!k || k->depth++ > 0 || k->used + 2 > KernelTimer::kEvents || !((k->mask >> kind) & 1u)) return;
+    if (k->period > 1 && (k->seen[kind & 63]++ % k->period) != k->phase % k->period) return;
     t = k;
     slot = k->used;
     k->kind[slot / 2] = kind;

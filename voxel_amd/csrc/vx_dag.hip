@@ -1422,7 +1422,7 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
 }
 
 inline void mark(vx_dag* g, uint32_t i, hipStream_t s) {
-  if (g && g->profile) hipEventRecord(g->ev[i], s);
+  if (g && g->profile && g->stage_events) hipEventRecord(g->ev[i], s);
 }
 
 // Everything after record generation: sort, heights, old refs, rebuild,
@@ -1615,7 +1615,7 @@ vx_status finish(vx_dag* g, const std::vector<vx_dag*>& dags, const EditWs& w, c
     return VX_DEVICE_ERROR;
   if (hipStreamSynchronize(s) != hipSuccess) return VX_DEVICE_ERROR;
   if (prof) {
-    for (uint32_t i = 0; i < VX_DAG_STAGES; ++i) {
+    for (uint32_t i = 0; i < VX_DAG_STAGES && g->stage_events; ++i) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, g->ev[i], g->ev[i + 1]) == hipSuccess) g->prof.ms[i] += ms;
     }
@@ -1733,15 +1733,17 @@ vx_status run_edit(const std::vector<vx_dag*>& dags, vx_dag* prof_dag, const Pla
     // Per-launch timing of this call's kernels while the dag is profiled.
     struct Install {
       KernelTimer* prev;
-      explicit Install(KernelTimer* t) : prev(current_timer()) {
+      Install(KernelTimer* t, uint64_t call) : prev(current_timer()) {
         if (t) {
           t->used = 0;
           t->depth = 0;
+          t->phase = uint32_t(call % t->period);
+          for (uint32_t& c : t->seen) c = 0;
           current_timer() = t;
         }
       }
       ~Install() { current_timer() = prev; }
-    } install(prof_dag && prof_dag->profile ? prof_dag->timer : nullptr);
+    } install(prof_dag && prof_dag->profile ? prof_dag->timer : nullptr, prof_dag ? prof_dag->prof.edits : 0);
     if ((st = device_pipeline(prof_dag, p, sc, w, n_cand, load, s)) != VX_OK) return st;
   }
   return finish(prof_dag, dags, w, p, hs, s);
@@ -2091,14 +2093,22 @@ extern "C" vx_status vx_dag_profile_enable(vx_dag* g, int32_t enable) {
         if (hipEventCreate(&e) != hipSuccess) return VX_DEVICE_ERROR;
     }
   }
-  g->profile = enable != 0;
+  g->profile = (enable & VX_PROFILE_ON) != 0;
+  g->stage_events = (enable & VX_PROFILE_NO_STAGES) == 0;
   g->prof = vx_dag_profile{};
+  if (g->timer) g->timer->period = 1;
   return VX_OK;
 }
 
 extern "C" vx_status vx_dag_profile_kernel_mask(vx_dag* g, uint64_t mask) {
   if (!g || !g->timer) return VX_INVALID_VALUE;  // enable profiling first
   g->timer->mask = mask;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_dag_profile_kernel_period(vx_dag* g, uint32_t period) {
+  if (!g || !g->timer || period == 0) return VX_INVALID_VALUE;  // enable profiling first
+  g->timer->period = period;
   return VX_OK;
 }
 

@@ -238,6 +238,7 @@ struct vx_dag {
   // Optional per-stage and per-kernel timing of edits (hipEvents on the
   // edit's stream).
   bool profile = false;
+  bool stage_events = true;  // per-stage events while profiled
   hipEvent_t ev[VX_DAG_STAGES + 1] = {};
   vx::KernelTimer* timer = nullptr;
   vx_dag_profile prof{};

@@ -75,6 +75,7 @@ register({
     "vx_dag_profile_enable": (_i32, [_P, _i32]),
     "vx_dag_profile_read": (_i32, [_P, _P]),
     "vx_dag_profile_kernel_mask": (_i32, [_P, _u64]),
+    "vx_dag_profile_kernel_period": (_i32, [_P, _u32]),
 })
 
 STAGES = ("records", "sort", "levels", "old_refs", "leaves", "nodes", "publish")
@@ -242,8 +243,14 @@ class HashDagGpu:
         check(lib().vx_dag_raycast(self.h, _ptr(r), n, _ptr(out), _stream(stream)), "vx_dag_raycast")
         return out
 
-    def profile(self, enable: bool = True):
-        check(lib().vx_dag_profile_enable(self.h, int(enable)), "vx_dag_profile_enable")
+    def profile(self, enable: bool = True, stages: bool = True):
+        """Per-stage and per-kernel events (stages=False: per-kernel only)."""
+        mode = (1 | (0 if stages else 2)) if enable else 0
+        check(lib().vx_dag_profile_enable(self.h, mode), "vx_dag_profile_enable")
+
+    def profile_period(self, period: int):
+        """Time one launch per call of each timed kind, rotating over `period` positions."""
+        check(lib().vx_dag_profile_kernel_period(self.h, period), "vx_dag_profile_kernel_period")
 
     def profile_kernels(self, kinds=None):
         """Time only these kernel kinds (names from KERNELS; None = all)."""
