@@ -336,7 +336,9 @@ def test_kernel_profile_counts_every_launch(gpu):
     # single-workgroup launch for the rest.
     assert launches["node_probe"] % 3 == 0 and launches["node_probe"] <= 3 * 7
     assert launches["tail_nodes"] == (3 if launches["node_probe"] < 3 * 7 else 0)
-    assert launches["tail_up"] == launches["tail_topdown"] == launches["tail_nodes"]
+    assert launches["tail_up"] == launches["tail_nodes"]
+    # Old refs: one root-walk launch for heights >= 2, one parent step for height 1.
+    assert launches["tail_topdown"] == 0 and launches["topdown"] == 3 * 2
     assert sum(ms.values()) <= sum(p.ms) * 1.05 + 0.05
     assert all(v > 0 for k, v in ms.items() if launches[k])
 

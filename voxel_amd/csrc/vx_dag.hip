@@ -857,6 +857,41 @@ __global__ void topdown_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t*
     topdown_entry(sc, parent, old_up, old, h, j);
 }
 
+// Old refs of the entries of heights h_lo..h_hi in one launch (one launch per
+// height would cost a launch each for a few thousand entries): each entry
+// walks down from its scene's root along its own key, through nodes that
+// the other walks keep in L2 near the root. Same result as the height-by-
+// height pass: the child of the parent's old node, a sentinel staying a
+// sentinel.
+struct OldRefHeights {
+  const uint64_t* keys[kMaxHeights + 1];
+  uint32_t* old[kMaxHeights + 1];
+};
+
+__global__ void old_refs_kernel(Scenes sc, const uint32_t* __restrict__ counts, OldRefHeights hs, uint32_t h_lo,
+                                uint32_t h_hi) {
+  __shared__ uint32_t start[kMaxHeights + 2];  // flat index of each height's first entry
+  if (threadIdx.x == 0) {
+    start[h_lo] = 0;
+    for (uint32_t h = h_lo; h <= h_hi; ++h) start[h + 1] = start[h] + counts[1 + h];
+  }
+  __syncthreads();
+  const uint32_t total = start[h_hi + 1];
+  const uint64_t mask = (uint64_t(1) << sc.codec.shift) - 1;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    uint32_t h = h_lo;
+    while (g >= start[h + 1]) ++h;
+    const uint32_t j = g - start[h];
+    Scenes sh = sc;
+    sh.keys = hs.keys[h];
+    const Dev& d = sh.of(j);
+    const uint64_t m = hs.keys[h][j] & mask;
+    uint32_t ref = d.ctl[kCtlRoot];
+    for (uint32_t l = d.depth - h; l-- > 0 && ref > kFull;) ref = child_of(d.node_pool, ref, uint32_t(m >> (3 * l)) & 7u);
+    hs.old[h][j] = ref;
+  }
+}
+
 // ---- rebuild ---------------------------------------------------------------
 // The leaf's old ref is found here (its parent's old node, one child read)
 // instead of in a separate top-down pass over the leaves: this kernel is
@@ -1177,15 +1212,6 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_up_kernel(TailArgs a) 
 }
 
 // Old refs of heights h_hi down to h_lo.
-__global__ __launch_bounds__(scan::kFusedBlock) void tail_topdown_kernel(TailArgs a) {
-  for (uint32_t h = a.h_hi + 1; h-- > a.h_lo;) {
-    Scenes sh = a.sc;
-    sh.keys = a.hs.keys[h];
-    const uint32_t n = tail_count(a, h);
-    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) topdown_entry(sh, a.hs.parent[h], a.hs.old[h + 1], a.hs.old[h], h, j);
-    __syncthreads();
-  }
-}
 
 // Node items of heights h_lo..h_hi: build, probe, commit, place. Each entry
 // stays with its thread from build to place (its item, hash and result in
@@ -1479,20 +1505,28 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
   }
   if ((st = last_launch()) != VX_OK) return st;
   mark(g, 3, s);
-  if (hu <= H) {
-    TimedLaunch tl(VX_K_TAIL_TOPDOWN, s);
-    tail_topdown_kernel<<<1, scan::kFusedBlock, 0, s>>>(ta);
+  // Heights 2..H walk from the root in one launch; height 1 (millions of
+  // entries) reads its parent's old node; the leaves' old refs are read by
+  // the leaf build.
+  if (H >= 2) {
+    OldRefHeights oh{};
+    uint64_t walkers = 0;
+    for (uint32_t h = 2; h <= H; ++h) {
+      oh.keys[h] = L.keys[h];
+      oh.old[h] = L.old[h];
+      walkers += L.cap[h];
+    }
+    TimedLaunch tl(VX_K_TOPDOWN, s);
+    old_refs_kernel<<<grid_for(walkers), 256, 0, s>>>(sc, cnt, oh, 2, H);
   }
   if ((st = last_launch()) != VX_OK) return st;
-  for (uint32_t h = hu; h-- > 1;) {  // the leaves' old refs are read by the leaf build
+  if (H >= 1) {
     Scenes sh = sc;
-    sh.keys = L.keys[h];
-    {
-      TimedLaunch tl(VX_K_TOPDOWN, s);
-      topdown_kernel<<<grid_for(L.cap[h]), 256, 0, s>>>(sh, &cnt[1 + h], L.parent[h], L.old[h + 1], L.old[h], h);
-    }
-    if ((st = last_launch()) != VX_OK) return st;
+    sh.keys = L.keys[1];
+    TimedLaunch tl(VX_K_TOPDOWN, s);
+    topdown_kernel<<<grid_for(L.cap[1]), 256, 0, s>>>(sh, &cnt[1 + 1], L.parent[1], L.old[2], L.old[1], 1);
   }
+  if ((st = last_launch()) != VX_OK) return st;
   mark(g, 4, s);
   {
     Scenes sh = sc;
