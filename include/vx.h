@@ -235,13 +235,15 @@ vx_status vx_dag_build_terrain(vx_dag* dag, const vx_terrain* terrain, void* str
 /* Drop-in for apply_voxel_edits_workspace_size / apply_voxel_edits
  * (edit.cuh:16-41): op 0 = Place, 1 = Destroy; material is the placed
  * material for a materials DAG. Out-of-world voxels are ignored; count 0 is a
- * no-op; NULL edits with count > 0 or a short workspace is VX_INVALID_VALUE. */
+ * no-op; NULL edits with count > 0, count >= 2^29 or a short workspace is
+ * VX_INVALID_VALUE. */
 vx_status vx_dag_voxel_edits_workspace_size(const vx_dag* dag, uint32_t count, size_t* bytes);
 vx_status vx_dag_apply_voxel_edits(vx_dag* dag, const vx_voxel* d_edits, uint32_t count, uint32_t op,
                                    uint32_t material, void* ws, size_t ws_bytes, void* stream);
 
 /* Brush edit batch; h_brushes is HOST memory (it is small and the host needs
- * it to size the work). */
+ * it to size the work). A batch (and a batch set, and a batcher's capacity)
+ * spans fewer than 2^29 candidate leaf bricks (VX_INVALID_VALUE otherwise). */
 vx_status vx_dag_brush_workspace_size(const vx_dag* dag, const vx_brush* h_brushes, uint32_t n, size_t* bytes);
 vx_status vx_dag_apply_brushes(vx_dag* dag, const vx_brush* h_brushes, uint32_t n, void* ws, size_t ws_bytes,
                                void* stream);

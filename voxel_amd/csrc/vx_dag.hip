@@ -52,6 +52,16 @@ namespace dag {
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMaxHeights = 16;
 
+// Level links: an entry's parent index (at the height above) and its child
+// slot in the parent (the low 3 bits of its key), packed as index << 3 | slot
+// so that the passes that follow a link read one word instead of the link
+// and the 8-byte key; kNone for a root. Entries per height stay below
+// kMaxEntries (the record count bounds them).
+constexpr uint32_t kMaxEntries = 1u << 29;
+__device__ __forceinline__ uint32_t link_make(uint32_t parent, uint64_t key) { return (parent << 3) | uint32_t(key & 7u); }
+__device__ __forceinline__ uint32_t link_parent(uint32_t link) { return link >> 3; }
+__device__ __forceinline__ uint32_t link_slot(uint32_t link) { return link & 7u; }
+
 // Composite keys: scene id above `shift` bits of (leaf Morton >> 3h).
 struct Codec {
   uint32_t shift;
@@ -781,7 +791,7 @@ struct RunUpStore {
       const uint32_t l = leaves - 1;
       keys0[l] = p.key;
       first0[l] = i;
-      parent0[l] = p.inc ? parents - 1 : kNone;
+      parent0[l] = p.inc ? link_make(parents - 1, p.key) : kNone;
       if (!p.inc) root_pos[codec.scene(p.key)] = l;
       if (PS::hi(v)) {
         keys1[PS::hi(excl)] = codec.parent(p.key);
@@ -828,7 +838,7 @@ struct UpStore {
   uint32_t* root_pos;  // per scene: index of its root entry at its root height
   __device__ void operator()(uint32_t i, const UpPayload& p, uint32_t f, uint32_t excl, uint32_t incl,
                              uint32_t n) const {
-    parent[i] = p.inc ? incl - 1 : kNone;
+    parent[i] = p.inc ? link_make(incl - 1, p.key) : kNone;
     if (!p.inc) root_pos[p.scene] = i;
     if (f) {
       up_keys[excl] = codec.parent(p.key);
@@ -847,7 +857,8 @@ __device__ __forceinline__ void topdown_entry(const Scenes& sc, const uint32_t* 
                                               const uint32_t* __restrict__ old_up, uint32_t* __restrict__ old,
                                               uint32_t h, uint32_t j) {
   const Dev& d = sc.of(j);
-  old[j] = d.depth == h ? d.ctl[kCtlRoot] : child_of(d.node_pool, old_up[parent[j]], uint32_t(sc.keys[j] & 7u));
+  const uint32_t link = parent[j];
+  old[j] = d.depth == h ? d.ctl[kCtlRoot] : child_of(d.node_pool, old_up[link_parent(link)], link_slot(link));
 }
 
 __global__ void topdown_kernel(Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ parent,
@@ -947,7 +958,8 @@ __global__ __launch_bounds__(kLeafBlock, 8) void leaf_build_kernel(
     if (j < n) {
       const Dev& d = sc.of(j);
       const uint32_t ref =
-          d.depth == 0 ? d.ctl[kCtlRoot] : child_of(d.node_pool, old_up[parent[j]], uint32_t(sc.keys[j] & 7u));
+          d.depth == 0 ? d.ctl[kCtlRoot]
+                       : child_of(d.node_pool, old_up[link_parent(parent[j])], link_slot(parent[j]));
       uint32_t w[LW];
       if (ref == kEmpty) {
 #pragma unroll
@@ -1065,7 +1077,7 @@ constexpr uint32_t kBuildBlock = 256;
 
 __global__ __launch_bounds__(kBuildBlock) void node_build_kernel(
     Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ old, const uint32_t* __restrict__ first,
-    const uint64_t* __restrict__ ckeys, const uint32_t* __restrict__ cparent, const uint64_t* __restrict__ cres,
+    const uint32_t* __restrict__ cparent, const uint64_t* __restrict__ cres,
     uint32_t* __restrict__ items, uint64_t* __restrict__ hashes, uint64_t* __restrict__ res) {
   __shared__ uint32_t s_ch[kBuildBlock * 8];
   const uint32_t n = *n_ptr;
@@ -1077,8 +1089,8 @@ __global__ __launch_bounds__(kBuildBlock) void node_build_kernel(
     __syncthreads();
     const uint32_t k1 = first[jn];
     for (uint32_t k = first[j0] + t; k < k1; k += kBuildBlock) {
-      const uint32_t pj = cparent[k];
-      if (pj >= j0 && pj < jn) s_ch[uint32_t(ckeys[k] & 7u) * kBuildBlock + (pj - j0)] = resolve(cres, k);
+      const uint32_t link = cparent[k], pj = link_parent(link);
+      if (pj >= j0 && pj < jn) s_ch[link_slot(link) * kBuildBlock + (pj - j0)] = resolve(cres, k);
     }
     __syncthreads();
     const uint32_t j = j0 + t;
@@ -1254,10 +1266,11 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
       children_of(sh.of(t).node_pool, a.hs.old[h][t], ch);
       const uint32_t k1 = a.hs.first[h][t + 1];
       for (uint32_t k = a.hs.first[h][t]; k < k1; ++k) {
-        if (a.hs.parent[h - 1][k] != t) continue;
+        const uint32_t link = a.hs.parent[h - 1][k];
+        if (link_parent(link) != t) continue;
         uint64_t cr = lds_children ? cres_lds[k] : cres_glb[k];
         if (res_kind(cr) == kResDup) cr = lds_children ? cres_lds[uint32_t(cr)] : cres_glb[uint32_t(cr)];
-        ch[uint32_t(a.hs.keys[h - 1][k] & 7u)] = uint32_t(cr);
+        ch[link_slot(link)] = uint32_t(cr);
       }
       uint32_t ref = 0;
       w = make_node_item(ch, it, &ref);
@@ -1573,8 +1586,8 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
     {
       TimedLaunch tl(VX_K_NODE_BUILD, s);
       node_build_kernel<<<grid_for(L.cap[h]), kBuildBlock, 0, s>>>(sh, &cnt[1 + h], L.old[h], L.first[h],
-                                                                   L.keys[h - 1], L.parent[h - 1], L.res[h - 1],
-                                                                   w.items, w.hashes, L.res[h]);
+                                                                   L.parent[h - 1], L.res[h - 1], w.items, w.hashes,
+                                                                   L.res[h]);
     }
     if ((st = last_launch()) != VX_OK) return st;
     if ((st = intern_scenes(sh, p.n_scenes, false, ItemSet{w.items, kNodeStride, 0}, w.hashes, &cnt[1 + h], L.cap[h],
@@ -1926,7 +1939,7 @@ extern "C" vx_status vx_dag_apply_voxel_edits(vx_dag* g, const vx_voxel* edits, 
                                               uint32_t material, void* ws, size_t ws_bytes, void* stream) {
   if (!g || (!edits && count) || op > 1) return VX_INVALID_VALUE;  // edit.cuh:28-37
   if (count == 0) return VX_OK;
-  if (count >= (1u << 30)) return VX_INVALID_VALUE;
+  if (count >= kMaxEntries) return VX_INVALID_VALUE;
   const Plan p = make_plan({g});
   EditWs w = carve(ws, p, count, 0, count);
   if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
@@ -1950,7 +1963,7 @@ extern "C" vx_status vx_dag_apply_brushes(vx_dag* g, const vx_brush* b, uint32_t
   std::vector<BrushJob> jobs;
   const uint64_t cand = make_jobs(b, n, 4u << g->depth, 0, 0, &jobs);
   if (cand == 0) return VX_OK;
-  if (cand >= (1u << 30)) return VX_INVALID_VALUE;
+  if (cand >= kMaxEntries) return VX_INVALID_VALUE;
   const Plan p = make_plan({g});
   EditWs w = carve(ws, p, cand, uint32_t(jobs.size()), cand);
   if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
@@ -2003,7 +2016,7 @@ extern "C" vx_status vx_dag_apply_batches(vx_dag* const* dags, const vx_brush* c
   vx_status st = batch_setup(dags, brushes, counts, n_batches, &scenes, &jobs, &cand);
   if (st != VX_OK) return st;
   if (cand == 0) return VX_OK;
-  if (cand >= (1u << 30) || scenes.size() > (1u << 20)) return VX_INVALID_VALUE;
+  if (cand >= kMaxEntries || scenes.size() > (1u << 20)) return VX_INVALID_VALUE;
   const Plan p = make_plan(std::vector<const vx_dag*>(scenes.begin(), scenes.end()));
   if (p.key_bits > 64) return VX_INVALID_VALUE;
   EditWs w = carve(ws, p, cand, uint32_t(jobs.size()), cand);
@@ -2226,7 +2239,7 @@ extern "C" vx_status vx_batcher_destroy(vx_batcher* b) {
 
 extern "C" vx_status vx_batcher_create(vx_dag* const* scenes, uint32_t n_scenes, uint64_t max_candidates,
                                        uint32_t max_jobs, int32_t use_graph, vx_batcher** out) {
-  if (!scenes || !n_scenes || !out || !max_candidates || !max_jobs || max_candidates >= (1u << 30))
+  if (!scenes || !n_scenes || !out || !max_candidates || !max_jobs || max_candidates >= kMaxEntries)
     return VX_INVALID_VALUE;
   auto* b = new (std::nothrow) vx_batcher();
   if (!b) return VX_DEVICE_ERROR;
