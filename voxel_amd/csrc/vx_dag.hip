@@ -82,6 +82,12 @@ struct Scenes {
   __device__ __forceinline__ uint32_t id(uint32_t i) const { return table ? codec.scene(keys[i]) : 0u; }
   __device__ __forceinline__ const Dev& of(uint32_t i) const { return table ? table[codec.scene(keys[i])] : single; }
   __device__ __forceinline__ const Dev& at(uint32_t s) const { return table ? table[s] : single; }
+  // The same with the level's keys given (no copy of the Scenes, whose
+  // embedded Dev would otherwise go to scratch once a reference to it is taken).
+  __device__ __forceinline__ uint32_t id(const uint64_t* k, uint32_t i) const { return table ? codec.scene(k[i]) : 0u; }
+  __device__ __forceinline__ const Dev& of(const uint64_t* k, uint32_t i) const {
+    return table ? table[codec.scene(k[i])] : single;
+  }
 };
 
 // ---- interning --------------------------------------------------------------
@@ -893,10 +899,9 @@ __global__ void old_refs_kernel(Scenes sc, const uint32_t* __restrict__ counts, 
     uint32_t h = h_lo;
     while (g >= start[h + 1]) ++h;
     const uint32_t j = g - start[h];
-    Scenes sh = sc;
-    sh.keys = hs.keys[h];
-    const Dev& d = sh.of(j);
-    const uint64_t m = hs.keys[h][j] & mask;
+    const uint64_t* keys = hs.keys[h];
+    const Dev& d = sc.of(keys, j);
+    const uint64_t m = keys[j] & mask;
     uint32_t ref = d.ctl[kCtlRoot];
     for (uint32_t l = d.depth - h; l-- > 0 && ref > kFull;) ref = child_of(d.node_pool, ref, uint32_t(m >> (3 * l)) & 7u);
     hs.old[h][j] = ref;
@@ -1250,8 +1255,8 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
   __shared__ uint64_t s_res[2][kTailCap];
   const uint32_t t = threadIdx.x;
   for (uint32_t h = a.h_lo; h <= a.h_hi; ++h) {
-    Scenes sh = a.sc;
-    sh.keys = a.hs.keys[h];
+    const Scenes& sh = a.sc;
+    const uint64_t* keys = a.hs.keys[h];
     const uint32_t n = tail_count(a, h);
     const bool act = t < n;
     const bool lds_children = h > a.h_lo;  // the height below was built here
@@ -1263,21 +1268,24 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
     uint64_t r = 0, hash = 0;
     if (act) {
       uint32_t ch[8];
-      children_of(sh.of(t).node_pool, a.hs.old[h][t], ch);
+      children_of(sh.of(keys, t).node_pool, a.hs.old[h][t], ch);
       const uint32_t k1 = a.hs.first[h][t + 1];
       for (uint32_t k = a.hs.first[h][t]; k < k1; ++k) {
         const uint32_t link = a.hs.parent[h - 1][k];
         if (link_parent(link) != t) continue;
         uint64_t cr = lds_children ? cres_lds[k] : cres_glb[k];
         if (res_kind(cr) == kResDup) cr = lds_children ? cres_lds[uint32_t(cr)] : cres_glb[uint32_t(cr)];
-        ch[link_slot(link)] = uint32_t(cr);
+        const uint32_t slot = link_slot(link);
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c)
+          if (c == slot) ch[c] = uint32_t(cr);  // constant indices: ch stays in registers
       }
       uint32_t ref = 0;
-      w = make_node_item(ch, it, &ref);
+      w = make_node_item_regs(ch, it, &ref);
       if (!w) {
         r = res_make(kResFinal, ref);
       } else {
-        hash = hash_words(it, w);
+        hash = hash_words_regs(it, w);
         r = res_make(kResPending, 0);
       }
 #pragma unroll
@@ -1292,7 +1300,7 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
     // an empty slot (the same protocol as probe_entry, with the item held in
     // registers and this height's tagged items read from LDS).
     if (act && res_kind(r) == kResPending) {
-      const Dev& d = sh.of(t);
+      const Dev& d = sh.of(keys, t);
       const Table tb = table_of(d, false);
       const uint32_t fp = uint32_t(hash >> 32);
       uint64_t idx = hash & tb.mask;
@@ -1316,8 +1324,9 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
           const uint32_t v = uint32_t(sv);
           if (v & kTag) {
             const uint32_t j = v & ~kTag;
-            uint32_t diff = 0;
-            for (uint32_t q = 0; q < w; ++q) diff |= it[q] ^ s_item[j * kNodeStride + q];
+            uint32_t diff = 0;  // words past the width are zero in both
+#pragma unroll
+            for (uint32_t q = 0; q < kNodeStride; ++q) diff |= it[q] ^ s_item[j * kNodeStride + q];
             if (!diff) {
               out = res_make(kResDup, j);
               done = true;
@@ -1340,13 +1349,13 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
     // Commit: pool offsets of the representatives (a segmented sum of their
     // widths per scene), then place: copy, publish the slot, become final.
     const uint32_t wr = (act && res_kind(r) == kResRep) ? w : 0u;
-    const uint32_t id = act ? sh.id(t) : 0u;
-    const bool head = act && (t == 0 || sh.id(t - 1) != id);
+    const uint32_t id = act ? sh.id(keys, t) : 0u;
+    const bool head = act && (t == 0 || sh.id(keys, t - 1) != id);
     uint32_t incl = 0;
     scan::fused_tile<scan::SegSumOp, 1>(0u, n, false, TailRegLoad{scan::SegSumOp::make(wr, head)},
                                         TailRegStore{&incl}, nullptr, s_wave, &s_prefix);
     if (act) {
-      const Dev& d = sh.of(t);
+      const Dev& d = sh.of(keys, t);
       const Table tb = table_of(d, false);
       const uint32_t base = *tb.base;
       if (wr) {
@@ -1362,7 +1371,7 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
           r = res_make(kResFinal, uint32_t(off));
         }
       }
-      if (t == n - 1 || sh.id(t + 1) != id) {
+      if (t == n - 1 || sh.id(keys, t + 1) != id) {
         const uint64_t used = uint64_t(base) + incl;
         *tb.used = used > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(used);
       }

@@ -118,9 +118,48 @@ __device__ __forceinline__ uint32_t make_node_item(const uint32_t (&ch)[8], uint
   return w;
 }
 
+// make_node_item into a register array: every index is a compile-time
+// constant (word q of the item is the child c whose rank among the table-
+// backed children is q - 1), so the array stays in registers instead of
+// going to scratch. Words past the width are zero.
+__device__ __forceinline__ uint32_t make_node_item_regs(const uint32_t (&ch)[8], uint32_t (&item)[9], uint32_t* ref) {
+  uint32_t cm = 0, fm = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (ch[c] == kFull) fm |= 1u << c;
+    else if (ch[c] != kEmpty) cm |= 1u << c;
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) item[q] = 0;
+  if (cm == 0 && (fm == 0 || fm == 0xffu)) {
+    *ref = fm ? kFull : kEmpty;
+    return 0;
+  }
+  item[0] = cm | (fm << 8);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t rank = 1 + __popc(cm & ((1u << c) - 1));
+    const bool on = (cm >> c) & 1u;
+#pragma unroll
+    for (int q = 1; q < 9; ++q)
+      if (on && rank == uint32_t(q)) item[q] = ch[c];
+  }
+  return 1 + __popc(cm);
+}
+
 __device__ __forceinline__ uint64_t hash_words(const uint32_t* w, uint32_t n) {
   uint64_t h = 0x243F6A8885A308D3ull ^ n;
   for (uint32_t k = 0; k < n; ++k) h = mix64(h ^ w[k]);
+  return h;
+}
+
+// hash_words of a register array of up to kN words (constant indices).
+template <uint32_t kN>
+__device__ __forceinline__ uint64_t hash_words_regs(const uint32_t (&w)[kN], uint32_t n) {
+  uint64_t h = 0x243F6A8885A308D3ull ^ n;
+#pragma unroll
+  for (uint32_t k = 0; k < kN; ++k)
+    if (k < n) h = mix64(h ^ w[k]);
   return h;
 }
 

@@ -24,31 +24,38 @@ struct Ray {
   uint64_t k[3];
 };
 
-// Whether crossing (a, ja) comes before crossing (b, jb).
-__device__ __forceinline__ bool before(const Ray& r, int a, uint64_t ja, int b, uint64_t jb) {
-  const uint64_t lhs = (2 * ja + 1) * r.ad[b], rhs = (2 * jb + 1) * r.ad[a];
+// Whether crossing (a, ja) of an axis with |d_a| = da comes before crossing
+// (b, jb) of an axis with |d_b| = db.
+__device__ __forceinline__ bool before(int a, uint64_t ja, uint64_t da, int b, uint64_t jb, uint64_t db) {
+  const uint64_t lhs = (2 * ja + 1) * db, rhs = (2 * jb + 1) * da;
   return lhs < rhs || (lhs == rhs && a < b);
 }
 
 // Leave the aligned cube [lo, lo + side) that holds the current voxel: returns
-// false if the ray cannot move (zero direction).
+// false if the ray cannot move (zero direction). The winning axis travels as
+// values (its index, crossing and |d|), so every array index is a constant and
+// the ray stays in registers.
 __device__ __forceinline__ bool exit_cube(Ray& r, const int64_t (&lo)[3], int64_t side) {
   int best = -1;
-  uint64_t jbest = 0;
+  uint64_t jbest = 0, Y = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (!r.ad[a]) continue;
     const int64_t j = r.sgn[a] > 0 ? lo[a] + side - 1 - r.o[a] : r.o[a] - lo[a];
-    if (best < 0 || before(r, a, uint64_t(j), best, jbest)) {
+    if (best < 0 || before(a, uint64_t(j), r.ad[a], best, jbest, Y)) {
       best = a;
       jbest = uint64_t(j);
+      Y = r.ad[a];
     }
   }
   if (best < 0) return false;
-  const uint64_t Y = r.ad[best];
 #pragma unroll
   for (int b = 0; b < 3; ++b) {
-    if (b == best || !r.ad[b]) continue;
+    if (b == best) {
+      r.k[b] = jbest + 1;
+      continue;
+    }
+    if (!r.ad[b]) continue;
     const uint64_t X = (2 * jbest + 1) * r.ad[b];
     // crossings (b, j) before (best, jbest): (2j+1)Y < X, or == X when b < best
     uint64_t cnt;
@@ -56,7 +63,6 @@ __device__ __forceinline__ bool exit_cube(Ray& r, const int64_t (&lo)[3], int64_
     else cnt = X > Y ? (X - Y - 1) / (2 * Y) + 1 : 0;
     if (cnt > r.k[b]) r.k[b] = cnt;
   }
-  r.k[best] = jbest + 1;
   return true;
 }
 
