@@ -588,7 +588,8 @@ def _kind_of(name, last_probe):
              ("hist_scan_kernel", "sort_scan"), ("partition_kernel", "sort_partition"), ("copy_kernel", "sort_copy"),
              ("RunUpLoad", "runs"), ("UpLoad", "up"), ("topdown_kernel", "topdown"), ("leaf_build_kernel", "leaf_build"),
              ("node_build_kernel", "node_build"), ("publish_kernel", "publish"), ("fill_u32_kernel", "fill"),
-             ("begin_kernel", "other")]
+             ("begin_kernel", "other"), ("old_refs_kernel", "topdown"), ("probe_slices_kernel", "leaf_probe"),
+             ("SliceLoad", "leaf_probe")]
     for key, kind in (("tail_up_kernel", "tail_up"), ("tail_topdown_kernel", "tail_topdown"),
                       ("tail_nodes_kernel", "tail_nodes")):
         if key in name:
@@ -597,8 +598,8 @@ def _kind_of(name, last_probe):
         return "node_probe" if "<9u>" in name else "leaf_probe"
     if "place_kernel" in name:
         return "node_place" if "<9u>" in name else "leaf_place"
-    if "CommitLoad" in name:
-        return "node_commit" if last_probe == "node_probe" else "leaf_commit"
+    if "CommitLoad" in name:  # the leaf commit scans 8 items per thread, the node heights 4
+        return "leaf_commit" if ", 8u," in name else "node_commit"
     for key, kind in table:
         if key in name:
             return kind
