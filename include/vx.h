@@ -449,7 +449,8 @@ enum {
   VX_K_FILL,
   VX_K_OTHER,
   VX_K_TAIL_UP,      /* single-workgroup kernels for the small top heights */
-  VX_K_TAIL_TOPDOWN,
+  VX_K_TAIL_TOPDOWN, /* not launched any more: every height's old refs are VX_K_TOPDOWN (kept so the
+                        * kinds after it keep their values) */
   VX_K_TAIL_NODES,
   VX_DAG_KERNELS
 };
