@@ -343,6 +343,30 @@ def test_kernel_profile_counts_every_launch(gpu):
     assert all(v > 0 for k, v in ms.items() if launches[k])
 
 
+def test_kernel_sampling_without_stage_events(gpu):
+    """Profiling without stage events records no stage times, and a kernel
+    period of P times one launch per edit of a kind launched P times per
+    edit, rotating over the launches (P edits time each launch once)."""
+    from voxel_amd.dag import KERNELS, STREAM_BRUSH, gen_brushes, terrain_for_world
+    d = _dag(7, True, node_words=1 << 24, leaf_words=1 << 25)
+    d.build_terrain(terrain_for_world(512, 1))
+    d.profile(True)
+    d.apply_brushes(gen_brushes(1, STREAM_BRUSH, 64, 512, 0))
+    passes = d.profile_read().sort_passes
+    assert passes > 1
+    d.profile(True, stages=False)
+    d.profile_kernels(["sort_partition"])
+    d.profile_period(passes)
+    for k in range(passes):
+        d.apply_brushes(gen_brushes(1, STREAM_BRUSH + ((k + 1) << 8), 64, 512, 0))
+    p = d.profile_read()
+    i = KERNELS.index("sort_partition")
+    assert p.edits == passes
+    assert list(p.ms) == [0.0] * len(p.ms)
+    assert p.kernel_launches[i] == passes
+    assert sum(p.kernel_launches) == passes and p.kernel_ms[i] > 0
+
+
 def test_pair_sort_layout_matches_packed(gpu, monkeypatch):
     """The (key, index) pair sort, used when a packed key would not fit 64
     bits, gives the same scene as the packed-key sort."""
