@@ -33,6 +33,7 @@
 // Status: first error wins per scene (detail/edit_common.cuh:96-99); a scene's
 // root is published only if that scene saw no error (detail/edit/root.cuh:13-25).
 #include <cmath>
+#include <type_traits>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -423,7 +424,8 @@ struct alignas(16) BrushJob {
   uint32_t start;
   uint32_t scene;
   float inv_nx, inv_ny;  // 1/nx, 1/ny rounded to nearest (see divmod_small)
-  uint32_t pad[2];
+  uint32_t row_start;    // first brick row (ny * nz rows per job)
+  uint32_t pad;
 };
 static_assert(sizeof(BrushJob) == 80, "BrushJob layout");
 
@@ -652,7 +654,9 @@ struct BrushCands {
   // loads) for the lowest active lane, then each lane steps forward over the
   // few jobs that start inside its wave's window. With a hint (the job of an
   // earlier candidate of the same lane) the search is skipped.
+  template <bool ROWS = false>
   __device__ __forceinline__ uint32_t job_of(uint32_t i, uint32_t nj, uint32_t hint) const {
+    auto first = [&](uint32_t k) { return ROWS ? jobs[k].row_start : jobs[k].start; };
     uint32_t lo = hint;
     if (hint == kNone) {
       const uint32_t i0 = __builtin_amdgcn_readfirstlane(i);
@@ -660,11 +664,11 @@ struct BrushCands {
       lo = 0;
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].start <= i0) lo = mid;
+        if (first(mid) <= i0) lo = mid;
         else hi = mid;
       }
     }
-    while (lo + 1 < nj && jobs[lo + 1].start <= i) ++lo;
+    while (lo + 1 < nj && first(lo + 1) <= i) ++lo;
     return lo;
   }
   __device__ __forceinline__ void brick_of(const BrushJob& j, uint32_t i, uint32_t& gx, uint32_t& gy,
@@ -725,6 +729,143 @@ struct BrushStore {
     if (i == n - 1) *out.n_rec = incl;
   }
 };
+
+// ---- brush records by brick rows ---------------------------------------------
+// A job's clipped brick box is walked by rows (fixed brick y and z, the box's
+// x run). The touched bricks of a row are one run: the whole row for a box;
+// for a sphere the bricks whose voxel nearest to the centre lies inside, an
+// interval in closed form (integer square root) with the same test as
+// brick_touched. A scan over the rows counts them and drops empty rows; then
+// one thread per record writes it, in the same order as a walk over every
+// brick of the box (rows in order, x within a row). Only touched bricks are
+// enumerated: about 37% of a sphere batch's box bricks are not.
+struct RowRef {
+  uint32_t job;
+  uint32_t yz;  // by | bz << 16 (job brick indices)
+  uint32_t x0;  // first touched brick of the row (job brick index)
+};
+
+// The touched run [x0, x1) of row (by, bz) of job j, in job brick indices.
+__device__ __forceinline__ void row_run(const BrushJob& j, uint32_t by, uint32_t bz, uint32_t& x0, uint32_t& x1) {
+  const vx_brush& b = j.b;
+  x0 = 0;
+  x1 = j.nx;
+  if (b.kind != 0) return;  // every brick of a box's clipped range is touched
+  const int64_t gy = int64_t(j.by0) + by, gz = int64_t(j.bz0) + bz;
+  const int64_t dy = max(int64_t(0), max(4 * gy - b.cy, int64_t(b.cy) - (4 * gy + 3)));
+  const int64_t dz = max(int64_t(0), max(4 * gz - b.cz, int64_t(b.cz) - (4 * gz + 3)));
+  const int64_t r2 = int64_t(b.hx) * b.hx - dy * dy - dz * dz;
+  if (r2 < 0) {
+    x1 = 0;
+    return;
+  }
+  int64_t q = int64_t(sqrt(double(r2)));  // floor(sqrt(r2)), corrected to exact
+  while (q * q > r2) --q;
+  while ((q + 1) * (q + 1) <= r2) ++q;
+  // Bricks g with voxels [4g, 4g + 3] meeting [cx - q, cx + q].
+  auto floor4 = [](int64_t a) { return a >= 0 ? a / 4 : -((3 - a) / 4); };
+  const int64_t lo = floor4(int64_t(b.cx) - q) - j.bx0, hi = floor4(int64_t(b.cx) + q) - j.bx0 + 1;
+  const int64_t nx = j.nx;
+  x0 = uint32_t(min(max(lo, int64_t(0)), nx));
+  x1 = uint32_t(min(max(hi, int64_t(x0)), nx));
+}
+
+struct RowLoad {
+  using Payload = RowRef;
+  BrushCands c;
+  uint32_t hint = kNone;
+  uint32_t nj = 0;
+  __device__ uint64_t operator()(uint32_t r, RowRef& p) {
+    if (hint == kNone) nj = *c.n_jobs;
+    p.job = c.job_of<true>(r, nj, hint);
+    hint = p.job;
+    const BrushJob j = c.jobs[p.job];
+    uint32_t by, bz;
+    divmod_small(r - j.row_start, j.ny, j.inv_ny, bz, by);
+    uint32_t x0, x1;
+    row_run(j, by, bz, x0, x1);
+    p.yz = by | (bz << 16);
+    p.x0 = x0;
+    return scan::PairSumOp::make(x1 > x0 ? 1u : 0u, x1 - x0);
+  }
+};
+
+// The touched rows, compacted: record start, job, brick y|z, first brick.
+struct TouchedRows {
+  uint32_t* rec;
+  uint32_t* job;
+  uint32_t* yz;
+  uint32_t* x0;
+  uint32_t* n_rows;
+};
+
+struct RowStore {
+  TouchedRows rows;
+  uint32_t* n_rec;
+  __device__ void operator()(uint32_t r, const RowRef& p, uint64_t v, uint64_t excl, uint64_t incl,
+                             uint32_t n) const {
+    using PS = scan::PairSumOp;
+    if (PS::lo(v)) {
+      const uint32_t k = PS::lo(excl);
+      rows.rec[k] = PS::hi(excl);
+      rows.job[k] = p.job;
+      rows.yz[k] = p.yz;
+      rows.x0[k] = p.x0;
+    }
+    if (r == n - 1) {
+      *rows.n_rows = PS::lo(incl);
+      *n_rec = PS::hi(incl);
+    }
+  }
+};
+
+// One thread per record. A wave takes a contiguous chunk of records; 64
+// consecutive records span at most 64 rows (no row is empty), so each step
+// loads the 64 row starts from the wave's current row in one coalesced load
+// and every lane finds its row among them (a 6-step search over the lanes).
+__global__ void brush_records_kernel(BrushCands c, TouchedRows rows, RecordOut out) {
+  const uint32_t n = *out.n_rec, nr = *rows.n_rows;
+  const uint32_t lane = lane_id();
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const uint32_t w = blockIdx.x * (blockDim.x / kWave) + wave_id();
+  const uint32_t chunk = (n + waves * kWave - 1) / (waves * kWave) * kWave;
+  const uint32_t r0 = w * chunk, r1 = min(n, r0 + chunk);
+  if (r0 >= r1) return;
+  // The row of the chunk's first record: the last row starting at or before it.
+  uint32_t row = 0;
+  {
+    uint32_t lo = 0, hi = nr;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (rows.rec[mid] <= r0) lo = mid;
+      else hi = mid;
+    }
+    row = lo;
+  }
+  for (uint32_t i0 = r0; i0 < r1; i0 += kWave) {
+    const uint32_t i = i0 + lane;
+    const uint32_t start = row + lane < nr ? rows.rec[row + lane] : 0xffffffffu;  // ascending over the lanes
+    // k = the number of lanes whose row starts at or before record i, minus one.
+    uint32_t k = 0;
+#pragma unroll
+    for (uint32_t step = 32; step >= 1; step >>= 1) {
+      const uint32_t probe = uint32_t(__shfl(start, int(k + step), kWave));
+      if (k + step < kWave && probe <= i) k += step;
+    }
+    const uint32_t my_row = row + k;
+    const uint32_t my_start = uint32_t(__shfl(start, int(k), kWave));
+    if (i < r1) {
+      const BrushJob j = c.jobs[rows.job[my_row]];
+      const uint32_t yz = rows.yz[my_row];
+      const uint32_t gx = j.bx0 + rows.x0[my_row] + (i - my_start);
+      const uint32_t gy = j.by0 + (yz & 0xffffu), gz = j.bz0 + (yz >> 16);
+      out.put_key(i, (uint64_t(j.scene) << c.codec.shift) | morton3(gx, gy, gz));
+      out.mask[i] = brick_mask(j.b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4));
+      out.mat[i] = c.materials ? j.b.material : (j.b.material ? 1u : 0u);
+    }
+    row += uint32_t(__builtin_amdgcn_readlane(int(k), kWave - 1));
+  }
+}
 
 struct VoxelLoad {
   using Payload = Record;
@@ -1158,7 +1299,7 @@ struct Heights {
   uint64_t cap[kMaxHeights + 1];
 };
 
-constexpr uint32_t kCntSlices = 59, kCntPending = 60;
+constexpr uint32_t kCntRows = 58, kCntSlices = 59, kCntPending = 60;
 
 struct EditWs {
   uint32_t* counts;  // [0] records, [1 + h] entries at height h, [kCntPending] pending leaves
@@ -1179,6 +1320,7 @@ struct EditWs {
   void* up_ws[kMaxHeights + 1];
   void* commit_ws[kMaxHeights + 1];
   uint32_t *pend, *pend_list, *slice_count, *slice_base;  // pending leaves (LeafSlices)
+  uint32_t *row_rec, *row_job, *row_yz, *row_x0;          // touched brick rows (TouchedRows)
   void* slice_ws;
   BrushJob* jobs;
   Dev* scenes;
@@ -1449,6 +1591,12 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
   const uint64_t slices = (w.hs.cap[0] + kWave - 1) / kWave;
   w.pend = c.take<uint32_t>(slices * kWave);
   w.pend_list = c.take<uint32_t>(w.hs.cap[0]);
+  // Touched brick rows of a brush batch: at most one per candidate brick.
+  const uint64_t rows_ub = std::max<uint64_t>(candidates, 1);
+  w.row_rec = c.take<uint32_t>(rows_ub);
+  w.row_job = c.take<uint32_t>(rows_ub);
+  w.row_yz = c.take<uint32_t>(rows_ub);
+  w.row_x0 = c.take<uint32_t>(rows_ub);
   w.slice_count = c.take<uint32_t>(slices);
   w.slice_base = c.take<uint32_t>(slices);
   w.hashes = c.take<uint64_t>(max_cap);
@@ -1621,8 +1769,10 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
 }
 
 // Brush jobs: brick bounding boxes clipped to the world; empty ones dropped.
+// Brush jobs; *rows (may be null) is the running count of brick rows (each
+// job's rows start there).
 uint64_t make_jobs(const vx_brush* b, uint32_t n, uint32_t world, uint32_t scene, uint64_t start,
-                   std::vector<BrushJob>* jobs) {
+                   std::vector<BrushJob>* jobs, uint64_t* rows = nullptr) {
   uint64_t total = start;
   for (uint32_t i = 0; i < n; ++i) {
     const vx_brush& br = b[i];
@@ -1650,6 +1800,10 @@ uint64_t make_jobs(const vx_brush* b, uint32_t n, uint32_t world, uint32_t scene
     j.scene = scene;
     j.inv_nx = float(1.0 / double(j.nx));
     j.inv_ny = float(1.0 / double(j.ny));
+    if (rows) {
+      j.row_start = uint32_t(*rows);
+      *rows += uint64_t(cnt[1]) * cnt[2];
+    }
     total += uint64_t(cnt[0]) * cnt[1] * cnt[2];
     if (jobs) jobs->push_back(j);
   }
@@ -1769,7 +1923,20 @@ vx_status device_pipeline(vx_dag* prof_dag, const Plan& p, const Scenes& sc, Edi
   mark(prof_dag, 0, s);
   {
     TimedLaunch tl(VX_K_RECORDS, s);
-    st = scan::fused_dev<scan::kSum>(&w.counts[kCntCandidates], cand_cap, load, store_for(load, w), w.rec_ws, s, false);
+    if constexpr (std::is_same_v<Load, BrushLoad>) {
+      // Brushes: a scan over the brick rows (counts[kCntCandidates] of them),
+      // then one thread per touched brick.
+      const TouchedRows rows{w.row_rec, w.row_job, w.row_yz, w.row_x0, &w.counts[kCntRows]};
+      st = scan::fused_dev_m<scan::PairSumOp>(&w.counts[kCntCandidates], cand_cap, RowLoad{load.c},
+                                              RowStore{rows, &w.counts[0]}, w.rec_ws, s, false);
+      if (st == VX_OK) {
+        brush_records_kernel<<<grid_for(cand_cap), 256, 0, s>>>(load.c, rows, record_out(w));
+        st = last_launch();
+      }
+    } else {
+      st = scan::fused_dev<scan::kSum>(&w.counts[kCntCandidates], cand_cap, load, store_for(load, w), w.rec_ws, s,
+                                       false);
+    }
   }
   if (st != VX_OK) return st;
   return rebuild(prof_dag, p, sc, w, cand_cap, s);
@@ -1777,13 +1944,15 @@ vx_status device_pipeline(vx_dag* prof_dag, const Plan& p, const Scenes& sc, Edi
 
 // Shared driver of the direct (uncaptured) calls.
 template <class Load>
+// n_first: the first scan's element count (voxels, or brush brick rows);
+// cap: the upper bound of the records (the grids and workspace are sized by it).
 vx_status run_edit(const std::vector<vx_dag*>& dags, vx_dag* prof_dag, const Plan& p, EditWs& w,
-                   const std::vector<BrushJob>* jobs, uint32_t n_cand, Load load, hipStream_t s) {
+                   const std::vector<BrushJob>* jobs, uint32_t n_first, uint64_t cap, Load load, hipStream_t s) {
   vx_status st;
   std::vector<Dev> devs;
   for (vx_dag* g : dags) devs.push_back(g->dev);
   HostStage& hs = dags[0]->stage;
-  if ((st = upload(w, devs, jobs, n_cand, hs, s)) != VX_OK) return st;
+  if ((st = upload(w, devs, jobs, n_first, hs, s)) != VX_OK) return st;
   const Scenes sc{p.n_scenes > 1 ? w.scenes : nullptr, nullptr, p.codec, devs[0]};
   {
     // Per-launch timing of this call's kernels while the dag is profiled.
@@ -1800,7 +1969,7 @@ vx_status run_edit(const std::vector<vx_dag*>& dags, vx_dag* prof_dag, const Pla
       }
       ~Install() { current_timer() = prev; }
     } install(prof_dag && prof_dag->profile ? prof_dag->timer : nullptr, prof_dag ? prof_dag->prof.edits : 0);
-    if ((st = device_pipeline(prof_dag, p, sc, w, n_cand, load, s)) != VX_OK) return st;
+    if ((st = device_pipeline(prof_dag, p, sc, w, cap, load, s)) != VX_OK) return st;
   }
   return finish(prof_dag, dags, w, p, hs, s);
 }
@@ -1954,7 +2123,7 @@ extern "C" vx_status vx_dag_apply_voxel_edits(vx_dag* g, const vx_voxel* edits, 
   if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
   const uint32_t mat = op == 1 ? 0u : (g->materials ? (material & 0xffu) : 1u);
   if (op == 0 && g->materials && mat == 0) return VX_INVALID_VALUE;
-  return run_edit({g}, g, p, w, nullptr, count, VoxelLoad{edits, 4u << g->depth, mat},
+  return run_edit({g}, g, p, w, nullptr, count, count, VoxelLoad{edits, 4u << g->depth, mat},
                   static_cast<hipStream_t>(stream));
 }
 
@@ -1970,13 +2139,14 @@ extern "C" vx_status vx_dag_apply_brushes(vx_dag* g, const vx_brush* b, uint32_t
                                           void* stream) {
   if (!g || (!b && n)) return VX_INVALID_VALUE;
   std::vector<BrushJob> jobs;
-  const uint64_t cand = make_jobs(b, n, 4u << g->depth, 0, 0, &jobs);
+  uint64_t rows = 0;
+  const uint64_t cand = make_jobs(b, n, 4u << g->depth, 0, 0, &jobs, &rows);
   if (cand == 0) return VX_OK;
   if (cand >= kMaxEntries) return VX_INVALID_VALUE;
   const Plan p = make_plan({g});
   EditWs w = carve(ws, p, cand, uint32_t(jobs.size()), cand);
   if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
-  return run_edit({g}, g, p, w, &jobs, uint32_t(cand),
+  return run_edit({g}, g, p, w, &jobs, uint32_t(rows), cand,
                   BrushLoad{BrushCands{w.jobs, &w.counts[kCntJobs], g->materials, p.codec}},
                   static_cast<hipStream_t>(stream));
 }
@@ -1985,7 +2155,7 @@ extern "C" vx_status vx_dag_apply_brushes(vx_dag* g, const vx_brush* b, uint32_t
 // order, scenes are independent; all of it runs as one pipeline.
 static vx_status batch_setup(vx_dag* const* dags, const vx_brush* const* brushes, const uint32_t* counts,
                              uint32_t n_batches, std::vector<vx_dag*>* scenes, std::vector<BrushJob>* jobs,
-                             uint64_t* cand) {
+                             uint64_t* cand, uint64_t* rows = nullptr) {
   if (!dags || !brushes || !counts || n_batches == 0) return VX_INVALID_VALUE;
   std::unordered_map<const vx_dag*, uint32_t> index;
   for (uint32_t i = 0; i < n_batches; ++i) {
@@ -2000,7 +2170,7 @@ static vx_status batch_setup(vx_dag* const* dags, const vx_brush* const* brushes
   for (uint32_t s = 0; s < scenes->size(); ++s)
     for (uint32_t i = 0; i < n_batches; ++i)
       if (index[dags[i]] == s)
-        *cand += make_jobs(brushes[i], counts[i], 4u << dags[i]->depth, s, *cand, jobs);
+        *cand += make_jobs(brushes[i], counts[i], 4u << dags[i]->depth, s, *cand, jobs, rows);
   return VX_OK;
 }
 
@@ -2021,8 +2191,8 @@ extern "C" vx_status vx_dag_apply_batches(vx_dag* const* dags, const vx_brush* c
                                           uint32_t n_batches, void* ws, size_t ws_bytes, void* stream) {
   std::vector<vx_dag*> scenes;
   std::vector<BrushJob> jobs;
-  uint64_t cand = 0;
-  vx_status st = batch_setup(dags, brushes, counts, n_batches, &scenes, &jobs, &cand);
+  uint64_t cand = 0, rows = 0;
+  vx_status st = batch_setup(dags, brushes, counts, n_batches, &scenes, &jobs, &cand, &rows);
   if (st != VX_OK) return st;
   if (cand == 0) return VX_OK;
   if (cand >= kMaxEntries || scenes.size() > (1u << 20)) return VX_INVALID_VALUE;
@@ -2030,7 +2200,7 @@ extern "C" vx_status vx_dag_apply_batches(vx_dag* const* dags, const vx_brush* c
   if (p.key_bits > 64) return VX_INVALID_VALUE;
   EditWs w = carve(ws, p, cand, uint32_t(jobs.size()), cand);
   if (!ws || ws_bytes < w.bytes) return VX_INVALID_VALUE;
-  return run_edit(scenes, scenes[0], p, w, &jobs, uint32_t(cand),
+  return run_edit(scenes, scenes[0], p, w, &jobs, uint32_t(rows), cand,
                   BrushLoad{BrushCands{w.jobs, &w.counts[kCntJobs], scenes[0]->materials, p.codec}},
                   static_cast<hipStream_t>(stream));
 }
@@ -2297,16 +2467,18 @@ extern "C" vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batc
   for (uint32_t i = 0; i < n_batches; ++i)
     if (scene_of_batch[i] >= b->scenes.size() || (!brushes[i] && counts[i])) return VX_INVALID_VALUE;
   b->jobs.clear();
-  uint64_t cand = 0;
+  uint64_t cand = 0, rows = 0;
   for (uint32_t s = 0; s < b->scenes.size(); ++s)
     for (uint32_t i = 0; i < n_batches; ++i)
-      if (scene_of_batch[i] == s) cand += make_jobs(brushes[i], counts[i], 4u << b->scenes[s]->depth, s, cand, &b->jobs);
+      if (scene_of_batch[i] == s)
+        cand += make_jobs(brushes[i], counts[i], 4u << b->scenes[s]->depth, s, cand, &b->jobs, &rows);
   if (candidates) *candidates = cand;
   if (cand > b->cap_cand || b->jobs.size() > b->cap_jobs) return VX_INVALID_VALUE;
   if (cand == 0) return VX_OK;
   hipStream_t s = b->stream;
   vx_status st;
-  if ((st = upload(b->w, b->devs, &b->jobs, uint32_t(cand), b->stage, s)) != VX_OK) return st;
+  // The brush path's first scan runs over brick rows.
+  if ((st = upload(b->w, b->devs, &b->jobs, uint32_t(rows), b->stage, s)) != VX_OK) return st;
   const Scenes sc{b->plan.n_scenes > 1 ? b->w.scenes : nullptr, nullptr, b->plan.codec, b->devs[0]};
   const BrushLoad load{BrushCands{b->w.jobs, &b->w.counts[kCntJobs], b->scenes[0]->materials, b->plan.codec}};
   if (b->use_graph) {
