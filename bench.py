@@ -32,7 +32,7 @@ METRIC = "voxel edits/s + HBM GB/s, batched vs plain, bit-exact DAG; p50/p99 lat
 HBM_PEAK_GBS = 8000.0
 # Kernel kinds of the edit pipeline -> the device symbol rocprofv3 reports.
 KERNEL_SYMBOL = {
-    "records": "fused_dev_kernel<SumOp, BrushLoad, BrushStore>",
+    "records": "fused_dev_kernel<PairSumOp, RowLoad, RowStore> + brush_records_kernel",
     "sort_partition": "partition_kernel<unsigned long, true, ...>",
     "leaf_build": "leaf_build_kernel<16u>",
     "leaf_probe": "probe_kernel<16u>",
@@ -554,7 +554,8 @@ def kernel_bytes(prof, leaf_words):
     Q = prof.pending_leaves  # leaves the build's lookup did not find: the probe, commit and place run over these
     lw = 4 * leaf_words
     return {
-        "records": (prof.sort_record_bytes + 12) * R,   # sort record + mask 8 + material 4 written
+        "records": (prof.sort_record_bytes + 12) * R,   # sort record + mask 8 + material 4 written (the touched
+                                                        # rows, 16 bytes each and ~R/16 of them: not counted)
         "sort_histogram": 8 * R,                        # keys read once for every pass's histogram
         "sort_partition": 2 * prof.sort_record_bytes * R * P,  # per pass: each record read and written
         "runs": 8 * R + 16 * L,                         # sorted keys read; leaf keys, run starts, parent links
@@ -584,7 +585,7 @@ PMC_COMMAND = ("cd /tmp && export TMPDIR=/tmp && cd <repo> && "
 
 def _kind_of(name, last_probe):
     """Edit kernel kind of a device symbol (see KERNEL_SYMBOL)."""
-    table = [("BrushLoad", "records"), ("VoxelLoad", "records"), ("histogram_kernel", "sort_histogram"),
+    table = [("RowLoad", "records"), ("brush_records_kernel", "records"), ("VoxelLoad", "records"), ("histogram_kernel", "sort_histogram"),
              ("hist_scan_kernel", "sort_scan"), ("partition_kernel", "sort_partition"), ("copy_kernel", "sort_copy"),
              ("RunUpLoad", "runs"), ("UpLoad", "up"), ("topdown_kernel", "topdown"), ("leaf_build_kernel", "leaf_build"),
              ("node_build_kernel", "node_build"), ("publish_kernel", "publish"), ("fill_u32_kernel", "fill"),

@@ -586,32 +586,6 @@ __device__ __forceinline__ uint64_t brick_mask(const vx_brush& b, int32_t X, int
   return m;
 }
 
-// Whether brick (X, Y, Z) of a job holds any voxel of its brush, without the
-// mask. Every brick of a box's clipped brick range overlaps the box on every
-// axis. For a sphere, the brick's lattice point nearest to the centre
-// (clamped axis by axis) minimises each squared term, so the brick is touched
-// iff that point is inside.
-__device__ __noinline__ bool sphere_touched64(int32_t cx, int32_t cy, int32_t cz, int32_t r, int32_t X, int32_t Y,
-                                             int32_t Z) {
-  const int64_t c[3] = {cx, cy, cz};
-  const int64_t lo[3] = {X, Y, Z};
-  int64_t d2 = 0;
-  for (int a = 0; a < 3; ++a) {
-    const int64_t p = c[a] < lo[a] ? lo[a] : (c[a] > lo[a] + 3 ? lo[a] + 3 : c[a]);
-    d2 += (p - c[a]) * (p - c[a]);
-  }
-  return d2 <= int64_t(r) * r;
-}
-
-__device__ __forceinline__ bool brick_touched(const vx_brush& b, int32_t X, int32_t Y, int32_t Z) {
-  if (b.kind != 0) return true;
-  if (b.hx >= (1 << 14)) return sphere_touched64(b.cx, b.cy, b.cz, b.hx, X, Y, Z);
-  const int32_t dx = min(max(X - b.cx, 0), X + 3 - b.cx) ;
-  const int32_t dy = min(max(Y - b.cy, 0), Y + 3 - b.cy);
-  const int32_t dz = min(max(Z - b.cz, 0), Z + 3 - b.cz);
-  return dx * dx + dy * dy + dz * dz <= b.hx * b.hx;
-}
-
 // Record outputs. With index_bits > 0 the sort key is packed above the
 // record index (key << index_bits | index) and sorted by its upper bits, so
 // the sort moves 8 bytes per record instead of a 12-byte (key, index) pair.
@@ -643,7 +617,8 @@ struct SortedRecs {
   }
 };
 
-// Brush candidates: candidate i is one brick of one job's clipped brick box.
+// Brush jobs on the device: job_of finds the job of brick row i (ROWS) or of
+// brick i of the concatenated clipped brick boxes.
 struct BrushCands {
   const BrushJob* jobs;
   const uint32_t* n_jobs;  // device count
@@ -671,71 +646,21 @@ struct BrushCands {
     while (lo + 1 < nj && first(lo + 1) <= i) ++lo;
     return lo;
   }
-  __device__ __forceinline__ void brick_of(const BrushJob& j, uint32_t i, uint32_t& gx, uint32_t& gy,
-                                           uint32_t& gz) const {
-    uint32_t row, bx, by, bz;
-    divmod_small(i - j.start, j.nx, j.inv_nx, row, bx);
-    divmod_small(row, j.ny, j.inv_ny, bz, by);
-    gx = j.bx0 + bx;
-    gy = j.by0 + by;
-    gz = j.bz0 + bz;
-  }
 };
 
-// A touched brick travelling from the Load half to the Store half: its job
-// and brick coordinates (each below 2^15 at inner depth <= 15).
-struct BrickRef {
-  uint32_t job;
-  uint32_t gxy;  // gx | gy << 16
-  uint32_t gz;
-};
-
-// Load half: the touched test only. A thread's candidates increase from one
-// call to the next, so the previous job is the next call's search hint (the
-// scan calls Load on a per-thread copy).
+// The brush path's input: its jobs (the records come from the brick rows
+// below).
 struct BrushLoad {
-  using Payload = BrickRef;
   BrushCands c;
-  uint32_t hint = kNone;
-  uint32_t nj = 0;
-  __device__ uint32_t operator()(uint32_t i, BrickRef& r) {
-    if (hint == kNone) nj = *c.n_jobs;
-    r.job = c.job_of(i, nj, hint);
-    hint = r.job;
-    const BrushJob j = c.jobs[r.job];
-    uint32_t gx, gy, gz;
-    c.brick_of(j, i, gx, gy, gz);
-    r.gxy = gx | (gy << 16);
-    r.gz = gz;
-    return brick_touched(j.b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4)) ? 1u : 0u;
-  }
-};
-
-// Store half: mask, key and material of each touched brick, at its compacted
-// position (brush order is preserved).
-struct BrushStore {
-  BrushCands c;
-  RecordOut out;
-  __device__ void operator()(uint32_t i, const BrickRef& r, uint32_t v, uint32_t excl, uint32_t incl,
-                             uint32_t n) const {
-    if (v) {
-      const BrushJob j = c.jobs[r.job];
-      const vx_brush& b = j.b;
-      const uint32_t gx = r.gxy & 0xffffu, gy = r.gxy >> 16, gz = r.gz;
-      out.put_key(excl, (uint64_t(j.scene) << c.codec.shift) | morton3(gx, gy, gz));
-      out.mask[excl] = brick_mask(b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4));
-      out.mat[excl] = c.materials ? b.material : (b.material ? 1u : 0u);
-    }
-    if (i == n - 1) *out.n_rec = incl;
-  }
 };
 
 // ---- brush records by brick rows ---------------------------------------------
 // A job's clipped brick box is walked by rows (fixed brick y and z, the box's
 // x run). The touched bricks of a row are one run: the whole row for a box;
 // for a sphere the bricks whose voxel nearest to the centre lies inside, an
-// interval in closed form (integer square root) with the same test as
-// brick_touched. A scan over the rows counts them and drops empty rows; then
+// interval in closed form (integer square root): a brick is touched iff its
+// voxel nearest to the centre (clamped axis by axis, which minimises each
+// squared term) is inside. A scan over the rows counts them and drops empty rows; then
 // one thread per record writes it, in the same order as a walk over every
 // brick of the box (rows in order, x within a row). Only touched bricks are
 // enumerated: about 37% of a sphere batch's box bricks are not.
@@ -1902,9 +1827,7 @@ inline RecordOut record_out(EditWs& w) {
 inline RecordStore store_for(const VoxelLoad&, EditWs& w) {
   return RecordStore{record_out(w)};
 }
-inline BrushStore store_for(const BrushLoad& l, EditWs& w) {
-  return BrushStore{l.c, record_out(w)};
-}
+
 
 // All device work of an edit (no host synchronisation, no memset nodes), so a
 // stream capture of it replays as a hipGraph. Grids are sized by the
