@@ -137,6 +137,30 @@ def test_sphere_radii_across_arithmetic_paths_match_oracle(gpu, materials):
 
 
 @pytest.mark.parametrize("materials", [False, True])
+def test_huge_sphere_caps_match_oracle(gpu, materials):
+    """Spheres of radius 2^14 .. 2^20 whose centres lie far outside a 512^3
+    world, so only a curved cap crosses it: the brick rows' touched runs come
+    from 64-bit squares and an integer square root near 2^40, and the masks
+    from the 64-bit sphere test."""
+    depth = 7
+    br = np.zeros(4, O.BRUSH_DTYPE)
+    caps = [(256, 250, -16300, (1 << 14) + 37, 9), (-20000, 300, 200, 20090, 0),
+            (700, 100, 256, (1 << 20) - 3, 31), (200, (1 << 20) + 500, 300, (1 << 20) + 433, 0)]
+    for i, (cx, cy, cz, r, m) in enumerate(caps):
+        br[i]["cx"], br[i]["cy"], br[i]["cz"], br[i]["hx"], br[i]["kind"], br[i]["material"] = cx, cy, cz, r, 0, m or 200
+    br[1]["material"] = 0  # a removal
+    br[3]["material"] = 0
+    from voxel_amd.dag import terrain_for_world
+    d = _dag(depth, materials, node_words=1 << 24, leaf_words=1 << 25)
+    d.build_terrain(terrain_for_world(4 << depth, 2))
+    d.apply_brushes(br)
+    ref = O.Dag(depth, materials)
+    ref.build_terrain(O.terrain_for_world(4 << depth, 2))
+    ref.apply_brushes(br)
+    assert np.array_equal(d.export(), ref.export())
+
+
+@pytest.mark.parametrize("materials", [False, True])
 def test_successive_batches_match_oracle(gpu, materials):
     depth = 7
     d = _dag(depth, materials)
