@@ -275,6 +275,11 @@ vx_status vx_batcher_destroy(vx_batcher* b);
 vx_status vx_batcher_info(const vx_batcher* b, uint64_t* workspace_bytes, int32_t* graph_ready);
 vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batch, const vx_brush* const* h_brushes,
                          const uint32_t* counts, uint32_t n_batches, uint64_t* candidates);
+/* Record layout of the executor's sort: *key_bits = scene bits + 3 bits per
+ * level of the deepest scene; *index_bits > 0 when (key, record index) pack
+ * into one u64 sorted by its upper bits, 0 when records sort as (u64 key,
+ * u32 index) pairs (key_bits + index bits above 64). */
+vx_status vx_batcher_layout(const vx_batcher* b, uint32_t* key_bits, uint32_t* index_bits);
 
 /* get_voxel for n device coordinates: d_out[i] = material (1 = solid in an
  * occupancy DAG), 0 if empty or out of the world (query.cuh:12-39). */
@@ -308,8 +313,17 @@ vx_status vx_dag_raycast(const vx_dag* dag, const vx_ray* d_rays, uint32_t n, vx
  *   [magic 0x56584443, inner_depth, leaf_words, root kind (0 EMPTY, 1 FULL,
  *    2 node), node count, leaf count], then per node in first-visit order
  *   [level, child_mask | fill_mask << 8, child ids...], then leaf payloads.
- * *n_words receives the size; words are written only if cap suffices. */
+ * Child ids of level inner_depth-1 nodes are leaf ids. *n_words receives the
+ * size; words are written only if cap suffices. Computed on the GPU
+ * (first-visit order from every node's smallest root path); synchronises
+ * the device and allocates temporaries. The reference checks its GPU DAG
+ * through get_voxel only (test/cuda/test_cuda_hash_dag_gpu.cu:181-535); this
+ * export is the bit-exact comparison form of the canonical DAG. */
 vx_status vx_dag_export_canonical(const vx_dag* dag, uint32_t* out, uint64_t cap, uint64_t* n_words);
+/* The same words into device memory d_out, ordered on `stream` (which it
+ * synchronises); with d_out NULL or cap short only *n_words is computed. */
+vx_status vx_dag_export_canonical_device(const vx_dag* dag, uint32_t* d_out, uint64_t cap, uint64_t* n_words,
+                                         void* stream);
 
 /* Garbage collection of the interned tables: keeps exactly the items
  * reachable from the root, each once, and rebuilds both slot tables over them

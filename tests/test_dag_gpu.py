@@ -405,3 +405,32 @@ def test_pair_sort_layout_matches_packed(gpu, monkeypatch):
             d.apply_brushes(b)
         out.append(d.export())
     assert np.array_equal(out[0], out[1])
+
+
+def test_batcher_keeps_its_sort_tuning(gpu):
+    """A sort tuning change after the executor is created does not change the
+    layout of the workspace it carved: runs without a graph still sort into
+    the space sized at creation and give the sequential result."""
+    from voxel_amd.dag import Batcher, batch_candidates, terrain_for_world
+    from voxel_amd.primitives import SORT_KEYS_ONLY, get_sort_tuning, set_sort_tuning
+    depths = [5, 6]
+    a = [_dag(d, True, node_words=1 << 22, leaf_words=1 << 23) for d in depths]
+    b = [_dag(d, True, node_words=1 << 22, leaf_words=1 << 23) for d in depths]
+    for k, d in enumerate(depths):
+        a[k].build_terrain(terrain_for_world(4 << d, 50 + k))
+        b[k].build_terrain(terrain_for_world(4 << d, 50 + k))
+    rb = [(i % 2, O.gen_brushes(300 + i, O.STREAM_BRUSH, 30, 4 << depths[i % 2], 1)) for i in range(4)]
+    ex = Batcher(a, batch_candidates(a, rb), 256, use_graph=False)
+    rows = [8 | SORT_KEYS_ONLY, 8]
+    saved = [get_sort_tuning(r) for r in rows]
+    try:
+        for r in rows:  # 8-bit digits in 2048-key tiles: twice the look-back rows of the default
+            set_sort_tuning(r, 8, 8, 256)
+        ex.run(rb)
+    finally:
+        for r, t in zip(rows, saved):
+            set_sort_tuning(r, *t)
+    for s, br in rb:
+        b[s].apply_brushes(br)
+    for k in range(2):
+        assert np.array_equal(a[k].export(), b[k].export()), k

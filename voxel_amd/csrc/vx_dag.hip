@@ -1232,6 +1232,7 @@ struct EditWs {
   uint64_t *rkey, *rkey_alt, *rmask;
   uint32_t *rval, *rval_alt, *rmat;
   void* sort_ws;
+  sort::Tuning sort_tn;  // the tuning sort_ws was sized for
   uint32_t index_bits;  // > 0: records sort as packed u64 keys (RecordOut)
   Heights hs;
   uint32_t* items;
@@ -1493,8 +1494,9 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
   w.rval = w.index_bits ? nullptr : c.take<uint32_t>(R);
   w.rval_alt = w.index_bits ? nullptr : c.take<uint32_t>(R);
   w.rmat = c.take<uint32_t>(R);
-  w.sort_ws = c.take<char>(w.index_bits ? sort::keys_u64_dev_workspace_bytes(R, std::max(p.key_bits, 1u))
-                                        : sort::pairs_u64_dev_workspace_bytes(R, p.key_bits));
+  w.sort_tn = w.index_bits ? sort::keys_u64_dev_tuning(std::max(p.key_bits, 1u)) : sort::pairs_u64_dev_tuning();
+  w.sort_ws = c.take<char>(w.index_bits ? sort::keys_u64_dev_workspace_bytes(R, std::max(p.key_bits, 1u), w.sort_tn)
+                                        : sort::pairs_u64_dev_workspace_bytes(R, p.key_bits, w.sort_tn));
   uint64_t max_cap = 1, max_items = 1;
   const uint32_t H = p.max_depth;
   for (uint32_t h = 0; h <= H + 1; ++h) {
@@ -1557,10 +1559,10 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
   mark(g, 1, s);
   if (w.index_bits)
     st = sort::keys_u64_dev(w.rkey, w.rkey_alt, &cnt[0], std::max<uint64_t>(r_ub, 1), w.index_bits,
-                            std::max(p.key_bits, 1u), w.sort_ws, &in_alt, s);
+                            std::max(p.key_bits, 1u), w.sort_ws, w.sort_tn, &in_alt, s);
   else
     st = sort::pairs_u64_dev(w.rkey, w.rval, w.rkey_alt, w.rval_alt, &cnt[0], std::max<uint64_t>(r_ub, 1),
-                             std::max(p.key_bits, 1u), w.sort_ws, &in_alt, s);
+                             std::max(p.key_bits, 1u), w.sort_ws, w.sort_tn, &in_alt, s);
   if (st != VX_OK) return st;
   mark(g, 2, s);
   const SortedRecs recs{in_alt ? w.rkey_alt : w.rkey, in_alt ? w.rval_alt : w.rval, w.index_bits};
@@ -1761,7 +1763,7 @@ vx_status finish(vx_dag* g, const std::vector<vx_dag*>& dags, const EditWs& w, c
     for (uint32_t h = 0; h <= p.max_depth && h < 16; ++h) g->prof.height_entries[h] += counts[1 + h];
     for (uint32_t h = 1; h <= p.max_depth; ++h) g->prof.nodes += counts[1 + h];
     const uint32_t kb = std::max(p.key_bits, 1u);
-    g->prof.sort_passes = w.index_bits ? sort::keys_u64_dev_passes(kb) : sort::pairs_u64_dev_passes(kb);
+    g->prof.sort_passes = sort::passes(kb, w.sort_tn);
     g->prof.sort_record_bytes = w.index_bits ? 8 : 12;
     if (g->timer) {
       KernelTimer& t = *g->timer;
@@ -2135,101 +2137,6 @@ extern "C" vx_status vx_dag_get_voxels(const vx_dag* g, const vx_voxel* q, uint3
   return last_launch();
 }
 
-// Canonical export: depth-first from the root, children 0..7, ids at first
-// visit, memo by (level, ref) — refs are content-unique per table, so the
-// export depends only on the voxel/material field.
-extern "C" vx_status vx_dag_export_canonical(const vx_dag* g, uint32_t* out, uint64_t cap, uint64_t* n_words) {
-  if (!g || !n_words) return VX_INVALID_VALUE;
-  if (hipDeviceSynchronize() != hipSuccess) return VX_DEVICE_ERROR;
-  uint32_t ctl[kCtlWords];
-  if (hipMemcpy(ctl, g->ctl, sizeof(ctl), hipMemcpyDeviceToHost) != hipSuccess) return VX_DEVICE_ERROR;
-  const uint64_t nu = std::min<uint64_t>(ctl[kCtlNodeUsed], g->dev.node_cap);
-  const uint64_t lu = std::min<uint64_t>(ctl[kCtlLeafUsed], g->dev.leaf_cap);
-  std::vector<uint32_t> np(nu), lp(lu);
-  if (nu && hipMemcpy(np.data(), g->node_pool, nu * 4, hipMemcpyDeviceToHost) != hipSuccess) return VX_DEVICE_ERROR;
-  if (lu && hipMemcpy(lp.data(), g->leaf_pool, lu * 4, hipMemcpyDeviceToHost) != hipSuccess) return VX_DEVICE_ERROR;
-  const uint32_t D = g->depth, LW = g->dev.leaf_words, root = ctl[kCtlRoot];
-  std::unordered_map<uint64_t, uint32_t> node_id, leaf_id;
-  std::vector<std::vector<uint32_t>> node_recs;
-  std::vector<uint32_t> leaf_refs;
-  struct Frame {
-    uint32_t level, ref, id, c;
-    uint32_t cm, fm;
-    std::vector<uint32_t> ids;
-  };
-  auto leaf_of = [&](uint32_t ref) -> uint32_t {
-    auto it = leaf_id.find(ref);
-    if (it != leaf_id.end()) return it->second;
-    const uint32_t id = uint32_t(leaf_refs.size());
-    leaf_id.emplace(ref, id);
-    leaf_refs.push_back(ref);
-    return id;
-  };
-  uint32_t root_kind = root == kEmpty ? 0 : (root == kFull ? 1 : 2);
-  if (root_kind == 2) {
-    std::vector<Frame> stack;
-    auto open = [&](uint32_t level, uint32_t ref) -> uint32_t {
-      const uint32_t id = uint32_t(node_recs.size());
-      node_id.emplace((uint64_t(level) << 32) | ref, id);
-      node_recs.emplace_back();
-      stack.push_back(Frame{level, ref, id, 0, 0, 0, {}});
-      return id;
-    };
-    open(0, root);
-    while (!stack.empty()) {
-      Frame& f = stack.back();
-      if (f.c == 8) {
-        std::vector<uint32_t>& rec = node_recs[f.id];
-        rec.push_back(f.level);
-        rec.push_back(f.cm | (f.fm << 8));
-        rec.insert(rec.end(), f.ids.begin(), f.ids.end());
-        stack.pop_back();
-        continue;
-      }
-      const uint32_t c = f.c++;
-      if (f.ref >= nu) return VX_DEVICE_ERROR;
-      const uint32_t h = np[f.ref], cm = h & 0xffu, fm = (h >> 8) & 0xffu, bit = 1u << c;
-      if (!(cm & bit)) {
-        if (fm & bit) f.fm |= bit;
-        continue;
-      }
-      const uint32_t child = np[f.ref + 1 + __builtin_popcount(cm & (bit - 1))];
-      f.cm |= bit;
-      if (f.level + 1 == D) {
-        if (child + LW > lu) return VX_DEVICE_ERROR;
-        f.ids.push_back(leaf_of(child));
-        continue;
-      }
-      auto it = node_id.find((uint64_t(f.level + 1) << 32) | child);
-      if (it != node_id.end()) {
-        f.ids.push_back(it->second);
-        continue;
-      }
-      const uint32_t lvl = f.level + 1;
-      const uint32_t id = uint32_t(node_recs.size());
-      f.ids.push_back(id);
-      open(lvl, child);  // invalidates f
-    }
-  }
-  uint64_t total = 6;
-  for (auto& r : node_recs) total += r.size();
-  total += uint64_t(leaf_refs.size()) * LW;
-  *n_words = total;
-  if (!out || cap < total) return VX_OK;
-  uint64_t k = 0;
-  out[k++] = 0x56584443u;
-  out[k++] = D;
-  out[k++] = LW;
-  out[k++] = root_kind;
-  out[k++] = uint32_t(node_recs.size());
-  out[k++] = uint32_t(leaf_refs.size());
-  for (auto& r : node_recs)
-    for (uint32_t v : r) out[k++] = v;
-  for (uint32_t ref : leaf_refs)
-    for (uint32_t i = 0; i < LW; ++i) out[k++] = lp[ref + i];
-  return VX_OK;
-}
-
 extern "C" vx_status vx_dag_profile_enable(vx_dag* g, int32_t enable) {
   if (!g) return VX_INVALID_VALUE;
   if (enable && !g->profile) {
@@ -2381,6 +2288,13 @@ extern "C" vx_status vx_batcher_info(const vx_batcher* b, uint64_t* workspace_by
   if (!b) return VX_INVALID_VALUE;
   if (workspace_bytes) *workspace_bytes = b->ws_bytes;
   if (graph_ready) *graph_ready = b->exec != nullptr;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_batcher_layout(const vx_batcher* b, uint32_t* key_bits, uint32_t* index_bits) {
+  if (!b) return VX_INVALID_VALUE;
+  if (key_bits) *key_bits = b->plan.key_bits;
+  if (index_bits) *index_bits = b->w.index_bits;
   return VX_OK;
 }
 

@@ -274,9 +274,6 @@ __global__ void copy_kernel(const T* __restrict__ in, T* __restrict__ out, uint3
 }
 
 // ---- tuning table -----------------------------------------------------------
-struct Tuning {
-  uint32_t rb, kpt, block;
-};
 std::mutex g_mu;
 // Rows: u32 pairs, u64 pairs, u32 keys only, u64 keys only.
 Tuning g_tuning[4] = {{8, 15, 256}, {8, 15, 256}, {8, 15, 256}, {9, 8, 1024}};
@@ -503,21 +500,24 @@ vx_status sort_u64(uint64_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits
               : run<uint64_t, false>(keys, nullptr, n, key_bits, ws, ws_size, s);
 }
 
-uint32_t pairs_u64_dev_passes(uint32_t key_bits) {
-  const Tuning tn = tuning_for(8);
-  return (key_bits + tn.rb - 1) / tn.rb;
+uint32_t passes(uint32_t key_bits, const Tuning& tn) { return (key_bits + tn.rb - 1) / tn.rb; }
+
+Tuning pairs_u64_dev_tuning() {
+  Tuning tn = tuning_for(8);
+  Launcher<uint64_t, true> L;
+  if (!pick(L, tn)) tn = Tuning{8, 8, 1024};
+  return tn;
 }
 
-size_t pairs_u64_dev_workspace_bytes(uint64_t n_upper, uint32_t key_bits) {
-  const Tuning tn = tuning_for(8);
+size_t pairs_u64_dev_workspace_bytes(uint64_t n_upper, uint32_t key_bits, const Tuning& tn) {
   Launcher<uint64_t, true> L;
   if (!pick(L, tn)) return 0;
   return layout(nullptr, n_upper, 8, key_bits, true, tn, L.tile, false).bytes;
 }
 
 vx_status pairs_u64_dev(uint64_t* keys, uint32_t* vals, uint64_t* alt_keys, uint32_t* alt_vals, const uint32_t* n_ptr,
-                        uint64_t n_upper, uint32_t key_bits, void* ws, bool* in_alt, hipStream_t s) {
-  const Tuning tn = tuning_for(8);
+                        uint64_t n_upper, uint32_t key_bits, void* ws, const Tuning& tn, bool* in_alt,
+                        hipStream_t s) {
   Launcher<uint64_t, true> L;
   if (!pick(L, tn)) return VX_UNSUPPORTED;
   if (n_upper >= (1u << 30)) return VX_INVALID_VALUE;
@@ -528,7 +528,7 @@ vx_status pairs_u64_dev(uint64_t* keys, uint32_t* vals, uint64_t* alt_keys, uint
 
 // Packed u64 keys sorted by bits [begin_bit, begin_bit + key_bits): the
 // u64 keys-only tuning row; 8-bit digits where 9-bit ones save no pass.
-Tuning packed_tuning(uint32_t key_bits) {
+Tuning keys_u64_dev_tuning(uint32_t key_bits) {
   Tuning tn = tuning_for(8, false);
   if (tn.rb == 9 && (key_bits + 8) / 9 == (key_bits + 7) / 8) {
     Launcher<uint64_t, false> L;
@@ -540,21 +540,14 @@ Tuning packed_tuning(uint32_t key_bits) {
   return tn;
 }
 
-uint32_t keys_u64_dev_passes(uint32_t key_bits) {
-  const Tuning tn = packed_tuning(key_bits);
-  return (key_bits + tn.rb - 1) / tn.rb;
-}
-
-size_t keys_u64_dev_workspace_bytes(uint64_t n_upper, uint32_t key_bits) {
-  const Tuning tn = packed_tuning(key_bits);
+size_t keys_u64_dev_workspace_bytes(uint64_t n_upper, uint32_t key_bits, const Tuning& tn) {
   Launcher<uint64_t, false> L;
   if (!pick(L, tn)) return 0;
   return layout(nullptr, n_upper, 8, key_bits, false, tn, L.tile, false).bytes;
 }
 
 vx_status keys_u64_dev(uint64_t* keys, uint64_t* alt_keys, const uint32_t* n_ptr, uint64_t n_upper, uint32_t begin_bit,
-                       uint32_t key_bits, void* ws, bool* in_alt, hipStream_t s) {
-  const Tuning tn = packed_tuning(key_bits);
+                       uint32_t key_bits, void* ws, const Tuning& tn, bool* in_alt, hipStream_t s) {
   Launcher<uint64_t, false> L;
   if (!pick(L, tn)) return VX_UNSUPPORTED;
   if (n_upper >= (1u << 30) || begin_bit + key_bits > 64) return VX_INVALID_VALUE;

@@ -455,6 +455,7 @@ struct Ws {
   uint64_t* recs;
   uint64_t* alt;
   void* sort_ws;
+  sort::Tuning sort_tn;
   void* fused_ws;
   uint32_t* mkey;
   uint64_t* mbits;
@@ -472,7 +473,8 @@ Ws carve(void* p, uint32_t count, bool edit) {
   w.cnt = c.take<uint32_t>(64);
   w.recs = c.take<uint64_t>(n);
   w.alt = c.take<uint64_t>(n);
-  w.sort_ws = c.take<char>(sort::keys_u64_dev_workspace_bytes(n, kKeyBits));
+  w.sort_tn = sort::keys_u64_dev_tuning(kKeyBits);
+  w.sort_ws = c.take<char>(sort::keys_u64_dev_workspace_bytes(n, kKeyBits, w.sort_tn));
   w.fused_ws = c.take<char>(scan::fused_dev_workspace_bytes(n));
   w.mkey = c.take<uint32_t>(n);
   w.mbits = c.take<uint64_t>(n);
@@ -495,7 +497,7 @@ vx_status build_masks(const vx_voxel* edits, uint32_t count, const Ws& w, hipStr
   if ((st = scan::fused<scan::kSum>(count, RecLoad{edits}, RecStore{w.recs, w.cnt + kNValid}, w.fused_ws, s)) != VX_OK)
     return st;
   bool in_alt = false;
-  if ((st = sort::keys_u64_dev(w.recs, w.alt, w.cnt + kNValid, count, 0, kKeyBits, w.sort_ws, &in_alt, s)) != VX_OK)
+  if ((st = sort::keys_u64_dev(w.recs, w.alt, w.cnt + kNValid, count, 0, kKeyBits, w.sort_ws, w.sort_tn, &in_alt, s)) != VX_OK)
     return st;
   const uint64_t* sorted = in_alt ? w.alt : w.recs;
   return scan::fused_dev<scan::kSum>(w.cnt + kNValid, count, MaskLoad{sorted, w.cnt + kNValid},

@@ -60,6 +60,7 @@ register({
     "vx_dag_get_voxels": (_i32, [_P, _P, _u32, _P, _P]),
     "vx_dag_raycast": (_i32, [_P, _P, _u32, _P, _P]),
     "vx_dag_export_canonical": (_i32, [_P, _P, _u64, C.POINTER(_u64)]),
+    "vx_dag_export_canonical_device": (_i32, [_P, _P, _u64, C.POINTER(_u64), _P]),
     "vx_dag_collect_garbage": (_i32, [_P, _P, _P]),
     "vx_dag_read_pools": (_i32, [_P, _P, _u64, C.POINTER(_u64), _P, _u64, C.POINTER(_u64)]),
     "vx_brush_voxel_ops": (_i32, [_P, _u32, _u32, C.POINTER(_u64)]),
@@ -72,6 +73,7 @@ register({
     "vx_batcher_destroy": (_i32, [_P]),
     "vx_batcher_info": (_i32, [_P, C.POINTER(_u64), C.POINTER(_i32)]),
     "vx_batcher_run": (_i32, [_P, _P, _P, _P, _u32, C.POINTER(_u64)]),
+    "vx_batcher_layout": (_i32, [_P, C.POINTER(_u32), C.POINTER(_u32)]),
     "vx_dag_profile_enable": (_i32, [_P, _i32]),
     "vx_dag_profile_read": (_i32, [_P, _P]),
     "vx_dag_profile_kernel_mask": (_i32, [_P, _u64]),
@@ -280,10 +282,20 @@ class HashDagGpu:
         return npool, lpool
 
     def export(self) -> np.ndarray:
+        """Canonical export (host array of u32 words)."""
         n = C.c_uint64()
         check(lib().vx_dag_export_canonical(self.h, None, 0, C.byref(n)), "export size")
         out = np.empty(n.value, np.uint32)
         check(lib().vx_dag_export_canonical(self.h, out.ctypes.data_as(_P), n.value, C.byref(n)), "export")
+        return out
+
+    def export_device(self, stream=None) -> torch.Tensor:
+        """Canonical export left in device memory (int32 tensor of the u32 words)."""
+        n = C.c_uint64()
+        s = _stream(stream)
+        check(lib().vx_dag_export_canonical_device(self.h, None, 0, C.byref(n), s), "export size")
+        out = torch.empty(n.value, dtype=torch.int32, device="cuda")
+        check(lib().vx_dag_export_canonical_device(self.h, _ptr(out), n.value, C.byref(n), s), "export")
         return out
 
 
@@ -361,6 +373,12 @@ class Batcher:
         b, g = C.c_uint64(), C.c_int32()
         check(lib().vx_batcher_info(self.h, C.byref(b), C.byref(g)), "vx_batcher_info")
         return b.value, bool(g.value)
+
+    def layout(self) -> tuple[int, int]:
+        """(key bits, index bits) of the record sort; index bits 0 = (key, index) pairs."""
+        k, i = C.c_uint32(), C.c_uint32()
+        check(lib().vx_batcher_layout(self.h, C.byref(k), C.byref(i)), "vx_batcher_layout")
+        return k.value, i.value
 
     def run(self, batches) -> int:
         """batches: list of (scene index, brush array). Returns candidates."""

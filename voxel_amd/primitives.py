@@ -113,6 +113,9 @@ def sort_keys(keys, key_bits=None, workspace=None, stream=None):
     return sort_by_key(keys, None, key_bits, workspace, stream)[0]
 
 
+SORT_KEYS_ONLY = 0x100  # or'ed into key_bytes: the tuning row of sorts without values
+
+
 def set_sort_tuning(key_bytes: int, radix_bits: int, keys_per_thread: int, block: int) -> None:
     check(lib().vx_sort_set_tuning(key_bytes, radix_bits, keys_per_thread, block), "vx_sort_set_tuning")
 
