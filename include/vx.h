@@ -56,6 +56,14 @@ vx_status vx_exclusive_max_u32(uint32_t* data, uint32_t n, void* ws,
                                size_t ws_bytes, void* stream);
 vx_status vx_inclusive_max_u32(uint32_t* data, uint32_t n, void* ws,
                                size_t ws_bytes, void* stream);
+/* The same scans of u64 elements (scan<T> with T = uint64_t, cuda/scan/scan.cuh:79-151;
+ * u64 wraparound for sums, identity 0 for max, ops.cuh:22-36); the workspace
+ * is sized by vx_scan_workspace_size_u64. */
+vx_status vx_scan_workspace_size_u64(uint32_t n, size_t* bytes);
+vx_status vx_exclusive_sum_u64(uint64_t* data, uint32_t n, void* ws, size_t ws_bytes, void* stream);
+vx_status vx_inclusive_sum_u64(uint64_t* data, uint32_t n, void* ws, size_t ws_bytes, void* stream);
+vx_status vx_exclusive_max_u64(uint64_t* data, uint32_t n, void* ws, size_t ws_bytes, void* stream);
+vx_status vx_inclusive_max_u64(uint64_t* data, uint32_t n, void* ws, size_t ws_bytes, void* stream);
 
 /* Scan tile shape (the reference picks it by template,
  * cuda/scan/block_based.cuh:13-61): workgroup `block` in {256, 512, 1024} and
@@ -73,9 +81,10 @@ vx_status vx_compact_u32(const uint32_t* in, const uint32_t* flags, uint32_t n,
                          size_t ws_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------
- * Sort. Replaces algo::cuda::sort::{sort_keys, sort_by_key} with one u32
- * value array and required_[sort_by_key_]workspace_size (cuda/sort/sort.cuh:100-167),
- * backed by a onesweep LSD radix sort (cuda/sort/radix/onesweep/host.cuh:12-205).
+ * Sort. Replaces algo::cuda::sort::{sort_keys, sort_by_key} for u32 / u64
+ * keys with at most one u32 value array, and required_[sort_by_key_]workspace_size
+ * (cuda/sort/sort.cuh:100-167; other layouts: vx_sort_by_key below).
+ * Backed by a onesweep LSD radix sort (cuda/sort/radix/onesweep/host.cuh:12-205).
  * In place, stable, ordered by the low key_bits bits (1..32 for u32 keys,
  * 1..64 for u64 keys). n <= 1 needs no workspace. n must be < 2^30
  * (cuda/sort/radix/onesweep/host.cuh:85-86). vals may be NULL for keys only.
@@ -89,6 +98,27 @@ vx_status vx_sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t n,
 vx_status vx_sort_pairs_u64(uint64_t* keys, uint32_t* vals, uint32_t n,
                             uint32_t key_bits, void* ws, size_t ws_bytes,
                             void* stream);
+
+/* General sort_by_key<Key, Values...> (cuda/sort/sort.cuh:118-167 with
+ * value_arrays_t, cuda/sort/value_arrays.cuh:12-140): keys of key_bytes =
+ * 4 * W bytes, W = 1..16 u32 words (u32, u64 and UIntKey<W> layouts,
+ * words[0] least significant, cuda/sort/uint_key.cuh:15-66), ordered stably
+ * by their low key_bits bits; any number of value arrays of any element size
+ * (e.g. the u64 voxel masks of build_leaf_masks/pipeline.cuh:94-96 or struct
+ * records) are permuted with the keys. In place. n <= 1 needs no workspace;
+ * NULL keys or a NULL array with n > 0, key_bits outside 1..8*key_bytes,
+ * n >= 2^30 or a short workspace is VX_INVALID_VALUE. u32/u64 keys with at
+ * most one 4-byte array take the direct (key, value) path; other layouts
+ * sort (key, record index) and gather every array through the order. */
+typedef struct vx_value_array {
+  void* data;          /* device array of n elements */
+  uint64_t elem_bytes; /* bytes per element */
+} vx_value_array;
+vx_status vx_sort_by_key_workspace_size(uint32_t n, uint32_t key_bytes, uint32_t key_bits,
+                                        const vx_value_array* arrays, uint32_t n_arrays, size_t* bytes);
+vx_status vx_sort_by_key(void* keys, uint32_t key_bytes, uint32_t n, uint32_t key_bits,
+                         const vx_value_array* arrays, uint32_t n_arrays, void* ws, size_t ws_bytes,
+                         void* stream);
 
 /* Sort tuning (the reference selects these by template arguments,
  * cuda/sort/radix.cuh:55-68). radix_bits in {4, 8} with keys_per_thread in

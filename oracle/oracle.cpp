@@ -606,6 +606,46 @@ void vxo_inclusive_max_u32(const uint32_t* in, uint64_t n, uint32_t* out) {
   for (uint64_t i = 0; i < n; ++i) { acc = std::max(acc, in[i]); out[i] = acc; }
 }
 
+// u64 forms (scan<T> with T = uint64_t, cuda/scan/scan.cuh:79-151): sums wrap
+// modulo 2^64, max starts from 0 (scan/ops.cuh:22-36).
+void vxo_exclusive_sum_u64(const uint64_t* in, uint64_t n, uint64_t* out) {
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < n; ++i) { const uint64_t v = in[i]; out[i] = acc; acc += v; }
+}
+void vxo_inclusive_sum_u64(const uint64_t* in, uint64_t n, uint64_t* out) {
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < n; ++i) { acc += in[i]; out[i] = acc; }
+}
+void vxo_exclusive_max_u64(const uint64_t* in, uint64_t n, uint64_t* out) {
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < n; ++i) { const uint64_t v = in[i]; out[i] = acc; acc = std::max(acc, v); }
+}
+void vxo_inclusive_max_u64(const uint64_t* in, uint64_t n, uint64_t* out) {
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < n; ++i) { acc = std::max(acc, in[i]); out[i] = acc; }
+}
+
+// Stable order of n keys of `words` u32 words (words[0] least significant,
+// cuda/sort/uint_key.cuh:15-66) by their low key_bits bits:
+// std::stable_sort of the indices (sort_by_key semantics, cuda/sort/sort.cuh:118-138).
+void vxo_sort_order(const uint32_t* keys, uint32_t words, uint64_t n, uint32_t key_bits, uint32_t* order) {
+  std::vector<uint32_t> mask(words, 0u);
+  for (uint32_t w = 0; w < words; ++w) {
+    const int64_t b = int64_t(key_bits) - 32 * int64_t(w);
+    mask[w] = b >= 32 ? 0xffffffffu : (b <= 0 ? 0u : ((1u << b) - 1));
+  }
+  std::vector<uint32_t> idx(n);
+  for (uint64_t i = 0; i < n; ++i) idx[i] = uint32_t(i);
+  std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+    for (uint32_t w = words; w-- > 0;) {
+      const uint32_t x = keys[uint64_t(a) * words + w] & mask[w], y = keys[uint64_t(b) * words + w] & mask[w];
+      if (x != y) return x < y;
+    }
+    return false;
+  });
+  for (uint64_t i = 0; i < n; ++i) order[i] = idx[i];
+}
+
 }  // extern "C"
 
 // std::stable_sort of (key, value) pairs by the low `bits` key bits
@@ -641,6 +681,16 @@ int32_t vxo_terrain_height(const vxo_terrain* t, uint32_t x, uint32_t z) {
   while (w <= std::max(x, z)) w *= 2;
   TerrainEval ev(*t, w);
   return ev.height(x, z);
+}
+
+// Heights of n columns (one lattice for all of them).
+void vxo_terrain_heights(const vxo_terrain* t, const uint32_t* xs, const uint32_t* zs, uint64_t n, int32_t* out) {
+  uint32_t m = 0;
+  for (uint64_t i = 0; i < n; ++i) m = std::max(m, std::max(xs[i], zs[i]));
+  uint32_t w = std::max(1u, t->period);
+  while (w <= m) w *= 2;
+  TerrainEval ev(*t, w);
+  for (uint64_t i = 0; i < n; ++i) out[i] = ev.height(xs[i], zs[i]);
 }
 
 void vxo_gen_brushes(uint64_t seed, uint64_t stream, uint32_t n, uint32_t world, uint32_t mix, vxo_brush* out) {

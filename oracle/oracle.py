@@ -71,10 +71,16 @@ def lib():
             "vxo_inclusive_sum_u32": (None, [P, u64, P]),
             "vxo_exclusive_max_u32": (None, [P, u64, P]),
             "vxo_inclusive_max_u32": (None, [P, u64, P]),
+            "vxo_exclusive_sum_u64": (None, [P, u64, P]),
+            "vxo_inclusive_sum_u64": (None, [P, u64, P]),
+            "vxo_exclusive_max_u64": (None, [P, u64, P]),
+            "vxo_inclusive_max_u64": (None, [P, u64, P]),
+            "vxo_sort_order": (None, [P, u32, u64, u32, P]),
             "vxo_sort_pairs_u32": (None, [P, P, u64, u32]),
             "vxo_sort_pairs_u64": (None, [P, P, u64, u32]),
             "vxo_set_contains_u64": (None, [P, u64, P, u64, P]),
             "vxo_terrain_height": (i32, [P, u32, u32]),
+            "vxo_terrain_heights": (None, [P, P, P, u64, P]),
             "vxo_gen_brushes": (None, [u64, u64, u32, u32, u32, P]),
             "vxo_brush_voxels": (u64, [P, u32]),
             "vxo_dag_create": (P, [u32, C.c_int]),
@@ -178,6 +184,14 @@ def terrain_height(t: Terrain, x, z):
     return lib().vxo_terrain_height(C.byref(t), x, z)
 
 
+def terrain_heights(t: Terrain, xs, zs) -> np.ndarray:
+    xs = np.ascontiguousarray(xs, np.uint32)
+    zs = np.ascontiguousarray(zs, np.uint32)
+    out = np.empty(xs.size, np.int32)
+    lib().vxo_terrain_heights(C.byref(t), _ptr(xs), _ptr(zs), xs.size, _ptr(out))
+    return out
+
+
 # ---- primitives ---------------------------------------------------------------
 def exclusive_sum(a):
     a = np.ascontiguousarray(a, np.uint32)
@@ -205,6 +219,48 @@ def inclusive_max(a):
     out = np.empty_like(a)
     lib().vxo_inclusive_max_u32(_ptr(a), a.size, _ptr(out))
     return out
+
+
+def _scan64(name, a):
+    a = np.ascontiguousarray(a, np.uint64)
+    out = np.empty_like(a)
+    getattr(lib(), name)(_ptr(a), a.size, _ptr(out))
+    return out
+
+
+def exclusive_sum_u64(a):
+    return _scan64("vxo_exclusive_sum_u64", a)
+
+
+def inclusive_sum_u64(a):
+    return _scan64("vxo_inclusive_sum_u64", a)
+
+
+def exclusive_max_u64(a):
+    return _scan64("vxo_exclusive_max_u64", a)
+
+
+def inclusive_max_u64(a):
+    return _scan64("vxo_inclusive_max_u64", a)
+
+
+def sort_order(key_words: np.ndarray, key_bits: int) -> np.ndarray:
+    """Stable order of keys given as an (n, W) u32 array (column 0 least
+    significant) by their low key_bits bits."""
+    k = np.ascontiguousarray(key_words, np.uint32)
+    if k.ndim == 1:
+        k = k.reshape(-1, 1)
+    order = np.empty(k.shape[0], np.uint32)
+    lib().vxo_sort_order(_ptr(k), k.shape[1], k.shape[0], key_bits, _ptr(order))
+    return order
+
+
+def sort_by_key(key_words: np.ndarray, arrays, key_bits: int):
+    """sort_by_key: keys (n,) or (n, W) u32 words and any arrays (first axis
+    n), permuted by the stable order; keys keep their shape."""
+    order = sort_order(key_words, key_bits)
+    k = np.ascontiguousarray(key_words, np.uint32)
+    return k[order], [np.asarray(a)[order] for a in arrays]
 
 
 def sort_pairs(keys, vals=None, key_bits=None):

@@ -28,3 +28,30 @@ def test_voxel_op_count_known_answers():
     b[2] = (100, 100, 100, 2, 0, 0, 0, 1)   # r=2: 33 voxels
     assert brush_voxel_ops(b, 64) == 7 + 60 + 0
     assert brush_voxel_ops(b, 128) == 7 + 60 + 33
+
+
+def test_sized_batches_match_workload_definition():
+    """Batching / serving requests (vx_gen_sized_batch): 1..max brushes, 50%
+    spheres / 50% cubes, 60/40 add/remove, and a voxel-op count whose log2 is
+    uniform over [lo, hi] (2^8..2^20 for batching) up to the rounding of the
+    brush sizes."""
+    from voxel_amd.dag import brush_voxel_ops, gen_sized_batch
+    n, lo, hi, max_brushes, world = 1500, 8, 20, 16, 1 << 15
+    logs, counts, kinds, adds, scenes = [], [], [], [], []
+    for i in range(n):
+        br, s = gen_sized_batch(3, 6, i, lo, hi, max_brushes, world, 8)
+        counts.append(len(br))
+        kinds += br["kind"].tolist()
+        adds += (br["material"] > 0).tolist()
+        scenes.append(s)
+        logs.append(np.log2(brush_voxel_ops(br, world)))
+    counts, kinds, adds, logs = map(np.asarray, (counts, kinds, adds, logs))
+    assert counts.min() == 1 and counts.max() == max_brushes
+    assert sorted(set(scenes)) == list(range(8))
+    m = len(kinds)
+    assert abs(kinds.mean() - 0.5) < 5 * np.sqrt(0.25 / m)
+    assert abs(adds.mean() - 0.6) < 5 * np.sqrt(0.24 / m)
+    q = np.quantile(logs, [0.1, 0.5, 0.9])
+    want = [lo + 0.1 * (hi - lo), (lo + hi) / 2, lo + 0.9 * (hi - lo)]
+    assert all(abs(a - b) < 1.0 for a, b in zip(q, want)), q
+    assert logs.min() > lo - 2 and logs.max() < hi + 1

@@ -234,3 +234,128 @@ def test_oracle_raycast_pins():
             assert h[0] == 0
     out = t.raycast(np.array([[-1, 5, 5, 1, 0, 0, 100, 0], [3, 3, 3, 0, 0, 0, 100, 0]]))
     assert out[0, 0] == 0 and (out[1, 0] == 0) == (t.get(3, 3, 3) == 0)
+
+
+# ---- pins of the scans' max forms, the u64 scans and the general sort order ----
+def test_exclusive_max_closed_form():
+    a = O.gen_u32(321, O.STREAM_SCAN, 100_000, 0, 0xFFFFFFFF)
+    want = np.concatenate([[0], np.maximum.accumulate(a)[:-1]]).astype(np.uint32)
+    assert np.array_equal(O.exclusive_max(a), want)
+    assert O.exclusive_max([3, 1, 4, 1, 5, 9, 2, 6, 5]).tolist() == [0, 3, 3, 4, 4, 5, 9, 9, 9]
+
+
+def test_u64_scans_closed_forms():
+    lo = O.gen_u32(11, O.STREAM_SCAN, 50_000, 0, 0xFFFFFFFF).astype(np.uint64)
+    hi = O.gen_u32(12, O.STREAM_SCAN, 50_000, 0, 0xFFFFFFFF).astype(np.uint64)
+    a = (hi << np.uint64(32)) | lo
+    inc = np.cumsum(a, dtype=np.uint64)  # numpy integer sums wrap modulo 2^64
+    assert np.array_equal(O.inclusive_sum_u64(a), inc)
+    assert np.array_equal(O.exclusive_sum_u64(a), np.concatenate([np.zeros(1, np.uint64), inc[:-1]]))
+    mx = np.maximum.accumulate(a)
+    assert np.array_equal(O.inclusive_max_u64(a), mx)
+    assert np.array_equal(O.exclusive_max_u64(a), np.concatenate([np.zeros(1, np.uint64), mx[:-1]]))
+    # test_cuda_scan.cu:716-813: odd numbers sum to squares; (37 i) mod 1021 over 2049 elements.
+    odd = np.arange(1, 34, 2, dtype=np.uint64)
+    assert O.inclusive_sum_u64(odd).tolist() == [(k + 1) ** 2 for k in range(17)]
+    b = (np.arange(2049, dtype=np.uint64) * 37) % 1021
+    assert O.exclusive_max_u64(b)[0] == 0 and O.inclusive_max_u64(b)[-1] == 1020
+
+
+def test_sort_order_matches_lexsort():
+    n, words = 20_000, 3
+    k = O.gen_u32(5, O.STREAM_KEYS, n * words, 0, 15).reshape(n, words)  # many ties
+    for bits in (96, 70, 33, 4):
+        masked = [k[:, w] & np.uint32((1 << max(0, min(32, bits - 32 * w))) - 1) for w in range(words)]
+        want = np.lexsort([np.arange(n)] + masked)  # last key primary, index breaks ties
+        assert np.array_equal(O.sort_order(k, bits), want), bits
+
+
+def test_sort_order_reference_fixed_word_keys():
+    keys = np.array([[0, 0, 1], [3, 0, 0], [0, 2, 0], [0xFFFFFFFF] * 3, [0, 0, 0], [1, 0, 1]], np.uint32)
+    assert O.sort_order(keys, 96).tolist() == [4, 1, 2, 0, 5, 3]
+
+
+# ---- the workload generators against the workload definitions -----------------
+def _lattice(seed, i, j):
+    """Lattice value of octave 0 at lattice point (i, j), in 1/65536 units."""
+    r = O.lib().vxo_rand(seed, 16, (i << 32) | j)
+    return (r >> 47) - 65536
+
+
+def test_terrain_at_lattice_points_and_midpoints():
+    """One octave: at lattice points the height is base + floor(amp * v) with
+    v the lattice value in [-1, 1); halfway between two lattice points along x
+    the smoothstep weight is exactly 1/2."""
+    seed, base, amp, period = 9, 512, 128, 64
+    t = O.Terrain(seed, base, amp, period, 1)
+    ii, jj = np.meshgrid(np.arange(16), np.arange(16), indexing="ij")
+    h = O.terrain_heights(t, (ii * period).ravel(), (jj * period).ravel())
+    want = [base + ((amp * _lattice(seed, int(i), int(j))) >> 16) for i, j in zip(ii.ravel(), jj.ravel())]
+    assert h.tolist() == want
+    mid = O.terrain_heights(t, (ii * period + period // 2).ravel(), (jj * period).ravel())
+    want = []
+    for i, j in zip(ii.ravel(), jj.ravel()):
+        a, b = _lattice(seed, int(i), int(j)), _lattice(seed, int(i) + 1, int(j))
+        want.append(base + ((amp * (a + ((b - a) >> 1))) >> 16))
+    assert mid.tolist() == want
+
+
+def test_terrain_monotone_between_lattice_points():
+    """One octave, z on the lattice: the smoothstep is monotone, so heights
+    between two lattice points stay between (and move towards) their values."""
+    seed, period = 4, 64
+    t = O.Terrain(seed, 256, 64, period, 1)
+    for i in range(6):
+        xs = np.arange(i * period, (i + 1) * period + 1)
+        h = O.terrain_heights(t, xs, np.full(xs.size, 3 * period))
+        d = np.diff(h.astype(np.int64))
+        assert (d >= 0).all() or (d <= 0).all()
+
+
+def test_terrain_config0_bounds_and_scale():
+    """Config-0 heightfield (base 256, amplitude 64, periods 64 / 2^o, weights
+    0.5^o, o < 4): every height within base +- 1.875 * amp (+1 for the floor),
+    the range well used, and amplitude 0 gives the flat base."""
+    rng = np.random.default_rng(1)
+    xs, zs = rng.integers(0, 512, 200_000), rng.integers(0, 512, 200_000)
+    h = O.terrain_heights(O.terrain_for_world(512, 1), xs, zs)
+    assert h.min() >= 256 - 120 - 1 and h.max() <= 256 + 120
+    assert h.max() - h.min() > 64
+    flat = O.terrain_heights(O.Terrain(1, 256, 0, 64, 4), xs[:1000], zs[:1000])
+    assert (flat == 256).all()
+    # x32 scale (config 2): the same field stretched, heights x32 (to the floor).
+    big = O.terrain_for_world(16384, 3)
+    assert (big.base, big.amp, big.period, big.octaves) == (8192, 2048, 2048, 4)
+
+
+def _binom_ok(k, n, p, sigmas=5.0):
+    return abs(k - n * p) <= sigmas * np.sqrt(n * p * (1 - p))
+
+
+def test_brush_generator_matches_workload_definition():
+    """mix 0 (config 0): spheres, r in [4, 16], 50/50 add/remove; mix 1
+    (configs 2-4): 50% spheres r in [8, 64], 50% boxes with half extents in
+    [4, 48], 60/40 add/remove; added material in [1, 255]; centres uniform in
+    the world — ranges hit at both ends, splits within 5 sigma."""
+    n, world = 100_000, 16384
+    b0 = O.gen_brushes(1, O.STREAM_BRUSH, n, world, 0)
+    assert (b0["kind"] == 0).all()
+    assert b0["hx"].min() == 4 and b0["hx"].max() == 16
+    assert _binom_ok(int((b0["material"] > 0).sum()), n, 0.5)
+    b1 = O.gen_brushes(3, O.STREAM_BRUSH, n, world, 1)
+    sph, box = b1[b1["kind"] == 0], b1[b1["kind"] == 1]
+    assert len(sph) + len(box) == n and _binom_ok(len(sph), n, 0.5)
+    assert sph["hx"].min() == 8 and sph["hx"].max() == 64
+    for ax in ("hx", "hy", "hz"):
+        assert box[ax].min() == 4 and box[ax].max() == 48
+    assert _binom_ok(int((b1["material"] > 0).sum()), n, 0.6)
+    for b in (b0, b1):
+        add = b[b["material"] > 0]["material"]
+        assert add.min() == 1 and add.max() == 255
+        for ax in ("cx", "cy", "cz"):
+            assert b[ax].min() >= 0 and b[ax].max() <= world - 1
+            counts = np.bincount(b[ax] * 8 // world, minlength=8)
+            assert all(_binom_ok(int(c), n, 1 / 8) for c in counts)
+    # Radii uniform: each value of [8, 64] about equally often.
+    rc = np.bincount(sph["hx"], minlength=65)[8:65]
+    assert all(_binom_ok(int(c), len(sph), 1 / 57) for c in rc)

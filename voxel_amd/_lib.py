@@ -73,6 +73,13 @@ SIGNATURES = {
     "vx_gen_morton": (_i32, [_u64, _u64, _u32, _u32, _P, _P]),
     "vx_gen_unique_u64": (_i32, [_u64, _u64, _u64, _u32, _P, _P]),
     "vx_iota_u32": (_i32, [_P, _u32, _P]),
+    "vx_scan_workspace_size_u64": (_i32, [_u32, _psz]),
+    "vx_exclusive_sum_u64": (_i32, [_P, _u32, _P, _sz, _P]),
+    "vx_inclusive_sum_u64": (_i32, [_P, _u32, _P, _sz, _P]),
+    "vx_exclusive_max_u64": (_i32, [_P, _u32, _P, _sz, _P]),
+    "vx_inclusive_max_u64": (_i32, [_P, _u32, _P, _sz, _P]),
+    "vx_sort_by_key_workspace_size": (_i32, [_u32, _u32, _u32, _P, _u32, _psz]),
+    "vx_sort_by_key": (_i32, [_P, _u32, _u32, _u32, _P, _u32, _P, _sz, _P]),
 }
 
 _lib = None

@@ -240,6 +240,171 @@ vx_status run(uint32_t* data, uint32_t n, bool inclusive, Op op, void* ws, hipSt
   return inclusive ? launch<kMax, true>(data, n, ws, s) : launch<kMax, false>(data, n, ws, s);
 }
 
+// ---- u64 scans -----------------------------------------------------------------
+// Same structure as the u32 scan (wave-contiguous segments, DPP wave scans,
+// a wave-wide look-back window), for 64-bit elements. A u64 value leaves no
+// room for status bits in the record, so a tile publishes {value, flag}: the
+// value first, then the flag with release semantics; a reader loads the flag
+// with acquire semantics before the value it selects (vals[2t] aggregate,
+// vals[2t + 1] inclusive prefix).
+namespace {
+
+constexpr uint32_t kBlock64 = 256;
+constexpr uint32_t kVec64 = 8;                            // uint4 (two elements) per lane and tile
+constexpr uint32_t kTile64 = kBlock64 * 2 * kVec64;       // 4096 elements, 32 KiB
+constexpr uint32_t kFlagAgg = 1, kFlagPrefix = 2;
+
+template <Op OP>
+__device__ __forceinline__ uint64_t combine64(uint64_t a, uint64_t b) {
+  if constexpr (OP == kSum) return a + b;
+  else return a > b ? a : b;
+}
+
+template <Op OP>
+__device__ __forceinline__ uint64_t wave_incl64(uint64_t v) {
+  return wave_inclusive_dpp(v, [](uint64_t a, uint64_t b) { return combine64<OP>(a, b); });
+}
+
+template <Op OP>
+__device__ uint64_t lookback64(const uint32_t* flags, const uint64_t* vals, uint32_t tile) {
+  const uint32_t lane = lane_id();
+  uint64_t excl = 0;  // identity of sum and of max over u64
+  int64_t base = int64_t(tile) - 1;
+  while (base >= 0) {
+    const int64_t idx = base - lane;
+    const uint32_t st =
+        idx >= 0 ? __hip_atomic_load(&flags[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : kFlagPrefix;
+    const uint64_t pref = ballot(st == kFlagPrefix);
+    const uint64_t inval = ballot(st == 0);
+    const uint32_t first = pref ? uint32_t(__builtin_ctzll(pref)) : 64u;
+    const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1);
+    if (inval & need) continue;
+    uint64_t mine = 0;
+    if (lane <= first && idx >= 0) mine = ld_agent(&vals[2 * idx + (st == kFlagPrefix ? 1 : 0)]);
+    const uint64_t rev = __shfl(mine, int(63 - lane), kWave);
+    excl = combine64<OP>(wave_last(wave_incl64<OP>(rev)), excl);
+    if (first < 64) break;
+    base -= 64;
+  }
+  return excl;
+}
+
+__device__ __forceinline__ void publish64(uint32_t* flag, uint64_t* val, uint64_t v, uint32_t f) {
+  st_agent(val, v);
+  __hip_atomic_store(flag, f, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <Op OP, bool INCL, bool ALIGNED>
+__global__ __launch_bounds__(kBlock64) void scan64_kernel(uint64_t* __restrict__ data, uint32_t n, uint32_t* counter,
+                                                          uint32_t* flags, uint64_t* vals) {
+  constexpr uint32_t kSeg = kWave * 2 * kVec64;
+  __shared__ uint64_t s_wave[kBlock64 / kWave];
+  __shared__ uint64_t s_prefix;
+  __shared__ uint32_t s_tile;
+  const bool multi = gridDim.x > 1;
+  if (threadIdx.x == 0) s_tile = multi ? atomicAdd(counter, 1u) : 0u;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t lane = lane_id(), wave = wave_id();
+  const uint64_t seg = uint64_t(tile) * kTile64 + uint64_t(wave) * kSeg;
+
+  uint64_t a[kVec64], b[kVec64], excl[kVec64];
+  uint64_t carry = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kVec64; ++k) {
+    const uint64_t i = seg + k * (2 * kWave) + lane * 2;
+    if (ALIGNED && i + 1 < n) {
+      const uint4 v = ld_stream(reinterpret_cast<const uint32_t*>(data + i));
+      a[k] = uint64_t(v.x) | (uint64_t(v.y) << 32);
+      b[k] = uint64_t(v.z) | (uint64_t(v.w) << 32);
+    } else {
+      a[k] = i < n ? data[i] : 0ull;
+      b[k] = i + 1 < n ? data[i + 1] : 0ull;
+    }
+    const uint64_t inc = wave_incl64<OP>(combine64<OP>(a[k], b[k]));
+    excl[k] = combine64<OP>(carry, wave_shift_up1(inc));
+    carry = combine64<OP>(carry, wave_last(inc));
+  }
+  if (lane == 0) s_wave[wave] = carry;
+  __syncthreads();
+  uint64_t before = 0, agg = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kBlock64 / kWave; ++w) {
+    const uint64_t t = s_wave[w];
+    if (w < wave) before = combine64<OP>(before, t);
+    agg = combine64<OP>(agg, t);
+  }
+  if (wave == 0) {
+    uint64_t prefix = 0;
+    if (multi) {
+      if (tile == 0) {
+        if (lane == 0) publish64(&flags[0], &vals[1], agg, kFlagPrefix);
+      } else {
+        if (lane == 0) publish64(&flags[tile], &vals[2 * size_t(tile)], agg, kFlagAgg);
+        prefix = lookback64<OP>(flags, vals, tile);
+        if (lane == 0) publish64(&flags[tile], &vals[2 * size_t(tile) + 1], combine64<OP>(prefix, agg), kFlagPrefix);
+      }
+    }
+    if (lane == 0) s_prefix = prefix;
+  }
+  __syncthreads();
+  const uint64_t prefix = combine64<OP>(s_prefix, before);
+#pragma unroll
+  for (uint32_t k = 0; k < kVec64; ++k) {
+    const uint64_t i = seg + k * (2 * kWave) + lane * 2;
+    const uint64_t e = combine64<OP>(prefix, excl[k]);
+    const uint64_t oa = INCL ? combine64<OP>(e, a[k]) : e;
+    const uint64_t ob = INCL ? combine64<OP>(combine64<OP>(e, a[k]), b[k]) : combine64<OP>(e, a[k]);
+    if (ALIGNED && i + 1 < n) {
+      st_stream(reinterpret_cast<uint32_t*>(data + i),
+                make_uint4(uint32_t(oa), uint32_t(oa >> 32), uint32_t(ob), uint32_t(ob >> 32)));
+    } else {
+      if (i < n) data[i] = oa;
+      if (i + 1 < n) data[i + 1] = ob;
+    }
+  }
+}
+
+template <Op OP, bool INCL>
+vx_status launch64(uint64_t* data, uint32_t n, void* ws, hipStream_t s) {
+  const uint32_t tiles = ceil_div(n, kTile64);
+  uint32_t* counter = nullptr;
+  uint32_t* flags = nullptr;
+  uint64_t* vals = nullptr;
+  if (tiles > 1) {
+    Carver c(ws);
+    counter = c.take<uint32_t>(64);
+    flags = c.take<uint32_t>(tiles);
+    vals = c.take<uint64_t>(2 * size_t(tiles));
+    const vx_status st =
+        fill_u32(counter, (reinterpret_cast<char*>(flags + tiles) - reinterpret_cast<char*>(counter)) / 4, 0, s);
+    if (st != VX_OK) return st;
+  }
+  if (reinterpret_cast<uintptr_t>(data) & 15)
+    scan64_kernel<OP, INCL, false><<<tiles, kBlock64, 0, s>>>(data, n, counter, flags, vals);
+  else
+    scan64_kernel<OP, INCL, true><<<tiles, kBlock64, 0, s>>>(data, n, counter, flags, vals);
+  return last_launch();
+}
+
+}  // namespace
+
+size_t workspace_bytes_u64(uint32_t n) {
+  const uint32_t tiles = ceil_div(n, kTile64);
+  if (tiles <= 1) return 0;
+  Carver c(nullptr);
+  c.take<uint32_t>(64);
+  c.take<uint32_t>(tiles);
+  c.take<uint64_t>(2 * size_t(tiles));
+  return c.size();
+}
+
+vx_status run_u64(uint64_t* data, uint32_t n, bool inclusive, Op op, void* ws, hipStream_t s) {
+  if (n == 0) return VX_OK;
+  if (op == kSum) return inclusive ? launch64<kSum, true>(data, n, ws, s) : launch64<kSum, false>(data, n, ws, s);
+  return inclusive ? launch64<kMax, true>(data, n, ws, s) : launch64<kMax, false>(data, n, ws, s);
+}
+
 // ---- generic fused scan (transform -> scan -> post-scan) ---------------------
 namespace {
 
@@ -317,6 +482,34 @@ extern "C" vx_status vx_exclusive_max_u32(uint32_t* d, uint32_t n, void* ws, siz
 }
 extern "C" vx_status vx_inclusive_max_u32(uint32_t* d, uint32_t n, void* ws, size_t b, void* s) {
   return scan_entry(d, n, ws, b, s, true, vx::scan::kMax);
+}
+
+extern "C" vx_status vx_scan_workspace_size_u64(uint32_t n, size_t* bytes) {
+  if (!bytes) return VX_INVALID_VALUE;
+  *bytes = vx::scan::workspace_bytes_u64(n);
+  return VX_OK;
+}
+
+static vx_status scan_entry_u64(uint64_t* data, uint32_t n, void* ws, size_t ws_bytes, void* stream, bool incl,
+                                Op op) {
+  if (!data && n) return VX_INVALID_VALUE;
+  if ((reinterpret_cast<uintptr_t>(data) & 7) != 0) return VX_INVALID_VALUE;
+  const size_t need = vx::scan::workspace_bytes_u64(n);
+  if (need && (!ws || ws_bytes < need)) return VX_INVALID_VALUE;
+  return vx::scan::run_u64(data, n, incl, op, ws, static_cast<hipStream_t>(stream));
+}
+
+extern "C" vx_status vx_exclusive_sum_u64(uint64_t* d, uint32_t n, void* ws, size_t b, void* s) {
+  return scan_entry_u64(d, n, ws, b, s, false, vx::scan::kSum);
+}
+extern "C" vx_status vx_inclusive_sum_u64(uint64_t* d, uint32_t n, void* ws, size_t b, void* s) {
+  return scan_entry_u64(d, n, ws, b, s, true, vx::scan::kSum);
+}
+extern "C" vx_status vx_exclusive_max_u64(uint64_t* d, uint32_t n, void* ws, size_t b, void* s) {
+  return scan_entry_u64(d, n, ws, b, s, false, vx::scan::kMax);
+}
+extern "C" vx_status vx_inclusive_max_u64(uint64_t* d, uint32_t n, void* ws, size_t b, void* s) {
+  return scan_entry_u64(d, n, ws, b, s, true, vx::scan::kMax);
 }
 
 extern "C" vx_status vx_scan_set_tuning(uint32_t block, uint32_t vectors_per_lane) {

@@ -15,6 +15,8 @@
 namespace vx {
 namespace scan {
 
+size_t workspace_bytes_u64(uint32_t n);
+vx_status run_u64(uint64_t* data, uint32_t n, bool inclusive, Op op, void* ws, hipStream_t s);
 vx_status generic_inplace(uint32_t* data, uint32_t n, bool inclusive, Op op, void* ws, hipStream_t s);
 vx_status compact(const uint32_t* in, const uint32_t* flags, uint32_t n, uint32_t* out, uint32_t* d_count, void* ws,
                   hipStream_t s);

@@ -36,36 +36,45 @@ def _workspace(nbytes: int, workspace, device):
 
 
 # ---- scan ---------------------------------------------------------------------
-def required_workspace_size(count: int) -> int:
-    """Bytes of workspace the scan needs for `count` elements (0 = none)."""
+def required_workspace_size(count: int, elem_bytes: int = 4) -> int:
+    """Bytes of workspace the scan needs for `count` elements of 4 or 8 bytes
+    (required_workspace_size<T>, cuda/scan/scan.cuh:153-161; 0 = none)."""
     n = C.c_size_t()
-    check(lib().vx_scan_workspace_size(count, C.byref(n)), "vx_scan_workspace_size")
+    if elem_bytes == 8:
+        check(lib().vx_scan_workspace_size_u64(count, C.byref(n)), "vx_scan_workspace_size_u64")
+    else:
+        check(lib().vx_scan_workspace_size(count, C.byref(n)), "vx_scan_workspace_size")
     return n.value
 
 
-def _scan(fn, data: torch.Tensor, workspace, stream):
-    if data.dtype not in (torch.int32, torch.uint32):
-        raise TypeError("scan supports 32-bit unsigned data")
+def _scan(name, data: torch.Tensor, workspace, stream):
+    if data.dtype in (torch.int32, torch.uint32):
+        eb = 4
+    elif data.dtype in (torch.int64, torch.uint64):
+        eb = 8
+    else:
+        raise TypeError("scan supports 32- and 64-bit unsigned data")
+    fn = getattr(lib(), f"vx_{name}_u{8 * eb}")
     n = data.numel()
-    ws, wsb = _workspace(required_workspace_size(n), workspace, data.device)
+    ws, wsb = _workspace(required_workspace_size(n, eb), workspace, data.device)
     check(fn(_ptr(data), n, _ptr(ws), wsb, _stream(stream)), fn.__name__)
     return data
 
 
 def exclusive_sum(data, workspace=None, stream=None):
-    return _scan(lib().vx_exclusive_sum_u32, data, workspace, stream)
+    return _scan("exclusive_sum", data, workspace, stream)
 
 
 def inclusive_sum(data, workspace=None, stream=None):
-    return _scan(lib().vx_inclusive_sum_u32, data, workspace, stream)
+    return _scan("inclusive_sum", data, workspace, stream)
 
 
 def exclusive_max(data, workspace=None, stream=None):
-    return _scan(lib().vx_exclusive_max_u32, data, workspace, stream)
+    return _scan("exclusive_max", data, workspace, stream)
 
 
 def inclusive_max(data, workspace=None, stream=None):
-    return _scan(lib().vx_inclusive_max_u32, data, workspace, stream)
+    return _scan("inclusive_max", data, workspace, stream)
 
 
 def compact(values: torch.Tensor, flags: torch.Tensor, workspace=None, stream=None):
@@ -96,16 +105,67 @@ def required_sort_workspace_size(count: int, key_bytes: int, key_bits: int | Non
     return n.value
 
 
-def sort_by_key(keys: torch.Tensor, values: torch.Tensor | None, key_bits: int | None = None, workspace=None,
-                stream=None):
-    """Stable in-place sort of keys (and one u32 value array) by the low key_bits bits."""
-    kbytes, fn = _key_info(keys)
+class ValueArray(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("elem_bytes", C.c_uint64)]
+
+
+def _elem_bytes(t: torch.Tensor) -> int:
+    """Bytes of one element of a value array: a row of an (n, ...) tensor."""
+    k = 1
+    for d in t.shape[1:]:
+        k *= d
+    return t.element_size() * k
+
+
+def _key_layout(keys: torch.Tensor):
+    """(key bytes, count): 1-D 32/64-bit keys, or (n, W) int32 words (UIntKey<W>)."""
+    if keys.dim() == 1:
+        if keys.element_size() not in (4, 8):
+            raise TypeError("sort keys must be 32- or 64-bit")
+        return keys.element_size(), keys.shape[0]
+    if keys.dim() == 2 and keys.element_size() == 4:
+        return 4 * keys.shape[1], keys.shape[0]
+    raise TypeError("multi-word keys are (n, W) 32-bit tensors")
+
+
+def _arrays(values, n):
+    if values is None:
+        return []
+    vs = [values] if isinstance(values, torch.Tensor) else list(values)
+    for v in vs:
+        if v.shape[0] != n or not v.is_contiguous():
+            raise ValueError("value arrays must be contiguous with one element per key")
+    return vs
+
+
+def required_sort_by_key_workspace_size(count: int, key_bytes: int, key_bits: int, elem_bytes=()) -> int:
+    arr = (ValueArray * max(1, len(elem_bytes)))(*[ValueArray(None, e) for e in elem_bytes])
+    n = C.c_size_t()
+    check(lib().vx_sort_by_key_workspace_size(count, key_bytes, key_bits, arr, len(elem_bytes), C.byref(n)),
+          "vx_sort_by_key_workspace_size")
+    return n.value
+
+
+def sort_by_key(keys: torch.Tensor, values=None, key_bits: int | None = None, workspace=None, stream=None):
+    """Stable in-place sort of keys by their low key_bits bits, permuting every
+    value array with them (sort_by_key<Key, Values...>, cuda/sort/sort.cuh:118-138).
+    keys: 1-D u32 / u64 tensor, or (n, W) u32 words (UIntKey<W>, word 0 least
+    significant). values: None, one tensor, or a sequence of tensors whose
+    first axis is n (an (n, k) tensor is one array of k-element records)."""
+    kbytes, n = _key_layout(keys)
     kb = key_bits if key_bits is not None else kbytes * 8
-    n = keys.numel()
-    if values is not None and values.numel() != n:
-        raise ValueError("values must match keys")
-    ws, wsb = _workspace(required_sort_workspace_size(n, kbytes, kb, values is not None), workspace, keys.device)
-    check(fn(_ptr(keys), _ptr(values), n, kb, _ptr(ws), wsb, _stream(stream)), fn.__name__)
+    vs = _arrays(values, n)
+    if kbytes in (4, 8) and keys.dim() == 1 and (not vs or (len(vs) == 1 and _elem_bytes(vs[0]) == 4)):
+        fn = lib().vx_sort_pairs_u32 if kbytes == 4 else lib().vx_sort_pairs_u64
+        v = vs[0] if vs else None
+        ws, wsb = _workspace(required_sort_workspace_size(n, kbytes, kb, v is not None), workspace, keys.device)
+        check(fn(_ptr(keys), _ptr(v), n, kb, _ptr(ws), wsb, _stream(stream)), fn.__name__)
+        return keys, values
+    arr = (ValueArray * max(1, len(vs)))(*[ValueArray(v.data_ptr(), _elem_bytes(v)) for v in vs])
+    need = required_sort_by_key_workspace_size(n, kbytes, kb, [_elem_bytes(v) for v in vs])
+    ws, wsb = _workspace(need, workspace, keys.device)
+    check(lib().vx_sort_by_key(_ptr(keys), kbytes, n, kb, arr, len(vs), _ptr(ws), wsb, _stream(stream)),
+          "vx_sort_by_key")
     return keys, values
 
 
