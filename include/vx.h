@@ -65,6 +65,17 @@ vx_status vx_inclusive_sum_u64(uint64_t* data, uint32_t n, void* ws, size_t ws_b
 vx_status vx_exclusive_max_u64(uint64_t* data, uint32_t n, void* ws, size_t ws_bytes, void* stream);
 vx_status vx_inclusive_max_u64(uint64_t* data, uint32_t n, void* ws, size_t ws_bytes, void* stream);
 
+/* Exclusive sum of u32 that allocates nothing and takes no workspace: for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * Same result as vx_exclusive_sum_u32; its look-back state lives in a
+ * 512 KiB static device array of the library, one per device, held until
+ * the process ends, and is cleared by a fill kernel on `stream` at every
+ * call. n is at most 2^28 (VX_INVALID_VALUE beyond). Calls that overlap on
+ * one device (two streams, two host threads) share that array and race:
+ * order them on one stream. */
+vx_status vx_exclusive_sum_u32_noalloc(uint32_t* data, uint32_t n, void* stream);
+
 /* Scan tile shape (the reference picks it by template,
  * cuda/scan/block_based.cuh:13-61): workgroup `block` in {256, 512, 1024} and
  * uint4 loads per lane `vectors_per_lane` in {4, 8, 16} (block * 4 * vectors

@@ -73,6 +73,7 @@ SIGNATURES = {
     "vx_gen_morton": (_i32, [_u64, _u64, _u32, _u32, _P, _P]),
     "vx_gen_unique_u64": (_i32, [_u64, _u64, _u64, _u32, _P, _P]),
     "vx_iota_u32": (_i32, [_P, _u32, _P]),
+    "vx_exclusive_sum_u32_noalloc": (_i32, [_P, _u32, _P]),
     "vx_scan_workspace_size_u64": (_i32, [_u32, _psz]),
     "vx_exclusive_sum_u64": (_i32, [_P, _u32, _P, _sz, _P]),
     "vx_inclusive_sum_u64": (_i32, [_P, _u32, _P, _sz, _P]),

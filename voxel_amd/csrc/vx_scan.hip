@@ -12,6 +12,8 @@
 // streaming (nontemporal). A tile publishes its
 // aggregate, then wave 0 inspects up to 64 predecessor records per step
 // (ballot over the 64 lanes) instead of walking back one tile at a time.
+#include <mutex>
+
 #include "vx_scan.hpp"
 
 namespace vx {
@@ -454,8 +456,56 @@ vx_status compact(const uint32_t* in, const uint32_t* flags, uint32_t n, uint32_
 }  // namespace scan
 }  // namespace vx
 
+// ---- scratch for the allocation-free entry point ------------------------------
+// vx_exclusive_sum_u32_noalloc keeps its look-back state (claim counter and
+// one record per tile) in this file-scope device array instead of a caller
+// workspace: 256 bytes of counters and one 8-byte record per tile of the
+// smallest tile (4096 elements), for up to 2^28 elements — 512 KiB per
+// device, held until the process ends. Host code never passes `scratch`
+// itself: its device address, per device, comes from hipGetSymbolAddress.
+namespace {
+constexpr uint32_t kScratchMaxN = 1u << 28;
+constexpr size_t kScratchBytes = 256 + size_t(kScratchMaxN / (256 * 16)) * 8;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).
+vx_status scratch_for_current_device(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return VX_DEVICE_ERROR;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return VX_DEVICE_ERROR;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return VX_OK;
+}
+}  // namespace
+
 // ---- C ABI -------------------------------------------------------------------
 using vx::scan::Op;
+
+extern "C" vx_status vx_exclusive_sum_u32_noalloc(uint32_t* data, uint32_t n, void* stream) {
+  if (!data && n) return VX_INVALID_VALUE;
+  if (n > kScratchMaxN) return VX_INVALID_VALUE;
+  const size_t need = std::max(vx::scan::workspace_bytes(n), vx::scan::fused_workspace_bytes(n));
+  if (need > kScratchBytes) return VX_INVALID_VALUE;
+  void* ws = nullptr;
+  const vx_status st = scratch_for_current_device(&ws);
+  if (st != VX_OK) return st;
+  // The scan clears the counter and look-back records it uses with a fill
+  // kernel before its own launch.
+  return vx::scan::run(data, n, false, vx::scan::kSum, ws, static_cast<hipStream_t>(stream));
+}
 
 extern "C" vx_status vx_scan_workspace_size(uint32_t n, size_t* bytes) {
   if (!bytes) return VX_INVALID_VALUE;
