@@ -534,7 +534,7 @@ def leg_serving(V, torch, n_requests=2000, seed=20, load=0.7):
     reqs = make_requests(n_requests, len(depths), worlds, seed, 6, 18, 64)
     srv.ticks, srv.tick_sizes = 0, []
     res = serve(srv, reqs, poisson_arrivals(len(reqs), rate, seed))
-    srv.ex.close()
+    srv.close()
     for g in scenes:
         g.close()
     res.update({"arrival_rate_per_s": rate, "saturated_requests_per_s": sat["requests"] / sat["makespan_s"],

@@ -240,7 +240,8 @@ typedef struct vx_voxel { uint32_t x, y, z; } vx_voxel; /* = hash_dag_gpu::Voxel
  * around an integer centre; covers voxel (x,y,z) iff dx^2+dy^2+dz^2 <= r^2 or
  * |dx|<=hx && |dy|<=hy && |dz|<=hz. material 0 removes, 1..255 adds (the
  * occupancy DAG treats any non-zero material as "place"). Brushes of one call
- * apply in array order: per voxel, the last brush wins. */
+ * apply in array order: per voxel, the last brush wins. A kind above 1 or a
+ * material above 255 makes the call VX_INVALID_VALUE. */
 typedef struct vx_brush {
   int32_t cx, cy, cz;
   int32_t hx, hy, hz;
@@ -262,7 +263,7 @@ typedef struct vx_dag_stats {
   uint32_t root;
   uint32_t inner_depth;
   uint32_t leaf_words; /* 2 (occupancy) or 16 (materials) */
-  uint32_t reserved;
+  uint32_t last_status; /* first-error word of the scene's last direct edit call: 0 ok, 1 overflow */
 } vx_dag_stats;
 
 vx_status vx_dag_create(uint32_t inner_depth, int32_t materials, const vx_dag_capacity* cap, vx_dag** out);
@@ -316,6 +317,26 @@ vx_status vx_batcher_destroy(vx_batcher* b);
 vx_status vx_batcher_info(const vx_batcher* b, uint64_t* workspace_bytes, int32_t* graph_ready);
 vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batch, const vx_brush* const* h_brushes,
                          const uint32_t* counts, uint32_t n_batches, uint64_t* candidates);
+/* Asynchronous form of vx_batcher_run for continuous batching: enqueues the
+ * batch set on the executor's stream (brush upload, graph launch, read-back
+ * of every scene's pool usage and status) and returns at once. One
+ * submission per executor is in flight; vx_batcher_query completes it before
+ * the next (VX_INVALID_VALUE otherwise). With `after` != NULL the device work
+ * waits for `after`'s last submission (on the device: the upload still
+ * overlaps it); submissions of two executors over disjoint scene sets may run
+ * concurrently (each executor keeps its own per-scene status words), and
+ * submissions that share a scene must be ordered with `after`. */
+vx_status vx_batcher_submit(vx_batcher* b, const uint32_t* scene_of_batch, const vx_brush* const* h_brushes,
+                            const uint32_t* counts, uint32_t n_batches, uint64_t* candidates,
+                            const vx_batcher* after);
+/* Completion of the last submission: *done = 0 while the device works on it
+ * (wait != 0 blocks until it is done), else 1 and the outcome is returned:
+ * VX_OUT_OF_CAPACITY if a scene overflowed (that scene keeps its old root;
+ * scene_status[i] != 0 names it), VX_OK otherwise. scene_status (n_scenes
+ * u32) and scene_used (2 * n_scenes u64: node words, leaf words used after
+ * the submission) may be NULL. */
+vx_status vx_batcher_query(vx_batcher* b, int32_t wait, int32_t* done, uint32_t* scene_status,
+                           uint64_t* scene_used);
 /* Record layout of the executor's sort: *key_bits = scene bits + 3 bits per
  * level of the deepest scene; *index_bits > 0 when (key, record index) pack
  * into one u64 sorted by its upper bits, 0 when records sort as (u64 key,

@@ -70,3 +70,72 @@ def test_admission_by_free_pool_space_keeps_scene_order(gpu):
     admitted = srv.admit([Request(0, 0, big), Request(1, 0, small), Request(2, 1, small)])
     assert [r.rid for r in admitted] == [2]
     assert [r.rid for r in srv.admit([Request(3, 0, small)])] == [3]
+
+
+def _replay(depth, seed, reqs, scene):
+    ref = O.Dag(depth, True)
+    ref.build_terrain(O.terrain_for_world(4 << depth, seed))
+    for r in reqs:
+        if r.scene == scene and r.status == "done":
+            ref.apply_brushes(r.brushes)
+    return ref.export()
+
+
+def test_next_tick_is_submitted_while_one_runs(gpu):
+    """Two executors: a tick is enqueued while the previous one is still in
+    flight (overlapped), ticks over disjoint scenes run without a device
+    wait (concurrent), and the final scenes still equal the oracle replay."""
+    from voxel_amd.serving import Server, make_requests, serve
+    depths = [6, 6, 5, 6]
+    scenes = _scenes(depths)
+    reqs = make_requests(120, len(depths), [4 << d for d in depths], seed=23, lo_log2=8, hi_log2=13, max_brushes=8)
+    srv = Server(scenes, max_candidates=40_000, max_jobs=512)
+    res = serve(srv, reqs, [0.0] * len(reqs), max_tick=3)  # saturated: always something to submit
+    assert res["completed"] == len(reqs) and res["rejected"] == 0
+    assert res["overlapped_ticks"] > 0 and res["concurrent_ticks"] > 0
+    srv.close()
+    for s, d in enumerate(depths):
+        assert np.array_equal(scenes[s].export(), _replay(d, 20 + s, reqs, s)), s
+
+
+def test_overflowing_scene_is_rejected_others_publish(gpu):
+    """A scene whose slot table overflows inside a tick keeps its old root and
+    its requests of that tick are rejected; the other scene of the tick
+    publishes; the loop keeps serving."""
+    from voxel_amd.dag import HashDagGpu, terrain_for_world
+    from voxel_amd.serving import Request, Server, serve
+    small = HashDagGpu(5, True, node_words=1 << 20, leaf_words=1 << 20, node_slots=64, leaf_slots=64)
+    roomy = HashDagGpu(5, True, node_words=1 << 22, leaf_words=1 << 23)
+    roomy.build_terrain(terrain_for_world(128, 21))
+    before = small.export()  # empty: 64 slots hold no terrain
+    br = O.gen_brushes(5, O.STREAM_BRUSH, 20, 128, 1)
+    reqs = [Request(0, 0, br), Request(1, 1, br)]
+    srv = Server([small, roomy], max_candidates=1 << 18, max_jobs=256)
+    res = serve(srv, reqs, [0.0, 0.0])
+    srv.close()
+    assert [r.status for r in reqs] == ["rejected", "done"] and res["completed"] == 1
+    assert np.array_equal(small.export(), before)
+    ref = O.Dag(5, True)
+    ref.build_terrain(O.terrain_for_world(128, 21))
+    ref.apply_brushes(br)
+    assert np.array_equal(roomy.export(), ref.export())
+
+
+def test_full_scene_is_collected_and_served(gpu):
+    """Requests that repaint one region fill a small scene with garbage;
+    admission then finds no room, the server collects the scene's garbage and
+    carries on; the result equals the oracle replay of every request."""
+    from voxel_amd.dag import BRUSH_DTYPE, HashDagGpu, terrain_for_world
+    from voxel_amd.serving import Request, Server, serve
+    g = HashDagGpu(5, True, node_words=40_000, leaf_words=60_000, node_slots=1 << 14, leaf_slots=1 << 14)
+    g.build_terrain(terrain_for_world(128, 20))
+    reqs = []
+    for i in range(40):
+        b = np.zeros(1, BRUSH_DTYPE)
+        b[0] = (64, 64, 64, 12, 0, 0, 0, 1 + (i % 250))
+        reqs.append(Request(i, 0, b))
+    srv = Server([g], max_candidates=1 << 16, max_jobs=64)
+    res = serve(srv, reqs, [0.0] * len(reqs), max_tick=1)
+    srv.close()
+    assert res["completed"] == len(reqs) and res["collections"] > 0
+    assert np.array_equal(g.export(), _replay(5, 20, reqs, 0))

@@ -102,8 +102,8 @@ struct Table {
 };
 
 __device__ __forceinline__ Table table_of(const Dev& d, bool leaves) {
-  return leaves ? Table{d.leaf_slots, d.leaf_mask, d.leaf_pool, d.leaf_cap, d.ctl + kCtlLeafUsed, d.ctl + kCtlLeafBase}
-                : Table{d.node_slots, d.node_mask, d.node_pool, d.node_cap, d.ctl + kCtlNodeUsed, d.ctl + kCtlNodeBase};
+  return leaves ? Table{d.leaf_slots, d.leaf_mask, d.leaf_pool, d.leaf_cap, d.ctl + kCtlLeafUsed, d.base + 1}
+                : Table{d.node_slots, d.node_mask, d.node_pool, d.node_cap, d.ctl + kCtlNodeUsed, d.base};
 }
 
 // Item words in registers, every load issued before any use: a word-by-word
@@ -211,7 +211,7 @@ __device__ __forceinline__ void probe_entry(const ItemSet& set, const uint64_t* 
       }
       idx = (idx + 1) & t.mask;
     }
-    if (!done) set_status(d.ctl, kStatusOverflow);
+    if (!done) set_status(d, kStatusOverflow);
     res[i] = out;
   }
 }
@@ -311,7 +311,7 @@ struct CommitStore {
       const uint64_t fp_hi = uint64_t(p.fp) << 32;
       const uint64_t off = uint64_t(base) + seg_incl - w;
       if (off + w > t.cap || off >= kTag) {
-        set_status(d.ctl, kStatusOverflow);
+        set_status(d, kStatusOverflow);
         st_agent(&t.slots[uint32_t(p.r)], fp_hi | kTomb);
         res[i] = res_make(kResFinal, kEmpty);
       } else {
@@ -1195,7 +1195,7 @@ __global__ void publish_kernel(Scenes sc, uint32_t n_scenes, const uint32_t* __r
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n_scenes; s += gridDim.x * blockDim.x) {
     const uint32_t pos = root_pos[s];
     const Dev& d = sc.at(s);
-    if (pos == kNone || d.ctl[kCtlStatus] != kStatusOk) continue;
+    if (pos == kNone || *d.status != kStatusOk) continue;
     d.ctl[kCtlRoot] = resolve(rh.p[d.depth], pos);
   }
 }
@@ -1207,7 +1207,7 @@ __global__ void begin_kernel(Scenes sc, uint32_t n_scenes, uint32_t* counts, uin
                              uint32_t* fused, uint64_t fused_words) {
   const uint64_t t0 = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x, stride = uint64_t(gridDim.x) * blockDim.x;
   for (uint64_t s = t0; s < n_scenes; s += stride) {
-    sc.at(uint32_t(s)).ctl[kCtlStatus] = kStatusOk;
+    *sc.at(uint32_t(s)).status = kStatusOk;
     root_pos[s] = kNone;
   }
   for (uint64_t i = t0; i < n_counts; i += stride) counts[i] = 0;
@@ -1229,6 +1229,8 @@ constexpr uint32_t kCntRows = 58, kCntSlices = 59, kCntPending = 60;
 struct EditWs {
   uint32_t* counts;  // [0] records, [1 + h] entries at height h, [kCntPending] pending leaves
   uint32_t* root_pos;
+  uint32_t* status;  // per-scene first-error words of an executor's edits
+  uint32_t* base;    // per-scene pool snapshots of an executor's edits (2 words each)
   uint64_t *rkey, *rkey_alt, *rmask;
   uint32_t *rval, *rval_alt, *rmat;
   void* sort_ws;
@@ -1410,7 +1412,7 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
         }
         idx = (idx + 1) & tb.mask;
       }
-      if (!done) set_status(d.ctl, kStatusOverflow);
+      if (!done) set_status(d, kStatusOverflow);
       r = out;
     }
     __syncthreads();  // every claim of this height is in the table before pool offsets are handed out
@@ -1430,7 +1432,7 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
         const uint64_t off = uint64_t(base) + incl - wr;
         const uint64_t fp_hi = uint64_t(uint32_t(hash >> 32)) << 32;
         if (off + wr > tb.cap || off >= kTag) {
-          set_status(d.ctl, kStatusOverflow);
+          set_status(d, kStatusOverflow);
           st_agent(&tb.slots[uint32_t(r)], fp_hi | kTomb);
           r = res_make(kResFinal, kEmpty);
         } else {
@@ -1480,6 +1482,8 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
   w.jobs_off = align_up(c.off, 256);
   w.jobs = c.take<BrushJob>(std::max<uint32_t>(1, n_jobs));
   w.root_pos = c.take<uint32_t>(p.n_scenes);
+  w.status = c.take<uint32_t>(p.n_scenes);
+  w.base = c.take<uint32_t>(2 * size_t(p.n_scenes));
   const uint64_t R = std::max<uint64_t>(1, r_ub);
   // Packed records when the index of the last record fits below the key
   // (VOXEL_AMD_SORT_PAIRS=1 forces the pair layout, for testing).
@@ -1698,6 +1702,14 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
 // Brush jobs: brick bounding boxes clipped to the world; empty ones dropped.
 // Brush jobs; *rows (may be null) is the running count of brick rows (each
 // job's rows start there).
+// Brush fields the header documents: kind 0 (sphere) or 1 (box), material
+// 0 (remove) or 1..255.
+bool brushes_valid(const vx_brush* b, uint32_t n) {
+  for (uint32_t i = 0; i < n; ++i)
+    if (b[i].kind > 1 || b[i].material > 255) return false;
+  return true;
+}
+
 uint64_t make_jobs(const vx_brush* b, uint32_t n, uint32_t world, uint32_t scene, uint64_t start,
                    std::vector<BrushJob>* jobs, uint64_t* rows = nullptr) {
   uint64_t total = start;
@@ -1811,7 +1823,7 @@ vx_status upload(EditWs& w, const std::vector<Dev>& devs, const std::vector<Brus
   const size_t n_jobs = jobs ? jobs->size() : 0;
   const size_t bytes = w.jobs_off + n_jobs * sizeof(BrushJob);
   hs.readback = align_up(bytes, 256);
-  if (hs.reserve(hs.readback + (devs.size() + 64) * sizeof(uint32_t)) != VX_OK) return VX_DEVICE_ERROR;
+  if (hs.reserve(hs.readback + (4 * devs.size() + 64) * sizeof(uint32_t)) != VX_OK) return VX_DEVICE_ERROR;
   char* h = static_cast<char*>(hs.p);
   uint32_t* cnt = reinterpret_cast<uint32_t*>(h);
   std::memset(cnt, 0, 64 * sizeof(uint32_t));
@@ -1986,6 +1998,8 @@ extern "C" vx_status vx_dag_create(uint32_t inner_depth, int32_t materials, cons
   d.node_cap = c.node_words;
   d.leaf_cap = c.leaf_words;
   d.ctl = g->ctl;
+  d.status = g->ctl + kCtlStatus;
+  d.base = g->ctl + kCtlNodeBase;
   d.depth = inner_depth;
   d.leaf_words = g->materials ? 16 : 2;
   vx_status st = vx_dag_reset(g, nullptr);
@@ -2028,7 +2042,7 @@ extern "C" vx_status vx_dag_stats_get(const vx_dag* g, vx_dag_stats* out) {
   out->root = ctl[kCtlRoot];
   out->inner_depth = g->depth;
   out->leaf_words = g->dev.leaf_words;
-  out->reserved = 0;
+  out->last_status = ctl[kCtlStatus];
   return VX_OK;
 }
 
@@ -2053,7 +2067,7 @@ extern "C" vx_status vx_dag_apply_voxel_edits(vx_dag* g, const vx_voxel* edits, 
 }
 
 extern "C" vx_status vx_dag_brush_workspace_size(const vx_dag* g, const vx_brush* b, uint32_t n, size_t* bytes) {
-  if (!g || !bytes || (!b && n)) return VX_INVALID_VALUE;
+  if (!g || !bytes || (!b && n) || (n && !brushes_valid(b, n))) return VX_INVALID_VALUE;
   std::vector<BrushJob> jobs;
   const uint64_t cand = make_jobs(b, n, 4u << g->depth, 0, 0, &jobs);
   *bytes = cand ? carve(nullptr, make_plan({g}), cand, uint32_t(jobs.size()), cand).bytes : 0;
@@ -2062,7 +2076,7 @@ extern "C" vx_status vx_dag_brush_workspace_size(const vx_dag* g, const vx_brush
 
 extern "C" vx_status vx_dag_apply_brushes(vx_dag* g, const vx_brush* b, uint32_t n, void* ws, size_t ws_bytes,
                                           void* stream) {
-  if (!g || (!b && n)) return VX_INVALID_VALUE;
+  if (!g || (!b && n) || (n && !brushes_valid(b, n))) return VX_INVALID_VALUE;
   std::vector<BrushJob> jobs;
   uint64_t rows = 0;
   const uint64_t cand = make_jobs(b, n, 4u << g->depth, 0, 0, &jobs, &rows);
@@ -2085,6 +2099,7 @@ static vx_status batch_setup(vx_dag* const* dags, const vx_brush* const* brushes
   std::unordered_map<const vx_dag*, uint32_t> index;
   for (uint32_t i = 0; i < n_batches; ++i) {
     if (!dags[i] || (!brushes[i] && counts[i])) return VX_INVALID_VALUE;
+    if (counts[i] && !brushes_valid(brushes[i], counts[i])) return VX_INVALID_VALUE;
     if (index.emplace(dags[i], uint32_t(scenes->size())).second) scenes->push_back(dags[i]);
   }
   const vx_dag* first = (*scenes)[0];
@@ -2233,11 +2248,18 @@ struct vx_batcher {
   std::vector<Dev> devs;
   std::vector<BrushJob> jobs;
   HostStage stage;
+  // Asynchronous submissions: one in flight at a time, completed by query.
+  hipEvent_t done = nullptr;
+  bool submitted = false;  // `done` has been recorded at least once
+  bool inflight = false;
+  bool empty = false;      // the submission in flight had no candidates
+  vx_status last = VX_OK;  // outcome of the last completed submission
 };
 
 extern "C" vx_status vx_batcher_destroy(vx_batcher* b) {
   if (!b) return VX_INVALID_VALUE;
   if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->done) hipEventDestroy(b->done);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   if (b->ws) hipFree(b->ws);
@@ -2275,8 +2297,17 @@ extern "C" vx_status vx_batcher_create(vx_dag* const* scenes, uint32_t n_scenes,
     return VX_OUT_OF_CAPACITY;
   }
   b->w = carve(b->ws, b->plan, max_candidates, max_jobs, max_candidates);
+  for (uint32_t i = 0; i < n_scenes; ++i) {
+    b->devs[i].status = b->w.status + i;
+    b->devs[i].base = b->w.base + 2 * i;
+  }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     b->stream = nullptr;
+    vx_batcher_destroy(b);
+    return VX_DEVICE_ERROR;
+  }
+  if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+    b->done = nullptr;
     vx_batcher_destroy(b);
     return VX_DEVICE_ERROR;
   }
@@ -2298,11 +2329,19 @@ extern "C" vx_status vx_batcher_layout(const vx_batcher* b, uint32_t* key_bits, 
   return VX_OK;
 }
 
-extern "C" vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batch, const vx_brush* const* brushes,
-                                    const uint32_t* counts, uint32_t n_batches, uint64_t* candidates) {
+// Read-back block of a submission (pinned host memory after the upload):
+// per scene ctl words [node used, leaf used] and the executor's status word.
+constexpr uint32_t kReadbackWords = 4;
+
+static vx_status batcher_submit(vx_batcher* b, const uint32_t* scene_of_batch, const vx_brush* const* brushes,
+                                const uint32_t* counts, uint32_t n_batches, uint64_t* candidates,
+                                const vx_batcher* after) {
   if (!b || (n_batches && (!scene_of_batch || !brushes || !counts))) return VX_INVALID_VALUE;
+  if (b->inflight || after == b) return VX_INVALID_VALUE;  // complete the last submission first
   for (uint32_t i = 0; i < n_batches; ++i)
-    if (scene_of_batch[i] >= b->scenes.size() || (!brushes[i] && counts[i])) return VX_INVALID_VALUE;
+    if (scene_of_batch[i] >= b->scenes.size() || (!brushes[i] && counts[i]) ||
+        (counts[i] && !brushes_valid(brushes[i], counts[i])))
+      return VX_INVALID_VALUE;
   b->jobs.clear();
   uint64_t cand = 0, rows = 0;
   for (uint32_t s = 0; s < b->scenes.size(); ++s)
@@ -2311,33 +2350,103 @@ extern "C" vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batc
         cand += make_jobs(brushes[i], counts[i], 4u << b->scenes[s]->depth, s, cand, &b->jobs, &rows);
   if (candidates) *candidates = cand;
   if (cand > b->cap_cand || b->jobs.size() > b->cap_jobs) return VX_INVALID_VALUE;
-  if (cand == 0) return VX_OK;
   hipStream_t s = b->stream;
   vx_status st;
-  // The brush path's first scan runs over brick rows.
-  if ((st = upload(b->w, b->devs, &b->jobs, uint32_t(rows), b->stage, s)) != VX_OK) return st;
-  const Scenes sc{b->plan.n_scenes > 1 ? b->w.scenes : nullptr, nullptr, b->plan.codec, b->devs[0]};
-  const BrushLoad load{BrushCands{b->w.jobs, &b->w.counts[kCntJobs], b->scenes[0]->materials, b->plan.codec}};
-  if (b->use_graph) {
-    if (!b->exec) {
-      if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) return VX_DEVICE_ERROR;
-      const vx_status cap_st = device_pipeline(nullptr, b->plan, sc, b->w, b->cap_cand, load, s);
-      hipGraph_t g = nullptr;
-      const hipError_t end = hipStreamEndCapture(s, &g);
-      if (cap_st != VX_OK || end != hipSuccess || !g) {
-        if (g) hipGraphDestroy(g);
-        return cap_st != VX_OK ? cap_st : VX_DEVICE_ERROR;
+  const uint32_t n_scenes = uint32_t(b->scenes.size());
+  b->empty = cand == 0;
+  if (!b->empty) {
+    // The brush path's first scan runs over brick rows. The upload goes to this
+    // executor's own workspace, so it may overlap another executor's edit.
+    if ((st = upload(b->w, b->devs, &b->jobs, uint32_t(rows), b->stage, s)) != VX_OK) return st;
+    if (after && after->submitted && hipStreamWaitEvent(s, after->done, 0) != hipSuccess) return VX_DEVICE_ERROR;
+    const Scenes sc{b->plan.n_scenes > 1 ? b->w.scenes : nullptr, nullptr, b->plan.codec, b->devs[0]};
+    const BrushLoad load{BrushCands{b->w.jobs, &b->w.counts[kCntJobs], b->scenes[0]->materials, b->plan.codec}};
+    if (b->use_graph) {
+      if (!b->exec) {
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) return VX_DEVICE_ERROR;
+        const vx_status cap_st = device_pipeline(nullptr, b->plan, sc, b->w, b->cap_cand, load, s);
+        hipGraph_t g = nullptr;
+        const hipError_t end = hipStreamEndCapture(s, &g);
+        if (cap_st != VX_OK || end != hipSuccess || !g) {
+          if (g) hipGraphDestroy(g);
+          return cap_st != VX_OK ? cap_st : VX_DEVICE_ERROR;
+        }
+        if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+          hipGraphDestroy(g);
+          b->exec = nullptr;
+          return VX_DEVICE_ERROR;
+        }
+        b->graph = g;
       }
-      if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
-        hipGraphDestroy(g);
-        b->exec = nullptr;
-        return VX_DEVICE_ERROR;
-      }
-      b->graph = g;
+      if (hipGraphLaunch(b->exec, s) != hipSuccess) return VX_DEVICE_ERROR;
+    } else {
+      if ((st = device_pipeline(nullptr, b->plan, sc, b->w, b->cap_cand, load, s)) != VX_OK) return st;
     }
-    if (hipGraphLaunch(b->exec, s) != hipSuccess) return VX_DEVICE_ERROR;
   } else {
-    if ((st = device_pipeline(nullptr, b->plan, sc, b->w, b->cap_cand, load, s)) != VX_OK) return st;
+    b->stage.readback = 0;
+    if (b->stage.reserve(size_t(n_scenes) * kReadbackWords * 4) != VX_OK) return VX_DEVICE_ERROR;
+    if (after && after->submitted && hipStreamWaitEvent(s, after->done, 0) != hipSuccess) return VX_DEVICE_ERROR;
   }
-  return finish(nullptr, b->scenes, b->w, b->plan, b->stage, s);
+  // Read-back of every scene's pool usage and status, then the completion event.
+  uint32_t* rb = reinterpret_cast<uint32_t*>(static_cast<char*>(b->stage.p) + b->stage.readback);
+  for (uint32_t i = 0; i < n_scenes; ++i) {
+    if (hipMemcpyAsync(rb + kReadbackWords * i, b->scenes[i]->ctl, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        (!b->empty && hipMemcpyAsync(rb + kReadbackWords * i + 2, b->w.status + i, sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, s) != hipSuccess))
+      return VX_DEVICE_ERROR;
+  }
+  if (hipEventRecord(b->done, s) != hipSuccess) return VX_DEVICE_ERROR;
+  b->submitted = true;
+  b->inflight = true;
+  return VX_OK;
+}
+
+static vx_status batcher_query(vx_batcher* b, int32_t wait, int32_t* done, uint32_t* scene_status,
+                               uint64_t* scene_used) {
+  if (!b) return VX_INVALID_VALUE;
+  const uint32_t n_scenes = uint32_t(b->scenes.size());
+  if (b->inflight) {
+    const hipError_t e = wait ? hipEventSynchronize(b->done) : hipEventQuery(b->done);
+    if (e == hipErrorNotReady) {
+      if (done) *done = 0;
+      return VX_OK;
+    }
+    b->inflight = false;
+    if (e != hipSuccess) return b->last = VX_DEVICE_ERROR;
+    const uint32_t* rb = reinterpret_cast<const uint32_t*>(static_cast<const char*>(b->stage.p) + b->stage.readback);
+    b->last = VX_OK;
+    for (uint32_t i = 0; i < n_scenes; ++i)
+      if (!b->empty && rb[kReadbackWords * i + 2] != kStatusOk) b->last = VX_OUT_OF_CAPACITY;
+  }
+  if (done) *done = 1;
+  if (b->submitted && (scene_status || scene_used)) {
+    const uint32_t* rb = reinterpret_cast<const uint32_t*>(static_cast<const char*>(b->stage.p) + b->stage.readback);
+    for (uint32_t i = 0; i < n_scenes; ++i) {
+      if (scene_status) scene_status[i] = b->empty ? kStatusOk : rb[kReadbackWords * i + 2];
+      if (scene_used) {
+        scene_used[2 * i] = rb[kReadbackWords * i + kCtlNodeUsed];
+        scene_used[2 * i + 1] = rb[kReadbackWords * i + kCtlLeafUsed];
+      }
+    }
+  }
+  return b->last;
+}
+
+extern "C" vx_status vx_batcher_submit(vx_batcher* b, const uint32_t* scene_of_batch, const vx_brush* const* brushes,
+                                       const uint32_t* counts, uint32_t n_batches, uint64_t* candidates,
+                                       const vx_batcher* after) {
+  return batcher_submit(b, scene_of_batch, brushes, counts, n_batches, candidates, after);
+}
+
+extern "C" vx_status vx_batcher_query(vx_batcher* b, int32_t wait, int32_t* done, uint32_t* scene_status,
+                                      uint64_t* scene_used) {
+  return batcher_query(b, wait, done, scene_status, scene_used);
+}
+
+extern "C" vx_status vx_batcher_run(vx_batcher* b, const uint32_t* scene_of_batch, const vx_brush* const* brushes,
+                                    const uint32_t* counts, uint32_t n_batches, uint64_t* candidates) {
+  const vx_status st = batcher_submit(b, scene_of_batch, brushes, counts, n_batches, candidates, nullptr);
+  if (st != VX_OK) return st;
+  return batcher_query(b, 1, nullptr, nullptr, nullptr);
 }

@@ -50,6 +50,15 @@ struct Dev {
   uint64_t node_mask, leaf_mask;
   uint64_t node_cap, leaf_cap;  // words
   uint32_t* ctl;
+  // First-error word of the edit in flight: the scene's own ctl[kCtlStatus]
+  // for direct calls; an executor points it into its workspace, so two
+  // executors editing disjoint scene sets never share a status word.
+  uint32_t* status;
+  // Pool usage snapshots taken when a level's probe starts ([0] nodes, [1]
+  // leaves; the commit places new items from there): ctl + kCtlNodeBase for
+  // direct calls, the executor's workspace for an executor, for the same
+  // reason as `status`.
+  uint32_t* base;
   uint32_t depth;       // inner levels D; leaves sit at level D
   uint32_t leaf_words;  // 2 or 16
 };
@@ -64,7 +73,7 @@ __device__ __forceinline__ uint32_t resolve(const uint64_t* res, uint32_t k) {
   return uint32_t(r);
 }
 
-__device__ __forceinline__ void set_status(uint32_t* ctl, uint32_t v) { atomicCAS(&ctl[kCtlStatus], kStatusOk, v); }
+__device__ __forceinline__ void set_status(const Dev& d, uint32_t v) { atomicCAS(d.status, kStatusOk, v); }
 
 // child_ref (detail/node.cuh:148-175): sentinels propagate; otherwise decode
 // the dense ref list with popcount.

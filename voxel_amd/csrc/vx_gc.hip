@@ -240,6 +240,8 @@ extern "C" vx_status vx_dag_collect_garbage(vx_dag* g, vx_gc_report* report, voi
   nd.node_cap = node_bound;
   nd.leaf_cap = leaf_bound;
   nd.ctl = nctl.get<uint32_t>();
+  nd.status = nd.ctl + kCtlStatus;
+  nd.base = nd.ctl + kCtlNodeBase;
 
   // 3. rebuild, bottom-up.
   if (root > kFull) {

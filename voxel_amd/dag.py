@@ -33,7 +33,7 @@ class Terrain(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("node_words_used", C.c_uint64), ("leaf_words_used", C.c_uint64), ("node_slots", C.c_uint64),
                 ("leaf_slots", C.c_uint64), ("root", C.c_uint32), ("inner_depth", C.c_uint32),
-                ("leaf_words", C.c_uint32), ("reserved", C.c_uint32)]
+                ("leaf_words", C.c_uint32), ("last_status", C.c_uint32)]
 
 
 class GcReport(C.Structure):
@@ -74,6 +74,8 @@ register({
     "vx_batcher_info": (_i32, [_P, C.POINTER(_u64), C.POINTER(_i32)]),
     "vx_batcher_run": (_i32, [_P, _P, _P, _P, _u32, C.POINTER(_u64)]),
     "vx_batcher_layout": (_i32, [_P, C.POINTER(_u32), C.POINTER(_u32)]),
+    "vx_batcher_submit": (_i32, [_P, _P, _P, _P, _u32, C.POINTER(_u64), _P]),
+    "vx_batcher_query": (_i32, [_P, _i32, C.POINTER(_i32), _P, _P]),
     "vx_dag_profile_enable": (_i32, [_P, _i32]),
     "vx_dag_profile_read": (_i32, [_P, _P]),
     "vx_dag_profile_kernel_mask": (_i32, [_P, _u64]),
@@ -380,16 +382,42 @@ class Batcher:
         check(lib().vx_batcher_layout(self.h, C.byref(k), C.byref(i)), "vx_batcher_layout")
         return k.value, i.value
 
-    def run(self, batches) -> int:
-        """batches: list of (scene index, brush array). Returns candidates."""
+    @staticmethod
+    def _args(batches):
         arrays = [np.ascontiguousarray(br, BRUSH_DTYPE) for _, br in batches]
         n = len(arrays)
-        scene_of = (_u32 * n)(*[s for s, _ in batches])
-        ptrs = (_P * n)(*[a.ctypes.data for a in arrays])
-        counts = (_u32 * n)(*[len(a) for a in arrays])
+        scene_of = (_u32 * max(1, n))(*[s for s, _ in batches])
+        ptrs = (_P * max(1, n))(*[a.ctypes.data for a in arrays])
+        counts = (_u32 * max(1, n))(*[len(a) for a in arrays])
+        return arrays, scene_of, ptrs, counts, n
+
+    def run(self, batches) -> int:
+        """batches: list of (scene index, brush array). Returns candidates."""
+        arrays, scene_of, ptrs, counts, n = self._args(batches)
         cand = C.c_uint64()
         check(lib().vx_batcher_run(self.h, scene_of, ptrs, counts, n, C.byref(cand)), "vx_batcher_run")
         return cand.value
+
+    def submit(self, batches, after: "Batcher | None" = None) -> int:
+        """Enqueue without waiting (vx_batcher_submit); `after` orders the
+        device work behind another executor's last submission."""
+        arrays, scene_of, ptrs, counts, n = self._args(batches)
+        cand = C.c_uint64()
+        check(lib().vx_batcher_submit(self.h, scene_of, ptrs, counts, n, C.byref(cand),
+                                      after.h if after is not None else None), "vx_batcher_submit")
+        return cand.value
+
+    def query(self, wait: bool = False):
+        """(done, status, per-scene status, per-scene (node, leaf) words used)
+        of the last submission; status is the vx_status of the completed one."""
+        n = len(self.scenes)
+        done = C.c_int32()
+        st = np.zeros(n, np.uint32)
+        used = np.zeros(2 * n, np.uint64)
+        r = lib().vx_batcher_query(self.h, int(wait), C.byref(done), st.ctypes.data_as(_P), used.ctypes.data_as(_P))
+        if not done.value:
+            return False, 0, None, None
+        return True, r, st, used.reshape(n, 2)
 
 
 def smoke_check(dev) -> None:

@@ -1,13 +1,33 @@
 """Continuous-batching edit server over resident scenes.
 
-Requests (scene, brush batch) arrive over time; every tick the server admits
-the pending requests whose worst-case table growth fits the free pool space of
-their scene (admission by free HBM reserved for node/leaf growth), batches
-all admitted requests into ONE multi-scene pipeline call
-(`vx_dag_apply_batches`), and completes them together. Requests that arrive
-while a tick runs on the GPU wait for the next tick. Per scene the requests
-apply in arrival order, so every scene's final DAG equals replaying its
-requests one by one, and a request's effect is the same alone or in the mix.
+Requests (scene, brush batch) arrive over time. The server keeps up to two
+ticks in flight on two persistent executors (vx_batcher, each with its own
+workspace, pinned staging buffer, stream and captured hipGraph):
+
+- admission: a pending request is admitted when its worst-case growth
+  (every touched brick a new leaf, every touched ancestor a new 9-word node)
+  fits its scene's pool words, counted as the usage read back by the scene's
+  last completed tick plus the bounds of its requests still in flight — the
+  free HBM reserved for table and pool growth — and when the tick stays
+  within the executor's candidate and brush capacity; a deferred request
+  blocks the later requests of its scene (per-scene arrival order), other
+  scenes proceed;
+- submission (vx_batcher_submit) returns at once: the host prepares and
+  uploads the next tick while the previous one runs. A tick whose scenes are
+  disjoint from the tick in flight runs concurrently with it on the device;
+  a tick that shares a scene waits for it on the device (vx_batcher_submit's
+  `after`), never on the host;
+- completion is polled (vx_batcher_query): a tick's requests complete when
+  the host sees its event, each scene's pool usage comes back with it, and a
+  scene that overflowed keeps its old root and its requests of that tick
+  are marked rejected;
+- a request that does not fit its scene even with nothing in flight triggers
+  a garbage collection of that scene (vx_dag_collect_garbage) after the
+  executors drain; if it still does not fit it is rejected.
+
+Per scene the requests apply in arrival order, so every scene's final DAG
+equals replaying its requests one by one, and a request's effect is the same
+alone or in the mix.
 """
 from __future__ import annotations
 
@@ -18,6 +38,7 @@ import time
 import numpy as np
 
 from . import dag as V
+from ._lib import check
 
 
 @dataclasses.dataclass
@@ -27,8 +48,10 @@ class Request:
     brushes: np.ndarray
     arrival: float = 0.0
     voxel_ops: int = 0
-    done: float = -1.0
+    done: float = -1.0      # seconds from the start of serve()
     status: str = "pending"
+    finished: float = -1.0  # host clock (time.perf_counter) at completion
+    bound: tuple = (0, 0)   # reserved (node words, leaf words) while admitted
 
 
 def growth_bound(d: V.HashDagGpu, brushes: np.ndarray) -> tuple[int, int]:
@@ -48,26 +71,80 @@ def growth_bound(d: V.HashDagGpu, brushes: np.ndarray) -> tuple[int, int]:
     return nodes * 9, leaf_words
 
 
+def brick_candidates(d: V.HashDagGpu, brushes: np.ndarray) -> int:
+    """Candidate leaf bricks of a request: the bricks of each brush's box
+    clipped to the world (what an executor's capacity counts)."""
+    world = d.world
+    total = 0
+    for b in brushes:
+        ext = (b["hx"], b["hx"], b["hx"]) if b["kind"] == 0 else (b["hx"], b["hy"], b["hz"])
+        n = 1
+        for c, e in zip((b["cx"], b["cy"], b["cz"]), ext):
+            lo, hi = max(0, int(c) - int(e)), min(world - 1, int(c) + int(e))
+            n *= max(0, (hi >> 2) - (lo >> 2) + 1)
+        total += n
+    return total
+
+
+@dataclasses.dataclass
+class Tick:
+    ex: int
+    reqs: list
+    scenes: set
+    bounds: dict  # scene -> [node words, leaf words] reserved by the tick
+
+
 class Server:
-    """Tick executor. With `max_candidates` set, ticks run on a persistent
-    Batcher (pipeline captured once into a hipGraph, replayed every tick) and
-    admission also keeps each tick within its candidate capacity."""
+    """Continuous-batching executor. With `max_candidates` set, ticks run on
+    `executors` persistent Batchers (pipeline captured once into a hipGraph
+    per executor, replayed every tick), submitted asynchronously; without it,
+    each tick is one synchronous multi-scene call (vx_dag_apply_batches)."""
 
     def __init__(self, scenes: list[V.HashDagGpu], max_candidates: int | None = None, max_jobs: int = 8192,
-                 use_graph: bool = True):
+                 use_graph: bool = True, executors: int = 2, collect_on_full: bool = True):
+        if not 1 <= executors <= 2:
+            raise ValueError("one or two executors")
         self.scenes = scenes
         self.caps = [(d._cap.node_words, d._cap.leaf_words) for d in scenes]
+        self.used = []
+        for d in scenes:
+            st = d.stats()
+            self.used.append([st.node_words_used, st.leaf_words_used])
         self.ticks = 0
         self.tick_sizes = []
+        self.overlapped = 0  # ticks submitted while another tick was still in flight
+        self.concurrent = 0  # ... of them, with no scene in common (no device wait)
+        self.collections = 0
         self.max_candidates = max_candidates
         self.max_jobs = max_jobs
-        self.ex = V.Batcher(scenes, max_candidates, max_jobs, use_graph) if max_candidates else None
+        self.collect_on_full = collect_on_full
+        self.exs = [V.Batcher(scenes, max_candidates, max_jobs, use_graph) for _ in range(executors)] \
+            if max_candidates else []
+        self.ex = self.exs[0] if self.exs else None
+        self.inflight: list[Tick] = []
+
+    def close(self):
+        self.drain()
+        for e in self.exs:
+            e.close()
+        self.exs, self.ex = [], None
+
+    # -- admission --------------------------------------------------------------
+    def _reserved(self, s: int) -> list[int]:
+        r = [0, 0]
+        for t in self.inflight:
+            b = t.bounds.get(s)
+            if b:
+                r[0] += b[0]
+                r[1] += b[1]
+        return r
 
     def admit(self, pending: list[Request]) -> list[Request]:
         """Requests whose worst-case growth fits their scene's free pool words
-        (cumulative within the tick) and, with a Batcher, whose candidates fit
-        its capacity; the rest stay pending, in order."""
-        used = {}
+        (usage known from the last completed tick + everything reserved by
+        ticks in flight + this tick) and, with executors, whose candidates fit
+        the executor's capacity; the rest stay pending, in order."""
+        extra = {}
         out = []
         cand = jobs = 0
         blocked = set()
@@ -75,35 +152,119 @@ class Server:
             if r.scene in blocked:
                 continue  # keep per-scene arrival order: nothing overtakes a deferred request
             d = self.scenes[r.scene]
-            if r.scene not in used:
-                st = d.stats()
-                used[r.scene] = [st.node_words_used, st.leaf_words_used]
+            e = extra.setdefault(r.scene, [0, 0])
+            res = self._reserved(r.scene)
             nw, lw = growth_bound(d, r.brushes)
             cap_n, cap_l = self.caps[r.scene]
-            u = used[r.scene]
-            c = V.batch_candidates(self.scenes, [(r.scene, r.brushes)]) if self.ex else 0
-            fits_pool = u[0] + nw <= cap_n and u[1] + lw <= cap_l
-            fits_exec = not self.ex or (cand + c <= self.max_candidates and jobs + len(r.brushes) <= self.max_jobs)
+            u = self.used[r.scene]
+            c = brick_candidates(d, r.brushes) if self.exs else 0
+            fits_pool = u[0] + res[0] + e[0] + nw <= cap_n and u[1] + res[1] + e[1] + lw <= cap_l
+            fits_exec = not self.exs or (cand + c <= self.max_candidates and jobs + len(r.brushes) <= self.max_jobs)
             if fits_pool and fits_exec:
-                u[0] += nw
-                u[1] += lw
+                e[0] += nw
+                e[1] += lw
                 cand += c
                 jobs += len(r.brushes)
+                r.bound = (nw, lw)
                 out.append(r)
             else:
                 blocked.add(r.scene)
         return out
 
-    def tick(self, reqs: list[Request], workspace=None) -> None:
-        """One pipeline call for every request of the tick, in arrival order."""
+    # -- ticks ------------------------------------------------------------------
+    def can_submit(self) -> bool:
+        return not self.exs or len(self.inflight) < len(self.exs)
+
+    def busy(self) -> bool:
+        return bool(self.inflight)
+
+    def submit(self, reqs: list[Request]) -> None:
+        """Start one tick for `reqs` (arrival order kept per scene)."""
         if not reqs:
             return
-        if self.ex:
-            self.ex.run([(r.scene, r.brushes) for r in reqs])
-        else:
-            V.apply_batches([self.scenes[r.scene] for r in reqs], [r.brushes for r in reqs], workspace=workspace)
         self.ticks += 1
         self.tick_sizes.append(len(reqs))
+        bounds = {}
+        for r in reqs:
+            b = bounds.setdefault(r.scene, [0, 0])
+            b[0] += r.bound[0]
+            b[1] += r.bound[1]
+        scenes = set(bounds)
+        if not self.exs:
+            self._tick_direct(reqs)
+            return
+        busy = {t.ex for t in self.inflight}
+        ex = next(i for i in range(len(self.exs)) if i not in busy)
+        after = None
+        for t in self.inflight:
+            if t.scenes & scenes:
+                after = self.exs[t.ex]
+        if self.inflight:
+            self.overlapped += 1
+            if after is None:
+                self.concurrent += 1
+        self.exs[ex].submit([(r.scene, r.brushes) for r in reqs], after=after)
+        self.inflight.append(Tick(ex, reqs, scenes, bounds))
+
+    def _tick_direct(self, reqs: list[Request]) -> None:
+        from ._lib import OutOfCapacity
+        failed = set()
+        try:
+            V.apply_batches([self.scenes[r.scene] for r in reqs], [r.brushes for r in reqs])
+        except OutOfCapacity:
+            failed = {s for s in {r.scene for r in reqs} if self.scenes[s].stats().last_status != 0}
+        for s in {r.scene for r in reqs}:
+            st = self.scenes[s].stats()
+            self.used[s] = [st.node_words_used, st.leaf_words_used]
+        now = time.perf_counter()
+        for r in reqs:
+            r.status = "rejected" if r.scene in failed else "done"
+            r.finished = now
+
+    def poll(self, wait: bool = False) -> list[Tick]:
+        """Complete the ticks the device has finished (with wait, at least the
+        oldest one). A scene that overflowed keeps its old root; its requests
+        of that tick are marked rejected."""
+        done = []
+        for k, t in enumerate(list(self.inflight)):
+            ok, status, st, used = self.exs[t.ex].query(wait=wait and k == 0)
+            if not ok:
+                continue
+            if status not in (0, 2):  # VX_OK, VX_OUT_OF_CAPACITY (per scene, below)
+                check(status, "vx_batcher_query")
+            now = time.perf_counter()
+            for s in t.scenes:
+                self.used[s] = [int(used[s][0]), int(used[s][1])]
+            for r in t.reqs:
+                r.status = "rejected" if st[r.scene] != 0 else "done"
+                r.finished = now
+            self.inflight.remove(t)
+            done.append(t)
+        return done
+
+    def drain(self) -> None:
+        while self.inflight:
+            self.poll(wait=True)
+
+    def tick(self, reqs: list[Request]) -> None:
+        """One tick, waited for (the synchronous form)."""
+        for r in reqs:
+            r.bound = growth_bound(self.scenes[r.scene], r.brushes)
+        self.submit(reqs)
+        self.drain()
+
+    def make_room(self, r: Request) -> bool:
+        """Garbage-collect r's scene (after the executors drain) and report
+        whether r fits it now."""
+        if not self.collect_on_full:
+            return False
+        self.drain()
+        d = self.scenes[r.scene]
+        d.collect_garbage()
+        self.collections += 1
+        st = d.stats()
+        self.used[r.scene] = [st.node_words_used, st.leaf_words_used]
+        return bool(self.admit([r]))
 
 
 def make_requests(n: int, n_scenes: int, worlds: list[int], seed: int, lo_log2: int = 6, hi_log2: int = 18,
@@ -124,53 +285,66 @@ def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate, n))
 
 
-def serve(server: Server, reqs: list[Request], arrivals: np.ndarray, max_tick: int = 1 << 30) -> dict:
-    """Real-time loop: arrivals are offsets in seconds from the start."""
-    import torch
+def serve(server: Server, reqs: list[Request], arrivals, max_tick: int = 1 << 30) -> dict:
+    """Real-time loop: arrivals are offsets in seconds from the start. Ticks
+    are submitted as soon as an executor is free and requests are pending;
+    completions are polled between arrivals."""
     for r, a in zip(reqs, arrivals):
         r.arrival = float(a)
     pending: list[Request] = []
     nxt = 0
     t0 = time.perf_counter()
-    rejected = 0
-    while nxt < len(reqs) or pending:
+    ticks0 = server.ticks
+    while nxt < len(reqs) or pending or server.busy():
         now = time.perf_counter() - t0
         while nxt < len(reqs) and reqs[nxt].arrival <= now:
             pending.append(reqs[nxt])
             nxt += 1
-        if not pending:
-            wait = reqs[nxt].arrival - now
+        if server.busy():
+            server.poll(wait=False)
+        if pending and server.can_submit():
+            batch = server.admit(pending[:max_tick])
+            if batch:
+                server.submit(batch)
+                ids = {r.rid for r in batch}
+                pending = [r for r in pending if r.rid not in ids]
+                continue
+            if not server.busy():
+                # Nothing fits with the GPU idle: collect the oldest request's
+                # scene, else reject the request.
+                if server.make_room(pending[0]):
+                    continue
+                pending[0].status = "rejected"
+                pending[0].finished = time.perf_counter()
+                pending.pop(0)
+                continue
+        if server.busy():
+            # Only completions can change anything before the next arrival.
+            if nxt >= len(reqs) or reqs[nxt].arrival > time.perf_counter() - t0:
+                time.sleep(0)
+            continue
+        if nxt < len(reqs):
+            wait = reqs[nxt].arrival - (time.perf_counter() - t0)
             if wait > 0:
                 time.sleep(min(wait, 0.001))
-            continue
-        batch = server.admit(pending[:max_tick])
-        if not batch:
-            # Nothing fits even alone with the GPU idle: reject the oldest.
-            pending[0].status = "rejected"
-            pending[0].done = time.perf_counter() - t0
-            pending.pop(0)
-            rejected += 1
-            continue
-        server.tick(batch)
-        torch.cuda.synchronize()
-        done = time.perf_counter() - t0
-        ids = {r.rid for r in batch}
-        for r in batch:
-            r.done = done
-            r.status = "done"
-        pending = [r for r in pending if r.rid not in ids]
     makespan = time.perf_counter() - t0
+    for r in reqs:
+        r.done = r.finished - t0
     lat = np.array([(r.done - r.arrival) * 1e3 for r in reqs if r.status == "done"])
     ops = sum(r.voxel_ops for r in reqs if r.status == "done")
+    ticks = server.ticks - ticks0
     return {
         "requests": len(reqs),
         "completed": int(lat.size),
-        "rejected": rejected,
+        "rejected": sum(1 for r in reqs if r.status == "rejected"),
         "p50_ms": float(np.percentile(lat, 50)) if lat.size else math.nan,
         "p99_ms": float(np.percentile(lat, 99)) if lat.size else math.nan,
         "mean_ms": float(lat.mean()) if lat.size else math.nan,
         "edits_per_s": ops / makespan,
         "makespan_s": makespan,
-        "ticks": server.ticks,
-        "mean_tick_requests": float(np.mean(server.tick_sizes)) if server.tick_sizes else 0.0,
+        "ticks": ticks,
+        "mean_tick_requests": float(np.mean(server.tick_sizes[ticks0:])) if ticks else 0.0,
+        "overlapped_ticks": server.overlapped,
+        "concurrent_ticks": server.concurrent,
+        "collections": server.collections,
     }
