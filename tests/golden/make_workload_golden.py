@@ -1,0 +1,105 @@
+"""Records the CPU oracle's canonical DAG digests for the batching and serving
+workloads at their full sizes (oracle/oracle.cpp replaying every batch in
+order; the digest is FNV-1a over the canonical export words):
+
+  batching: 8 materials heightfield scenes of inner depth 8, 9, 10, 11, 12, 9,
+    10, 11 (worlds 1024^3..16384^3, heightfield seeds 10..17); 256 warm-up
+    batches (request stream seed 103) then 256 batches (seed 3), each 2^8..2^20
+    voxel ops of 1..16 brushes (50% spheres / 50% cubes, 60/40 add/remove);
+  serving: 16 materials heightfield scenes of inner depth 10, 11, 12, 13 x 4
+    (worlds 4096^3..32768^3, seeds 20..35); 1000 calibration requests (seed
+    1020) then 2000 requests (seed 20), 1..64 brushes of 2^6..2^18 voxel ops
+    each, applied per scene in request order.
+
+The inputs come from the library's seeded host generators (the request
+streams the benchmark and the GPU tests replay). Scenes are independent, so
+they run in parallel worker processes; the largest (inner depth 13) takes a
+few minutes of one core. Run:
+
+    make -C oracle && python tests/golden/make_workload_golden.py
+"""
+import concurrent.futures as cf
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "workloads.json")
+
+BATCH_DEPTHS = [8, 9, 10, 11, 12, 9, 10, 11]
+BATCH_SEEDS = (103, 3)      # warm-up stream, then the timed stream
+BATCH_COUNT = 256
+SERVE_DEPTHS = [10, 11, 12, 13] * 4
+SERVE_SEEDS = ((1020, 1000), (20, 2000))  # (request stream seed, count): calibration, then Poisson run
+
+
+def batching_requests():
+    from voxel_amd.serving import make_requests
+    worlds = [4 << d for d in BATCH_DEPTHS]
+    return [make_requests(BATCH_COUNT, len(BATCH_DEPTHS), worlds, s, 8, 20, 16) for s in BATCH_SEEDS]
+
+
+def serving_requests():
+    from voxel_amd.serving import make_requests
+    worlds = [4 << d for d in SERVE_DEPTHS]
+    return [make_requests(n, len(SERVE_DEPTHS), worlds, s, 6, 18, 64) for s, n in SERVE_SEEDS]
+
+
+def replay(depth, seed, batches):
+    from oracle import oracle as O
+    t0 = time.time()
+    d = O.Dag(depth, True)
+    d.build_terrain(O.terrain_for_world(4 << depth, seed))
+    for br in batches:
+        d.apply_brushes(br)
+    e = d.export()
+    out = {"depth": depth, "terrain_seed": seed, "batches": len(batches), "words": int(e.size),
+           "nodes": int(e[4]), "leaves": int(e[5]), "digest": "%016x" % O.digest(e)}
+    del e
+    print(f"depth {depth} seed {seed}: {out['nodes']} nodes {out['leaves']} leaves "
+          f"({time.time() - t0:.0f} s)", flush=True)
+    return out
+
+
+def scene_jobs(depths, seed0, streams):
+    jobs = []
+    for s, depth in enumerate(depths):
+        batches = [r.brushes for reqs in streams for r in reqs if r.scene == s]
+        jobs.append((depth, seed0 + s, batches))
+    return jobs
+
+
+def main():
+    workers = int(os.environ.get("GOLDEN_WORKERS", "4"))
+    jobs = {"batching": scene_jobs(BATCH_DEPTHS, 10, batching_requests()),
+            "serving": scene_jobs(SERVE_DEPTHS, 20, serving_requests())}
+    # Largest scenes first, so the pool's tail is short.
+    order = sorted(((k, i) for k in jobs for i in range(len(jobs[k]))), key=lambda ki: -jobs[ki[0]][ki[1]][0])
+    res = {k: [None] * len(v) for k, v in jobs.items()}
+    with cf.ProcessPoolExecutor(max_workers=workers) as ex:
+        futs = {ex.submit(replay, *jobs[k][i]): (k, i) for k, i in order}
+        for f in cf.as_completed(futs):
+            k, i = futs[f]
+            res[k][i] = f.result()
+    doc = {
+        "about": "CPU oracle canonical digests (FNV-1a over the export words) for the batching and serving "
+                 "workloads; generated by tests/golden/make_workload_golden.py",
+        "batching": {"depths": BATCH_DEPTHS, "terrain_seeds": list(range(10, 18)), "request_seeds": list(BATCH_SEEDS),
+                     "batches_per_stream": BATCH_COUNT, "log2_ops": [8, 20], "max_brushes": 16,
+                     "scenes": res["batching"]},
+        "serving": {"depths": SERVE_DEPTHS, "terrain_seeds": list(range(20, 36)),
+                    "request_streams": [list(x) for x in SERVE_SEEDS], "log2_ops": [6, 18], "max_brushes": 64,
+                    "scenes": res["serving"]},
+    }
+    with open(OUT, "w") as f:
+        json.dump(doc, f, indent=1)
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()

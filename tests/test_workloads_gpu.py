@@ -1,0 +1,105 @@
+"""The batching and serving workloads at their full sizes, bit-exact against
+the CPU oracle: every scene's canonical export (computed on the GPU) has the
+FNV-1a digest the oracle recorded for the same terrain and batches
+(tests/golden/workloads.json, written by tests/golden/make_workload_golden.py).
+
+batching: 8 materials heightfields of inner depth 8..12 (1024^3..16384^3,
+seeds 10..17), 2 x 256 batches of 2^8..2^20 voxel ops (1..16 brushes each),
+once through the persistent executor (one graph-replayed multi-scene
+pipeline per batch set) and once as plain per-batch calls.
+serving: 16 heightfields of inner depth 10..13 (4096^3..32768^3, seeds
+20..35), 1000 calibration requests served back to back and 2000 Poisson
+requests, 1..64 brushes of 2^6..2^18 voxel ops each, through the continuous-
+batching server with a 2^22-candidate executor; 16 scenes and 39 Morton bits
+leave no room for a packed record index, so its sort runs on (key, index)
+pairs.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "workloads.json")
+
+
+def _golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def _scenes(depths, seed0):
+    from voxel_amd.dag import HashDagGpu, scene_capacity, terrain_for_world
+    out = []
+    for k, d in enumerate(depths):
+        g = HashDagGpu(d, True, **scene_capacity(d))
+        g.build_terrain(terrain_for_world(4 << d, seed0 + k))
+        out.append(g)
+    return out
+
+
+def _check(scenes, golden_scenes, what):
+    for s, (g, want) in enumerate(zip(scenes, golden_scenes)):
+        e = g.export_device().cpu().numpy().view(np.uint32)
+        got = {"words": int(e.size), "nodes": int(e[4]), "leaves": int(e[5]), "digest": "%016x" % O.digest(e)}
+        exp = {k: want[k] for k in got}
+        assert got == exp, f"{what} scene {s} (inner depth {want['depth']})"
+
+
+@pytest.mark.timeout(600)
+def test_batching_workload_matches_oracle(gpu):
+    from voxel_amd.dag import Batcher, batch_candidates
+    from voxel_amd.serving import make_requests
+    G = _golden()["batching"]
+    depths = G["depths"]
+    worlds = [4 << d for d in depths]
+    lo, hi = G["log2_ops"]
+    streams = [[(r.scene, r.brushes) for r in make_requests(G["batches_per_stream"], len(depths), worlds, seed, lo,
+                                                            hi, G["max_brushes"])]
+               for seed in G["request_seeds"]]
+    batched = _scenes(depths, G["terrain_seeds"][0])
+    cap = max(batch_candidates(batched, b) for b in streams)
+    jobs = max(sum(len(br) for _, br in b) for b in streams)
+    ex = Batcher(batched, cap, jobs, use_graph=True)
+    for b in streams:
+        ex.run(b)
+    assert ex.info()[1]  # the second batch set replayed the captured graph
+    ex.close()
+    _check(batched, G["scenes"], "batched")
+    for g in batched:
+        g.close()
+    plain = _scenes(depths, G["terrain_seeds"][0])
+    for b in streams:
+        for s, br in b:
+            plain[s].apply_brushes(br)
+    _check(plain, G["scenes"], "plain")
+    for g in plain:
+        g.close()
+
+
+@pytest.mark.timeout(900)
+def test_serving_workload_matches_oracle(gpu):
+    from voxel_amd.serving import Server, make_requests, poisson_arrivals, serve
+    G = _golden()["serving"]
+    depths = G["depths"]
+    worlds = [4 << d for d in depths]
+    lo, hi = G["log2_ops"]
+    (cal_seed, n_cal), (seed, n) = G["request_streams"]
+    cal = make_requests(n_cal, len(depths), worlds, cal_seed, lo, hi, G["max_brushes"])
+    reqs = make_requests(n, len(depths), worlds, seed, lo, hi, G["max_brushes"])
+    scenes = _scenes(depths, G["terrain_seeds"][0])
+    srv = Server(scenes, max_candidates=1 << 22, max_jobs=1 << 14)
+    key_bits, index_bits = srv.ex.layout()
+    assert key_bits == 3 * 13 + 4 and index_bits == 0  # (key, index) pair records
+    sat = serve(srv, cal, [0.0] * len(cal), max_tick=256)
+    res = serve(srv, reqs, poisson_arrivals(len(reqs), 3000.0, seed))
+    assert sat["completed"] == n_cal and res["completed"] == n
+    assert res["ticks"] < n  # requests shared ticks
+    srv.close()
+    _check(scenes, G["scenes"], "served")
+    for g in scenes:
+        g.close()
