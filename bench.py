@@ -84,6 +84,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-svo", action="store_true", help="skip the pointer-SVO leg")
     ap.add_argument("--no-config0", action="store_true", help="skip the config-0 leg (GPU and reference CPU)")
+    ap.add_argument("--no-voxel-edits", action="store_true",
+                    help="skip the reference Hash DAG benchmark workloads on the voxel-edit entry point")
     ap.add_argument("--node-slots-log2", type=int, default=27, help="log2 of the scene's node slot-table size")
     ap.add_argument("--leaf-slots-log2", type=int, default=27, help="log2 of the scene's leaf slot-table size")
     ap.add_argument("--cpu-brushes", type=int, default=1024, help="brushes in the CPU baseline sample")
@@ -223,6 +225,131 @@ class RefConfig0(threading.Thread):
                                      f"{world * world} column boxes in {setup:.1f} s untimed)"}
         except Exception as e:  # reported, never fatal for the GPU figure
             self.error = repr(e)
+
+
+VOXEL_EDIT_SIZES = [1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24]
+VOXEL_PARITY_SIZE = 1 << 20
+
+
+def dense_voxels(count):
+    """make_dense_edits (bench/bench_cuda_hash_dag_gpu.cu:156-166): a cube of
+    side ceil(cbrt(count)) at the origin, x fastest, the first `count` voxels."""
+    import numpy as np
+    side = max(1, int(np.ceil(np.cbrt(count))))
+    while side ** 3 < count:
+        side += 1
+    i = np.arange(count, dtype=np.uint64)
+    return np.stack([i % side, (i // side) % side, i // (side * side)], 1).astype(np.uint32)
+
+
+class VoxelParity(threading.Thread):
+    """CPU oracle digests of the reference benchmark's placements at 2^20
+    voxels on the 1024^3 world (dense cube and xorshift32 voxels), for the
+    GPU's canonical exports of the same placements."""
+
+    def __init__(self):
+        super().__init__(daemon=True)
+        self.digests = {}
+        self.error = None
+
+    def run(self):
+        try:
+            from oracle import oracle as O
+            from voxel_amd.svo import random_edits
+            for name, xyz in (("dense", dense_voxels(VOXEL_PARITY_SIZE)),
+                              ("random", random_edits(VOXEL_PARITY_SIZE, 1024))):
+                d = O.Dag(8, False)
+                d.apply_voxels(xyz, 1)
+                self.digests[name] = O.digest(d.export())
+        except Exception as e:  # reported, never fatal for the GPU figure
+            self.error = repr(e)
+
+
+def leg_voxel_edits(V, torch, parity=None, reps=3):
+    """The reference Hash DAG benchmark's workloads on the drop-in voxel-edit
+    entry point (bench/bench_cuda_hash_dag_gpu.cu:416-467): a 1024^3
+    occupancy DAG (inner depth 8); place on an emptied DAG, destroy the same
+    voxels after placing them, and get_voxel of the same coordinates, for a
+    dense cube and for xorshift32 voxels, 2^16..2^24 per call (hipEvent
+    median of `reps`). At 2^24 one profiled placement gives the dominant
+    kernel's achieved bytes; at 2^20 the placements' canonical digests are
+    compared with the CPU oracle's."""
+    import numpy as np
+    from voxel_amd.svo import random_edits
+    gens = {"dense": dense_voxels, "random": lambda n: random_edits(n, 1024)}
+    d = V.HashDagGpu(8, False, node_words=1 << 27, leaf_words=1 << 26, node_slots=1 << 25, leaf_slots=1 << 25)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    rows, prof_rows, digests = [], {}, {}
+    for name, gen in gens.items():
+        for n in VOXEL_EDIT_SIZES:
+            xyz = torch.from_numpy(gen(n).view(np.int32)).cuda()
+            ws = torch.empty(d.apply_voxel_edits_workspace_size(n), dtype=torch.uint8, device="cuda")
+            t = {"place": [], "destroy": [], "get": []}
+            hits = 0
+            for r in range(reps + 1):
+                d.reset()
+                torch.cuda.synchronize()
+                ev[0].record()
+                d.place_voxel_edits(xyz, workspace=ws)
+                ev[1].record()
+                ev[1].synchronize()
+                if r == 0 and n == VOXEL_PARITY_SIZE:
+                    from oracle import oracle as O  # the checker's digest of the GPU export
+                    digests[name] = O.digest(d.export())
+                ev[2].record()
+                got = d.get_voxels(xyz)
+                ev[3].record()
+                ev[3].synchronize()
+                if r == 0:
+                    hits = int(got.sum().item())
+                ev[4].record()
+                d.destroy_voxel_edits(xyz, workspace=ws)
+                ev[5].record()
+                ev[5].synchronize()
+                if r:
+                    t["place"].append(ev[0].elapsed_time(ev[1]))
+                    t["get"].append(ev[2].elapsed_time(ev[3]))
+                    t["destroy"].append(ev[4].elapsed_time(ev[5]))
+            empty_after = int(d.export()[3]) == 0
+            row = {"workload": name, "voxels": n, "get_hits": hits, "empty_after_destroy": empty_after}
+            for k, v in t.items():
+                ms = sorted(v)[len(v) // 2]
+                row[f"{k}_ms"] = ms
+                row[f"{k}_per_s"] = n / ms * 1e3
+            rows.append(row)
+            if n == VOXEL_EDIT_SIZES[-1]:
+                d.reset()
+                d.profile(True)
+                d.place_voxel_edits(xyz, workspace=ws)
+                torch.cuda.synchronize()
+                p = d.profile_read()
+                d.profile(False)
+                kb = kernel_bytes(p, 2)
+                ms = {k: p.kernel_ms[i] for i, k in enumerate(V.KERNELS) if p.kernel_launches[i]}
+                dom = max(ms, key=ms.get)
+                di = V.KERNELS.index(dom)
+                avg_us = p.kernel_ms[di] * 1e3 / max(p.kernel_launches[di], 1)
+                bpl = kb.get(dom, 0) / max(p.kernel_launches[di], 1)
+                gbs = bpl / (avg_us * 1e-6) / 1e9 if avg_us > 0 and bpl else None
+                prof_rows[name] = {"dominant_kernel": dom, "launches": p.kernel_launches[di],
+                                   "avg_launch_us": avg_us, "bytes_per_launch": bpl, "achieved_gbs": gbs,
+                                   "frac": gbs / HBM_PEAK_GBS if gbs else None,
+                                   "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+                                   "records": p.records, "leaves": p.leaves, "nodes": p.nodes}
+            del xyz, ws
+    d.close()
+    out = {"workload": "reference Hash DAG benchmark: 1024^3 occupancy DAG, place / destroy / get_voxel of a dense "
+                       "cube and of xorshift32 voxels, 2^16..2^24 per call",
+           "rows": rows, "profile_2^24_place": prof_rows}
+    if parity is not None:
+        parity.join()
+        out["parity_2^20"] = {"check": "canonical DAG digest after placing 2^20 voxels: GPU vs CPU oracle",
+                              "gpu": {k: f"{v:016x}" for k, v in digests.items()},
+                              "cpu": {k: f"{v:016x}" for k, v in parity.digests.items()},
+                              "match": bool(parity.digests) and all(parity.digests.get(k) == v
+                                                                    for k, v in digests.items()),
+                              "error": parity.error}
+    return out
 
 
 def config0_leg(V, torch, reps=5):
@@ -459,6 +586,33 @@ def _sample_equal(V, torch, a, b, n=1 << 20, seed=0):
     return bool(torch.equal(a.get_voxels(q), b.get_voxels(q)))
 
 
+GOLDEN_WORKLOADS = os.path.join(ROOT, "tests", "golden", "workloads.json")
+
+
+def _golden_parity(key, groups):
+    """Every scene's canonical export (computed on the GPU) against the CPU
+    oracle's digest for the same terrain and batches, recorded in
+    tests/golden/workloads.json by tests/golden/make_workload_golden.py."""
+    import numpy as np
+
+    from oracle import oracle as O  # the checker's digest of the GPU exports
+    with open(GOLDEN_WORKLOADS) as f:
+        want = json.load(f)[key]["scenes"]
+    out = {"check": "canonical DAG digest of every scene vs the CPU oracle's (tests/golden/workloads.json)"}
+    t0 = time.perf_counter()
+    ok = True
+    for name, scenes in groups.items():
+        got = []
+        for g in scenes:
+            e = g.export_device().cpu().numpy().view(np.uint32)
+            got.append("%016x" % O.digest(e))
+        out[name] = got
+        ok = ok and got == [w["digest"] for w in want]
+    out["match"] = ok
+    out["export_s"] = time.perf_counter() - t0
+    return out
+
+
 def leg_batched(V, torch, n_batches=256, seed=3):
     """256 batches (log-uniform 2^8..2^20 voxel ops, config-2 shapes scaled)
     over 8 resident scenes of inner depth 8..12 (worlds 1024^3..16384^3,
@@ -499,6 +653,7 @@ def leg_batched(V, torch, n_batches=256, seed=3):
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t0
     same = all(_sample_equal(V, torch, p, b, seed=k) for k, (p, b) in enumerate(zip(plain, batched)))
+    parity = _golden_parity("batching", {"plain": plain, "batched": batched})
     ex.close()
     for g in plain + batched:
         g.close()
@@ -511,6 +666,7 @@ def leg_batched(V, torch, n_batches=256, seed=3):
         "batched_edits_per_s": ops / t_batched,
         "speedup": t_plain / t_batched,
         "same_result_sampled": same,
+        "parity": parity,
     }
 
 
@@ -535,6 +691,7 @@ def leg_serving(V, torch, n_requests=2000, seed=20, load=0.7):
     srv.ticks, srv.tick_sizes = 0, []
     res = serve(srv, reqs, poisson_arrivals(len(reqs), rate, seed))
     srv.close()
+    res["parity"] = _golden_parity("serving", {"served": scenes})
     for g in scenes:
         g.close()
     res.update({"arrival_rate_per_s": rate, "saturated_requests_per_s": sat["requests"] / sat["makespan_s"],
@@ -580,7 +737,7 @@ def kernel_bytes(prof, leaf_words):
 PMC_COMMAND = ("cd /tmp && export TMPDIR=/tmp && cd <repo> && "
                "rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmc_{tag} -o {tag} -- "
                "python3 bench.py --warmup 1 --profile-steps 0 --steps 1 --no-cpu --no-primitives --no-batched "
-               "--no-serving --no-svo --no-config0 --copy-calibration")
+               "--no-serving --no-svo --no-config0 --no-voxel-edits --copy-calibration")
 
 
 def _kind_of(name, last_probe):
@@ -750,10 +907,13 @@ def main():
     batches = [V.gen_brushes(args.seed, V.STREAM_BRUSH + (k << 8), args.brushes, world, 1)
                for k in range(args.warmup + n_prof + n_conf + args.steps)]
     check = not args.no_cpu and not args.no_parity and args.warmup > 0
-    cpu = ref0 = None
+    cpu = ref0 = vpar = None
     if not args.no_cpu:
         cpu = CpuBaseline(args.inner_depth, args.seed, args.cpu_brushes, batches[0] if check else None)
         cpu.start()
+        if not args.no_voxel_edits:
+            vpar = VoxelParity()
+            vpar.start()
         if not args.no_config0:
             ref0 = RefConfig0()
             ref0.start()
@@ -850,6 +1010,14 @@ def main():
     bytes_per_launch = kernel_bytes(prof, 16).get(dom, 0) / max(launches_all, 1)
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
 
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    exp = scene.export_device()
+    torch.cuda.synchronize()
+    export = {"workload": f"canonical export of the edited {world}^3 scene, computed on the GPU",
+              "s": time.perf_counter() - t0, "words": int(exp.numel()), "nodes": int(exp[4].item()),
+              "leaves": int(exp[5].item())}
+    del exp
     rays = ray_leg(torch, scene, world, dev)
     gc = gc_leg(torch, scene, world, dev)
     scene.close()
@@ -865,6 +1033,8 @@ def main():
         prims.update(hash_tables(torch, P))
     if not args.no_primitives and not args.no_vendor_sort and os.path.exists(VENDOR_SORT_BIN):
         prims["vendor"] = vendor_sort()
+    torch.cuda.empty_cache()
+    voxel_edits = None if args.no_voxel_edits else leg_voxel_edits(V, torch, vpar)
     torch.cuda.empty_cache()
     batched = None if args.no_batched else leg_batched(V, torch)
     torch.cuda.empty_cache()
@@ -943,10 +1113,12 @@ def main():
         "dominance_check_ms_per_edit": confirm,
         "scene": {"build_s": build_s, "node_words": st.node_words_used, "leaf_words": st.leaf_words_used},
         "primitives": prims,
+        "export": export,
         "rays": rays,
         "gc": gc,
         "svo": svo,
         "config0": c0,
+        "voxel_edits": voxel_edits,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu_res,

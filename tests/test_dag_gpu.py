@@ -434,3 +434,69 @@ def test_batcher_keeps_its_sort_tuning(gpu):
         b[s].apply_brushes(br)
     for k in range(2):
         assert np.array_equal(a[k].export(), b[k].export()), k
+
+
+def _dense_cube(count):
+    """make_dense_edits (bench/bench_cuda_hash_dag_gpu.cu:156-166)."""
+    side = max(1, int(np.ceil(np.cbrt(count))))
+    while side ** 3 < count:
+        side += 1
+    i = np.arange(count, dtype=np.uint64)
+    return np.stack([i % side, (i // side) % side, i // (side * side)], 1).astype(np.uint32)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("workload", ["dense", "random"])
+def test_reference_benchmark_workloads_match_oracle(gpu, workload):
+    """The reference Hash DAG benchmark's inputs at 2^20 voxels on the 1024^3
+    world (bench/bench_cuda_hash_dag_gpu.cu:156-190): place on an empty DAG
+    (canonical export equal to the oracle's), get_voxel of the placed voxels
+    (all set), destroy half (equal to the oracle again), destroy all (empty)."""
+    from voxel_amd.svo import random_edits
+    n = 1 << 20
+    xyz = _dense_cube(n) if workload == "dense" else random_edits(n, 1024)
+    d = _dag(8, False, node_words=1 << 24, leaf_words=1 << 24, node_slots=1 << 22, leaf_slots=1 << 22)
+    dx = _dev_xyz(xyz, gpu)
+    d.place_voxel_edits(dx)
+    ref = O.Dag(8, False)
+    ref.apply_voxels(xyz, 1)
+    assert np.array_equal(d.export(), ref.export())
+    assert int(d.get_voxels(dx).sum().item()) == n
+    half = xyz[: n // 2]
+    d.destroy_voxel_edits(_dev_xyz(half, gpu))
+    ref.apply_voxels(half, 0)
+    assert np.array_equal(d.export(), ref.export())
+    d.destroy_voxel_edits(dx)
+    assert d.export()[3] == 0
+
+
+def test_corner_miss_spheres_leave_scene_unchanged(gpu):
+    """Spheres whose clipped bounding boxes hold brick rows but no voxel of the
+    sphere (centre (-100,-100,-100), r 150: the box reaches the world corner,
+    the sphere does not) give no records; the scene is unchanged."""
+    from voxel_amd.dag import terrain_for_world
+    br = np.zeros(3, O.BRUSH_DTYPE)
+    br[0] = (-100, -100, -100, 150, 0, 0, 0, 7)
+    br[1] = (-100, -100, -100, 150, 0, 0, 0, 0)
+    br[2] = (1100, 1100, -100, 130, 0, 0, 0, 9)  # nearest world voxel at distance^2 21858 > 130^2
+    d = _dag(8, True, node_words=1 << 23, leaf_words=1 << 24)
+    d.build_terrain(terrain_for_world(1024, 6))
+    before = d.export()
+    d.apply_brushes(br)
+    assert np.array_equal(d.export(), before)
+    ref = O.Dag(8, True)
+    ref.build_terrain(O.terrain_for_world(1024, 6))
+    ref.apply_brushes(br)
+    assert np.array_equal(before, ref.export())
+
+
+def test_invalid_brush_kind_or_material_is_rejected(gpu):
+    from voxel_amd import InvalidValue
+    d = _dag(5, True)
+    for field, value in (("kind", 2), ("material", 256)):
+        br = np.zeros(1, O.BRUSH_DTYPE)
+        br[0] = (10, 10, 10, 3, 3, 3, 1, 5)
+        br[0][field] = value
+        with pytest.raises(InvalidValue):
+            d.apply_brushes(br)
+    assert d.export()[3] == 0

@@ -419,14 +419,3 @@ class Batcher:
             return False, 0, None, None
         return True, r, st, used.reshape(n, 2)
 
-
-def smoke_check(dev) -> None:
-    """Tiny end-to-end edit checked against the oracle (used by smoke())."""
-    from oracle import oracle as O
-    d = HashDagGpu(4, materials=True)
-    br = O.gen_brushes(3, O.STREAM_BRUSH, 12, d.world, 0)
-    d.apply_brushes(br)
-    ref = O.Dag(4, True)
-    ref.apply_brushes(br)
-    assert np.array_equal(d.export(), ref.export()), "GPU DAG differs from the oracle"
-    d.close()
