@@ -208,6 +208,58 @@ vx_status vx_cuckoo_set_insert(vx_cuckoo_set* set, const uint64_t* d_keys, uint3
 vx_status vx_cuckoo_set_contains(const vx_cuckoo_set* set, const uint64_t* d_keys, uint32_t n, uint8_t* d_found,
                                  void* stream);
 
+/* Content-addressed set / map of items of item_words u32 words (1..16):
+ * replaces AccelerationHashSet32<W> / AccelerationHashMap32<W>
+ * (cuda/hash_table/acceleration_hash/set/host.cuh:15-117, map/host.cuh:15-116).
+ * Capacity is the reference's, (buckets + overflow_slabs) * 31 items.
+ *   find: d_ptrs[i] = a non-zero handle and status 6 (Found) when an equal
+ *     item (all words compared) is stored, else 0 and status 0 (NotFound);
+ *     a map also returns the item's value (0 when not found).
+ *   insert_unique_unchecked: stores every item (the caller guarantees no
+ *     equal item is present; equal items inserted anyway are all stored and
+ *     find returns one of them, the same for every caller), status 1
+ *     (Inserted) with its handle, or 4 (Overflow) with handle 0 once the
+ *     capacity is used up.
+ *   read: the item (and value) behind each handle; d_ok[i] = 0 for a handle
+ *     that names no stored item.
+ *   view: the device layout, for the device API of include/vx_accel_device.hpp
+ *     (find / insert_unique_unchecked / read_item from the caller's kernels,
+ *     the reference's DeviceAccelerationHashSet32 / Map32).
+ * d_ptrs / d_status / d_values / d_ok are device arrays and may be NULL. */
+typedef struct vx_accel_view {
+  unsigned long long* slots; /* slot_mask + 1 slots {fingerprint:32 | item index + 1:32}, 0 = empty */
+  uint64_t slot_mask;
+  uint32_t* items;           /* capacity * words */
+  uint32_t* values;          /* capacity values (map), NULL (set) */
+  uint32_t* used;            /* device counter of item indices handed out */
+  uint32_t capacity;
+  uint32_t words;
+} vx_accel_view;
+typedef struct vx_accel_set vx_accel_set;
+vx_status vx_accel_set_create(uint32_t item_words, uint32_t buckets, uint32_t overflow_slabs, vx_accel_set** out);
+vx_status vx_accel_set_destroy(vx_accel_set* set);
+vx_status vx_accel_set_clear(vx_accel_set* set, void* stream);
+vx_status vx_accel_set_find(const vx_accel_set* set, const uint32_t* d_items, uint32_t n, uint32_t* d_ptrs,
+                            uint32_t* d_status, void* stream);
+vx_status vx_accel_set_insert_unique_unchecked(vx_accel_set* set, const uint32_t* d_items, uint32_t n,
+                                               uint32_t* d_ptrs, uint32_t* d_status, void* stream);
+vx_status vx_accel_set_read(const vx_accel_set* set, const uint32_t* d_ptrs, uint32_t n, uint32_t* d_items,
+                            uint32_t* d_ok, void* stream);
+vx_status vx_accel_set_view(const vx_accel_set* set, vx_accel_view* out);
+vx_status vx_accel_set_used(const vx_accel_set* set, uint32_t* items);
+typedef struct vx_accel_map vx_accel_map;
+vx_status vx_accel_map_create(uint32_t item_words, uint32_t buckets, uint32_t overflow_slabs, vx_accel_map** out);
+vx_status vx_accel_map_destroy(vx_accel_map* map);
+vx_status vx_accel_map_clear(vx_accel_map* map, void* stream);
+vx_status vx_accel_map_find(const vx_accel_map* map, const uint32_t* d_items, uint32_t n, uint32_t* d_ptrs,
+                            uint32_t* d_values, uint32_t* d_status, void* stream);
+vx_status vx_accel_map_insert_unique_unchecked(vx_accel_map* map, const uint32_t* d_items, const uint32_t* d_values,
+                                               uint32_t n, uint32_t* d_ptrs, uint32_t* d_status, void* stream);
+vx_status vx_accel_map_read(const vx_accel_map* map, const uint32_t* d_ptrs, uint32_t n, uint32_t* d_items,
+                            uint32_t* d_values, uint32_t* d_ok, void* stream);
+vx_status vx_accel_map_view(const vx_accel_map* map, vx_accel_view* out);
+vx_status vx_accel_map_used(const vx_accel_map* map, uint32_t* items);
+
 /* ---------------------------------------------------------------------------
  * Hash DAG. Replaces algo::svt::hash_dag_gpu (HashDagGpu storage.cuh:16-105,
  * apply/place/destroy_voxel_edits edit.cuh:16-61, get_voxel query.cuh:12-39)

@@ -37,7 +37,29 @@ register({
     "vx_cuckoo_set_clear": (_i32, [_P, _P]),
     "vx_cuckoo_set_insert": (_i32, [_P, _P, _u32, _P, _P]),
     "vx_cuckoo_set_contains": (_i32, [_P, _P, _u32, _P, _P]),
+    "vx_accel_set_create": (_i32, [_u32, _u32, _u32, C.POINTER(_P)]),
+    "vx_accel_set_destroy": (_i32, [_P]),
+    "vx_accel_set_clear": (_i32, [_P, _P]),
+    "vx_accel_set_find": (_i32, [_P, _P, _u32, _P, _P, _P]),
+    "vx_accel_set_insert_unique_unchecked": (_i32, [_P, _P, _u32, _P, _P, _P]),
+    "vx_accel_set_read": (_i32, [_P, _P, _u32, _P, _P, _P]),
+    "vx_accel_set_view": (_i32, [_P, _P]),
+    "vx_accel_set_used": (_i32, [_P, C.POINTER(_u32)]),
+    "vx_accel_map_create": (_i32, [_u32, _u32, _u32, C.POINTER(_P)]),
+    "vx_accel_map_destroy": (_i32, [_P]),
+    "vx_accel_map_clear": (_i32, [_P, _P]),
+    "vx_accel_map_find": (_i32, [_P, _P, _u32, _P, _P, _P, _P]),
+    "vx_accel_map_insert_unique_unchecked": (_i32, [_P, _P, _P, _u32, _P, _P, _P]),
+    "vx_accel_map_read": (_i32, [_P, _P, _u32, _P, _P, _P, _P]),
+    "vx_accel_map_view": (_i32, [_P, _P]),
+    "vx_accel_map_used": (_i32, [_P, C.POINTER(_u32)]),
 })
+
+
+class AccelView(C.Structure):
+    """vx_accel_view: the device layout a caller's kernel works on."""
+    _fields_ = [("slots", C.c_void_p), ("slot_mask", C.c_uint64), ("items", C.c_void_p), ("values", C.c_void_p),
+                ("used", C.c_void_p), ("capacity", C.c_uint32), ("words", C.c_uint32)]
 
 
 def slab_buckets_for(n_keys: int, load: float = 0.8, slots_per_slab: int = 63) -> int:
@@ -136,3 +158,91 @@ class CuckooSetU64(_Owned):
         check(lib().vx_cuckoo_set_contains(self.h, _ptr(keys), keys.numel(), _ptr(found), _stream(stream)),
               "vx_cuckoo_set_contains")
         return found
+
+
+class AccelerationHashSet32(_Owned):
+    """AccelerationHashSet32<W> (acceleration_hash/set/host.cuh:15-117): a
+    content-addressed set of W-word items; find_batch /
+    insert_unique_unchecked_batch / read_items with handles (0 = none) and
+    status codes (FOUND / NOT_FOUND, INSERTED / OVERFLOW)."""
+    _destroy = "vx_accel_set_destroy"
+    _map = False
+
+    def __init__(self, item_words: int, bucket_count: int, overflow_slab_capacity: int):
+        self.h = C.c_void_p()
+        self.item_words = item_words
+        prefix = "vx_accel_map" if self._map else "vx_accel_set"
+        self._p = prefix
+        check(getattr(lib(), prefix + "_create")(item_words, bucket_count, overflow_slab_capacity, C.byref(self.h)),
+              prefix + "_create")
+
+    def clear(self, stream=None):
+        check(getattr(lib(), self._p + "_clear")(self.h, _stream(stream)), self._p + "_clear")
+
+    def view(self) -> AccelView:
+        v = AccelView()
+        check(getattr(lib(), self._p + "_view")(self.h, C.byref(v)), self._p + "_view")
+        return v
+
+    def used(self) -> int:
+        n = C.c_uint32()
+        check(getattr(lib(), self._p + "_used")(self.h, C.byref(n)), self._p + "_used")
+        return n.value
+
+    def _n(self, items: torch.Tensor) -> int:
+        if items.dim() != 2 or items.shape[1] != self.item_words or items.element_size() != 4:
+            raise ValueError("items are an (n, item_words) 32-bit tensor")
+        return items.shape[0]
+
+    def find_batch(self, items: torch.Tensor, stream=None):
+        """(handles, status) of every item; a map also returns the values."""
+        n = self._n(items)
+        ptrs = torch.zeros(n, dtype=torch.int32, device=items.device)
+        st = torch.zeros(n, dtype=torch.int32, device=items.device)
+        if self._map:
+            vals = torch.zeros(n, dtype=torch.int32, device=items.device)
+            check(lib().vx_accel_map_find(self.h, _ptr(items), n, _ptr(ptrs), _ptr(vals), _ptr(st), _stream(stream)),
+                  "vx_accel_map_find")
+            return ptrs, vals, st
+        check(lib().vx_accel_set_find(self.h, _ptr(items), n, _ptr(ptrs), _ptr(st), _stream(stream)),
+              "vx_accel_set_find")
+        return ptrs, st
+
+    def read_items(self, ptrs: torch.Tensor, stream=None):
+        """(items, ok) behind each handle; a map returns (items, values, ok)."""
+        n = ptrs.numel()
+        items = torch.zeros((n, self.item_words), dtype=torch.int32, device=ptrs.device)
+        ok = torch.zeros(n, dtype=torch.int32, device=ptrs.device)
+        if self._map:
+            vals = torch.zeros(n, dtype=torch.int32, device=ptrs.device)
+            check(lib().vx_accel_map_read(self.h, _ptr(ptrs), n, _ptr(items), _ptr(vals), _ptr(ok), _stream(stream)),
+                  "vx_accel_map_read")
+            return items, vals, ok
+        check(lib().vx_accel_set_read(self.h, _ptr(ptrs), n, _ptr(items), _ptr(ok), _stream(stream)),
+              "vx_accel_set_read")
+        return items, ok
+
+    def insert_unique_unchecked_batch(self, items: torch.Tensor, stream=None):
+        n = self._n(items)
+        ptrs = torch.zeros(n, dtype=torch.int32, device=items.device)
+        st = torch.zeros(n, dtype=torch.int32, device=items.device)
+        check(lib().vx_accel_set_insert_unique_unchecked(self.h, _ptr(items), n, _ptr(ptrs), _ptr(st),
+                                                         _stream(stream)), "vx_accel_set_insert_unique_unchecked")
+        return ptrs, st
+
+
+class AccelerationHashMap32(AccelerationHashSet32):
+    """AccelerationHashMap32<W> (acceleration_hash/map/host.cuh:15-116): the
+    set with one u32 value per stored item."""
+    _destroy = "vx_accel_map_destroy"
+    _map = True
+
+    def insert_unique_unchecked_batch(self, items: torch.Tensor, values: torch.Tensor, stream=None):
+        n = self._n(items)
+        if values.numel() != n:
+            raise ValueError("one value per item")
+        ptrs = torch.zeros(n, dtype=torch.int32, device=items.device)
+        st = torch.zeros(n, dtype=torch.int32, device=items.device)
+        check(lib().vx_accel_map_insert_unique_unchecked(self.h, _ptr(items), _ptr(values), n, _ptr(ptrs), _ptr(st),
+                                                         _stream(stream)), "vx_accel_map_insert_unique_unchecked")
+        return ptrs, st
