@@ -165,7 +165,8 @@ vx_status vx_iota_u32(uint32_t* out, uint32_t n, void* stream);
  * Slab set of u64 keys: replaces SlabHashSetU32 (slab_hash/set/host.cuh:15-107,
  * device.cuh:32-316) with 64-bit keys and 512-byte slabs (63 slots + link,
  * one u64 per lane of a 64-lane wavefront). Keys ~0 and ~0-1 are reserved
- * (InvalidKey). Overflow slabs come from a bump allocator.
+ * (InvalidKey). Overflow slabs come from the slab allocator (vx_slab_alloc
+ * below); slabs_used = buckets + overflow slabs in use.
  * ------------------------------------------------------------------------- */
 typedef struct vx_slab_set vx_slab_set;
 vx_status vx_slab_set_create(uint32_t buckets, uint32_t overflow_slabs, vx_slab_set** out);
@@ -178,6 +179,43 @@ vx_status vx_slab_set_contains(const vx_slab_set* set, const uint64_t* d_keys, u
 vx_status vx_slab_set_erase(vx_slab_set* set, const uint64_t* d_keys, uint32_t n, uint32_t* d_status /*opt*/,
                             void* stream);
 vx_status vx_slab_set_slabs_used(const vx_slab_set* set, uint32_t* used);
+
+/* Slab set of u32 keys: replaces SlabHashSetU32 (slab_hash/set/host.cuh:15-107)
+ * with the same slabs as the u64 set (keys stored zero-extended); keys
+ * 0xffffffff and 0xfffffffe are reserved (InvalidKey, slab_hash/common.cuh:10-30). */
+typedef struct vx_slab_set_u32 vx_slab_set_u32;
+vx_status vx_slab_set_u32_create(uint32_t buckets, uint32_t overflow_slabs, vx_slab_set_u32** out);
+vx_status vx_slab_set_u32_destroy(vx_slab_set_u32* set);
+vx_status vx_slab_set_u32_clear(vx_slab_set_u32* set, void* stream);
+vx_status vx_slab_set_u32_insert(vx_slab_set_u32* set, const uint32_t* d_keys, uint32_t n, uint32_t* d_status /*opt*/,
+                                 void* stream);
+vx_status vx_slab_set_u32_contains(const vx_slab_set_u32* set, const uint32_t* d_keys, uint32_t n,
+                                   uint8_t* d_found /*opt*/, uint32_t* d_status /*opt*/, void* stream);
+vx_status vx_slab_set_u32_erase(vx_slab_set_u32* set, const uint32_t* d_keys, uint32_t n, uint32_t* d_status /*opt*/,
+                                void* stream);
+vx_status vx_slab_set_u32_slabs_used(const vx_slab_set_u32* set, uint32_t* used);
+
+/* Reusable slab allocator: replaces SlabAlloc (cuda/hash_table/slab_alloc.cuh:26-251),
+ * the default allocator of the slab tables above (whose overflow slabs are
+ * its ordinals; clearing a table frees them all). Ordinals 0..capacity-1
+ * in a bitmap of one u64 word per 64 slabs; allocate writes one ordinal per
+ * request (0xffffffff once every ordinal is taken), deallocate frees them
+ * (out-of-range ordinals and double frees are no-ops) for later allocations.
+ * used = allocated ordinals (synchronises the device). The view is the
+ * layout the device API of include/vx_slab_alloc_device.hpp works on. */
+typedef struct vx_slab_alloc_view {
+  unsigned long long* bitmap; /* words u64 words, bit set = allocated */
+  uint32_t capacity;
+  uint32_t words;
+} vx_slab_alloc_view;
+typedef struct vx_slab_alloc vx_slab_alloc;
+vx_status vx_slab_alloc_create(uint32_t capacity, vx_slab_alloc** out);
+vx_status vx_slab_alloc_destroy(vx_slab_alloc* alloc);
+vx_status vx_slab_alloc_clear(vx_slab_alloc* alloc, void* stream);
+vx_status vx_slab_alloc_allocate(vx_slab_alloc* alloc, uint32_t n, uint32_t* d_ordinals, void* stream);
+vx_status vx_slab_alloc_deallocate(vx_slab_alloc* alloc, const uint32_t* d_ordinals, uint32_t n, void* stream);
+vx_status vx_slab_alloc_used(const vx_slab_alloc* alloc, uint32_t* used);
+vx_status vx_slab_alloc_view_get(const vx_slab_alloc* alloc, vx_slab_alloc_view* out);
 
 /* Slab map u32 -> u32: replaces SlabHashMapU32 (slab_hash/map/host.cuh:15-108,
  * device.cuh:49-393); each slot packs key << 32 | value and changes by one

@@ -13,6 +13,9 @@
 // The cuckoo set follows cuckoo_hash.cuh:73-120 (four salted hash functions,
 // atomicExch eviction chains), storing the u64 key itself in the slot so one
 // 64-bit exchange moves a whole entry.
+#include <vector>
+
+#include "../../include/vx_slab_alloc_device.hpp"
 #include "vx_common.hpp"
 
 namespace vx {
@@ -38,17 +41,19 @@ struct SlabTable {
   uint64_t* slabs;     // [slab][64]
   uint32_t buckets;
   uint32_t capacity;   // total slabs (buckets + overflow)
-  uint32_t* alloc;     // next free overflow slab
+  vx_slab_alloc_view alloc;  // overflow slabs: slab buckets + ordinal
 };
 
 __host__ __device__ __forceinline__ uint32_t bucket_of(uint64_t key, uint32_t buckets) {
   return uint32_t(mix64(key) % buckets);
 }
 
-// Bump allocation of an overflow slab (slabs are pre-cleared to ~0).
-__device__ __forceinline__ uint32_t alloc_slab(const SlabTable& t) {
-  const uint32_t id = atomicAdd(t.alloc, 1u);
-  return id < t.capacity ? id : kNull;
+// An overflow slab from the table's slab allocator (SlabAlloc, the
+// reference's default, slab_hash/set/host.cuh:107): overflow slabs are
+// pre-cleared to ~0, and the hint spreads concurrent callers over the bitmap.
+__device__ __forceinline__ uint32_t alloc_slab(const SlabTable& t, uint32_t hint) {
+  const uint32_t o = vx_slab_alloc_dev::allocate(t.alloc, hint);
+  return o == vx_slab_alloc_dev::kNull ? kNull : t.buckets + o;
 }
 
 // Link word values (low 32 bits of lane 63's word) besides slab ids.
@@ -113,7 +118,7 @@ __device__ uint32_t group_link(const SlabTable& t, uint64_t* s, const Group& G) 
     const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s[63]), (unsigned long long)kEmpty64,
                                     (unsigned long long)(0xffffffff00000000ull | kLinkLock));
     if (prev == kEmpty64) {
-      const uint32_t id = alloc_slab(t);
+      const uint32_t id = alloc_slab(t, blockIdx.x * blockDim.x + threadIdx.x);
       link = id == kNull ? kLinkFull : id;
       st_agent(&s[63], id == kNull ? uint64_t(0xffffffff00000000ull | kLinkFull) : uint64_t(id));
     } else {
@@ -151,8 +156,15 @@ __device__ __forceinline__ uint64_t own_word(const uint64_t (&v)[kLaneWords], ui
 // ---- set --------------------------------------------------------------------
 enum class SetOp { kInsert, kContains, kErase };
 
-template <SetOp OP>
-__global__ void slab_set_kernel(SlabTable t, const uint64_t* __restrict__ keys, uint32_t n, uint8_t* found,
+// Keys of K bits (u64, or u32 stored zero-extended); the two largest values
+// of K are reserved (empty and deleted slots, slab_hash/common.cuh:10-30).
+template <class K>
+__device__ __forceinline__ bool reserved_key(uint64_t key) {
+  return key == uint64_t(K(~K(0))) || key == uint64_t(K(~K(0)) - 1);
+}
+
+template <SetOp OP, class K = uint64_t>
+__global__ void slab_set_kernel(SlabTable t, const K* __restrict__ keys, uint32_t n, uint8_t* found,
                                 uint32_t* status) {
   const uint32_t lane = lane_id();
   const Group G = my_group();
@@ -160,10 +172,10 @@ __global__ void slab_set_kernel(SlabTable t, const uint64_t* __restrict__ keys, 
   const uint32_t wave_global = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   for (uint32_t base = wave_global * kWave; base < n; base += waves * kWave) {
     const uint32_t i = base + lane;
-    uint64_t key = i < n ? keys[i] : 0;
+    uint64_t key = i < n ? uint64_t(keys[i]) : 0;
     bool active = i < n;
     uint32_t st = kNotFound;
-    if (active && (key == kEmpty64 || key == kTomb64)) {
+    if (active && reserved_key<K>(key)) {
       st = kInvalidKey;
       active = false;
     }
@@ -437,10 +449,14 @@ using namespace vx::hash;
 struct vx_slab_table {
   SlabTable t{};
   uint64_t* mem = nullptr;
-  uint32_t* alloc = nullptr;
+  unsigned long long* bitmap = nullptr;
 };
 struct vx_slab_set : vx_slab_table {};
+struct vx_slab_set_u32 : vx_slab_table {};
 struct vx_slab_map : vx_slab_table {};
+struct vx_slab_alloc {
+  vx_slab_alloc_view v{};
+};
 struct vx_cuckoo_set {
   uint64_t* table = nullptr;
   uint64_t slots = 0;
@@ -449,28 +465,42 @@ struct vx_cuckoo_set {
 
 namespace {
 
+uint32_t bitmap_words(uint32_t capacity) { return (capacity + 63) / 64; }
+
 vx_status slab_create(vx_slab_table* h, uint32_t buckets, uint32_t overflow) {
   if (buckets == 0) return VX_INVALID_VALUE;
   const uint64_t cap = uint64_t(buckets) + overflow;
   if (cap >= kNull) return VX_INVALID_VALUE;
   if (hipMalloc(&h->mem, cap * kWave * 8) != hipSuccess) return VX_OUT_OF_CAPACITY;
-  if (hipMalloc(&h->alloc, 256) != hipSuccess) return VX_OUT_OF_CAPACITY;
-  h->t = SlabTable{h->mem, buckets, uint32_t(cap), h->alloc};
+  if (hipMalloc(&h->bitmap, std::max<size_t>(256, size_t(bitmap_words(overflow)) * 8)) != hipSuccess)
+    return VX_OUT_OF_CAPACITY;
+  h->t = SlabTable{h->mem, buckets, uint32_t(cap), vx_slab_alloc_view{h->bitmap, overflow, bitmap_words(overflow)}};
   return VX_OK;
 }
 
-__global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
-
+// Every slab empty and every overflow slab free (the allocator reset with
+// the table, as SlabHashSetU32::clear does).
 vx_status slab_clear(vx_slab_table* h, hipStream_t s) {
   vx_status st = fill_u32(reinterpret_cast<uint32_t*>(h->mem), uint64_t(h->t.capacity) * kWave * 2, 0xffffffffu, s);
   if (st != VX_OK) return st;
-  set_u32_kernel<<<1, 1, 0, s>>>(h->alloc, h->t.buckets);
-  return last_launch();
+  return fill_u32(reinterpret_cast<uint32_t*>(h->bitmap), uint64_t(h->t.alloc.words) * 2, 0, s);
 }
 
 void slab_free(vx_slab_table* h) {
   if (h->mem) hipFree(h->mem);
-  if (h->alloc) hipFree(h->alloc);
+  if (h->bitmap) hipFree(h->bitmap);
+}
+
+// Allocated ordinals of a bitmap (host popcount over a device copy).
+vx_status bitmap_used(const vx_slab_alloc_view& a, uint32_t* used) {
+  if (hipDeviceSynchronize() != hipSuccess) return VX_DEVICE_ERROR;
+  std::vector<unsigned long long> words(a.words);
+  if (a.words && hipMemcpy(words.data(), a.bitmap, size_t(a.words) * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return VX_DEVICE_ERROR;
+  uint32_t n = 0;
+  for (unsigned long long w : words) n += uint32_t(__builtin_popcountll(w));
+  *used = n;
+  return VX_OK;
 }
 
 uint32_t slab_grid(uint32_t n) {
@@ -536,9 +566,145 @@ extern "C" vx_status vx_slab_set_erase(vx_slab_set* h, const uint64_t* keys, uin
 
 extern "C" vx_status vx_slab_set_slabs_used(const vx_slab_set* h, uint32_t* used) {
   if (!h || !used) return VX_INVALID_VALUE;
-  if (hipDeviceSynchronize() != hipSuccess) return VX_DEVICE_ERROR;
-  if (hipMemcpy(used, h->alloc, 4, hipMemcpyDeviceToHost) != hipSuccess) return VX_DEVICE_ERROR;
-  *used = std::min(*used, h->t.capacity);
+  uint32_t overflow = 0;
+  const vx_status st = bitmap_used(h->t.alloc, &overflow);
+  if (st != VX_OK) return st;
+  *used = h->t.buckets + overflow;
+  return VX_OK;
+}
+
+// ---- C ABI: slab set of u32 keys (SlabHashSetU32) ----------------------------------
+extern "C" vx_status vx_slab_set_u32_create(uint32_t buckets, uint32_t overflow_slabs, vx_slab_set_u32** out) {
+  if (!out) return VX_INVALID_VALUE;
+  auto* h = new vx_slab_set_u32();
+  vx_status st = slab_create(h, buckets, overflow_slabs);
+  if (st == VX_OK) st = slab_clear(h, nullptr);
+  if (st == VX_OK) st = from_hip(hipDeviceSynchronize());
+  if (st != VX_OK) {
+    slab_free(h);
+    delete h;
+    return st;
+  }
+  *out = h;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_slab_set_u32_destroy(vx_slab_set_u32* h) {
+  if (!h) return VX_INVALID_VALUE;
+  slab_free(h);
+  delete h;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_slab_set_u32_clear(vx_slab_set_u32* h, void* stream) {
+  if (!h) return VX_INVALID_VALUE;
+  return slab_clear(h, static_cast<hipStream_t>(stream));
+}
+
+extern "C" vx_status vx_slab_set_u32_insert(vx_slab_set_u32* h, const uint32_t* keys, uint32_t n, uint32_t* status,
+                                            void* stream) {
+  if (!h || (!keys && n)) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_set_kernel<SetOp::kInsert, uint32_t><<<slab_grid(n), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      h->t, keys, n, nullptr, status);
+  return last_launch();
+}
+
+extern "C" vx_status vx_slab_set_u32_contains(const vx_slab_set_u32* h, const uint32_t* keys, uint32_t n,
+                                              uint8_t* found, uint32_t* status, void* stream) {
+  if (!h || (!keys && n) || (!found && !status && n)) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_set_kernel<SetOp::kContains, uint32_t><<<slab_grid(n), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      h->t, keys, n, found, status);
+  return last_launch();
+}
+
+extern "C" vx_status vx_slab_set_u32_erase(vx_slab_set_u32* h, const uint32_t* keys, uint32_t n, uint32_t* status,
+                                           void* stream) {
+  if (!h || (!keys && n)) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_set_kernel<SetOp::kErase, uint32_t><<<slab_grid(n), 256, 0, static_cast<hipStream_t>(stream)>>>(
+      h->t, keys, n, nullptr, status);
+  return last_launch();
+}
+
+extern "C" vx_status vx_slab_set_u32_slabs_used(const vx_slab_set_u32* h, uint32_t* used) {
+  if (!h || !used) return VX_INVALID_VALUE;
+  uint32_t overflow = 0;
+  const vx_status st = bitmap_used(h->t.alloc, &overflow);
+  if (st != VX_OK) return st;
+  *used = h->t.buckets + overflow;
+  return VX_OK;
+}
+
+// ---- C ABI: the slab allocator itself (SlabAlloc) ----------------------------------
+namespace {
+__global__ void slab_alloc_kernel(vx_slab_alloc_view a, uint32_t n, uint32_t* out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = vx_slab_alloc_dev::allocate(a, i);
+}
+__global__ void slab_dealloc_kernel(vx_slab_alloc_view a, const uint32_t* ordinals, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    vx_slab_alloc_dev::deallocate(a, ordinals[i]);
+}
+}  // namespace
+
+extern "C" vx_status vx_slab_alloc_create(uint32_t capacity, vx_slab_alloc** out) {
+  if (!out) return VX_INVALID_VALUE;
+  auto* h = new vx_slab_alloc();
+  h->v.capacity = capacity;
+  h->v.words = bitmap_words(capacity);
+  if (hipMalloc(&h->v.bitmap, std::max<size_t>(256, size_t(h->v.words) * 8)) != hipSuccess) {
+    delete h;
+    return VX_OUT_OF_CAPACITY;
+  }
+  vx_status st = fill_u32(reinterpret_cast<uint32_t*>(h->v.bitmap), uint64_t(h->v.words) * 2, 0, nullptr);
+  if (st == VX_OK) st = from_hip(hipDeviceSynchronize());
+  if (st != VX_OK) {
+    hipFree(h->v.bitmap);
+    delete h;
+    return st;
+  }
+  *out = h;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_slab_alloc_destroy(vx_slab_alloc* h) {
+  if (!h) return VX_INVALID_VALUE;
+  if (h->v.bitmap) hipFree(h->v.bitmap);
+  delete h;
+  return VX_OK;
+}
+
+extern "C" vx_status vx_slab_alloc_clear(vx_slab_alloc* h, void* stream) {
+  if (!h) return VX_INVALID_VALUE;
+  return fill_u32(reinterpret_cast<uint32_t*>(h->v.bitmap), uint64_t(h->v.words) * 2, 0,
+                  static_cast<hipStream_t>(stream));
+}
+
+extern "C" vx_status vx_slab_alloc_allocate(vx_slab_alloc* h, uint32_t n, uint32_t* d_ordinals, void* stream) {
+  if (!h || (n && !d_ordinals)) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_alloc_kernel<<<grid_for(n), 256, 0, static_cast<hipStream_t>(stream)>>>(h->v, n, d_ordinals);
+  return last_launch();
+}
+
+extern "C" vx_status vx_slab_alloc_deallocate(vx_slab_alloc* h, const uint32_t* d_ordinals, uint32_t n,
+                                              void* stream) {
+  if (!h || (n && !d_ordinals)) return VX_INVALID_VALUE;
+  if (!n) return VX_OK;
+  slab_dealloc_kernel<<<grid_for(n), 256, 0, static_cast<hipStream_t>(stream)>>>(h->v, d_ordinals, n);
+  return last_launch();
+}
+
+extern "C" vx_status vx_slab_alloc_used(const vx_slab_alloc* h, uint32_t* used) {
+  if (!h || !used) return VX_INVALID_VALUE;
+  return bitmap_used(h->v, used);
+}
+
+extern "C" vx_status vx_slab_alloc_view_get(const vx_slab_alloc* h, vx_slab_alloc_view* out) {
+  if (!h || !out) return VX_INVALID_VALUE;
+  *out = h->v;
   return VX_OK;
 }
 

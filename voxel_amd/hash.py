@@ -37,6 +37,20 @@ register({
     "vx_cuckoo_set_clear": (_i32, [_P, _P]),
     "vx_cuckoo_set_insert": (_i32, [_P, _P, _u32, _P, _P]),
     "vx_cuckoo_set_contains": (_i32, [_P, _P, _u32, _P, _P]),
+    "vx_slab_set_u32_create": (_i32, [_u32, _u32, C.POINTER(_P)]),
+    "vx_slab_set_u32_destroy": (_i32, [_P]),
+    "vx_slab_set_u32_clear": (_i32, [_P, _P]),
+    "vx_slab_set_u32_insert": (_i32, [_P, _P, _u32, _P, _P]),
+    "vx_slab_set_u32_contains": (_i32, [_P, _P, _u32, _P, _P, _P]),
+    "vx_slab_set_u32_erase": (_i32, [_P, _P, _u32, _P, _P]),
+    "vx_slab_set_u32_slabs_used": (_i32, [_P, C.POINTER(_u32)]),
+    "vx_slab_alloc_create": (_i32, [_u32, C.POINTER(_P)]),
+    "vx_slab_alloc_destroy": (_i32, [_P]),
+    "vx_slab_alloc_clear": (_i32, [_P, _P]),
+    "vx_slab_alloc_allocate": (_i32, [_P, _u32, _P, _P]),
+    "vx_slab_alloc_deallocate": (_i32, [_P, _P, _u32, _P]),
+    "vx_slab_alloc_used": (_i32, [_P, C.POINTER(_u32)]),
+    "vx_slab_alloc_view_get": (_i32, [_P, _P]),
     "vx_accel_set_create": (_i32, [_u32, _u32, _u32, C.POINTER(_P)]),
     "vx_accel_set_destroy": (_i32, [_P]),
     "vx_accel_set_clear": (_i32, [_P, _P]),
@@ -109,6 +123,67 @@ class SlabHashSetU64(_Owned):
     def slabs_used(self) -> int:
         v = C.c_uint32()
         check(lib().vx_slab_set_slabs_used(self.h, C.byref(v)), "vx_slab_set_slabs_used")
+        return v.value
+
+
+class SlabHashSetU32(_Owned):
+    """SlabHashSetU32 (slab_hash/set/host.cuh:15-107): u32 keys, 0xffffffff and
+    0xfffffffe reserved; overflow slabs from the slab allocator."""
+    _destroy = "vx_slab_set_u32_destroy"
+
+    def __init__(self, bucket_count: int, slab_capacity: int):
+        self.h = C.c_void_p()
+        check(lib().vx_slab_set_u32_create(bucket_count, slab_capacity, C.byref(self.h)), "vx_slab_set_u32_create")
+
+    def clear(self, stream=None):
+        check(lib().vx_slab_set_u32_clear(self.h, _stream(stream)), "vx_slab_set_u32_clear")
+
+    def insert_batch(self, keys: torch.Tensor, statuses: torch.Tensor | None = None, stream=None):
+        check(lib().vx_slab_set_u32_insert(self.h, _ptr(keys), keys.numel(), _ptr(statuses), _stream(stream)),
+              "vx_slab_set_u32_insert")
+
+    def contains_batch(self, keys: torch.Tensor, statuses: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        found = torch.empty(keys.numel(), dtype=torch.uint8, device=keys.device)
+        check(lib().vx_slab_set_u32_contains(self.h, _ptr(keys), keys.numel(), _ptr(found), _ptr(statuses),
+                                             _stream(stream)), "vx_slab_set_u32_contains")
+        return found
+
+    def erase_batch(self, keys: torch.Tensor, statuses: torch.Tensor | None = None, stream=None):
+        check(lib().vx_slab_set_u32_erase(self.h, _ptr(keys), keys.numel(), _ptr(statuses), _stream(stream)),
+              "vx_slab_set_u32_erase")
+
+    def slabs_used(self) -> int:
+        v = C.c_uint32()
+        check(lib().vx_slab_set_u32_slabs_used(self.h, C.byref(v)), "vx_slab_set_u32_slabs_used")
+        return v.value
+
+
+class SlabAlloc(_Owned):
+    """SlabAlloc (cuda/hash_table/slab_alloc.cuh:26-251): reusable slab
+    ordinals; allocate returns one per request (NULL_SLAB when none is
+    left), deallocate frees them."""
+    _destroy = "vx_slab_alloc_destroy"
+    NULL_SLAB = 0xFFFFFFFF
+
+    def __init__(self, capacity: int):
+        self.h = C.c_void_p()
+        check(lib().vx_slab_alloc_create(capacity, C.byref(self.h)), "vx_slab_alloc_create")
+
+    def clear(self, stream=None):
+        check(lib().vx_slab_alloc_clear(self.h, _stream(stream)), "vx_slab_alloc_clear")
+
+    def allocate(self, n: int, device="cuda", stream=None) -> torch.Tensor:
+        out = torch.empty(n, dtype=torch.int32, device=device)
+        check(lib().vx_slab_alloc_allocate(self.h, n, _ptr(out), _stream(stream)), "vx_slab_alloc_allocate")
+        return out
+
+    def deallocate(self, ordinals: torch.Tensor, stream=None):
+        check(lib().vx_slab_alloc_deallocate(self.h, _ptr(ordinals), ordinals.numel(), _stream(stream)),
+              "vx_slab_alloc_deallocate")
+
+    def used(self) -> int:
+        v = C.c_uint32()
+        check(lib().vx_slab_alloc_used(self.h, C.byref(v)), "vx_slab_alloc_used")
         return v.value
 
 
