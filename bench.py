@@ -101,6 +101,11 @@ def parse():
                     help="compile bench_vendor_sort.hip (rocPRIM radix sort on the sort rows' shapes) and exit")
     ap.add_argument("--no-vendor-sort", action="store_true",
                     help="skip the rocPRIM comparison (run with the primitives when it has been built)")
+    ap.add_argument("--primitives-only", action="store_true",
+                    help="run only the primitive legs (sorts of 2^28 u32 / 2^27 u64 pairs, 2^28 scan) and print them")
+    ap.add_argument("--counters", nargs="+", metavar=("OUT", "CSV"),
+                    help="summarise rocprofv3 --pmc counter CSVs of the sort partition kernels (averages per "
+                         "dispatch, per kernel instantiation) into OUT (JSON), then exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="finish with one 1 GiB read + 1 GiB write elementwise kernel (bitwise_not on int32), "
                          "the known-size reference for hardware byte counters")
@@ -875,8 +880,74 @@ def traffic_summary(fetch_csv, write_csv, out_path):
     print(json.dumps(out["kernels"], indent=1))
 
 
+COUNTER_PASSES = [
+    "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_INSTS_BRANCH SQ_INSTS_SMEM",
+    "SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_WAIT_INST_LDS SQ_ACTIVE_INST_VALU "
+    "SQ_ACTIVE_INST_LDS SQ_ACTIVE_INST_VMEM",
+    "SQ_LDS_BANK_CONFLICT SQ_LDS_ADDR_CONFLICT SQ_LEVEL_WAVES SQ_INST_LEVEL_VMEM SQ_INST_LEVEL_LDS "
+    "SQ_ACTIVE_INST_SCA SQ_INST_CYCLES_VMEM_RD SQ_INST_CYCLES_VMEM_WR",
+    "FETCH_SIZE",
+    "WRITE_SIZE",
+]
+
+
+COUNTER_KERNELS = ("partition_kernel", "histogram_kernel", "leaf_build_kernel")
+
+
+def counter_summary(out_path, csv_paths):
+    """Averages per dispatch of every counter of the sort and leaf-build
+    kernels (and the calibration copy, for the byte counters) in rocprofv3 --pmc CSVs,
+    per kernel instantiation, with derived ratios: instructions per wave,
+    waits and active cycles as fractions of wave cycles."""
+    import csv
+    acc = {}
+    for path in csv_paths:
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"]
+            hit = next((k for k in COUNTER_KERNELS if k in name), None)
+            if hit:
+                key = name[name.index(hit):].split("(")[0]
+            elif "bitwise_not" in name:
+                key = "calibration copy (bitwise_not, 1 GiB read + 1 GiB written)"
+            else:
+                continue
+            c = acc.setdefault(key, {}).setdefault(r["Counter_Name"], [])
+            c.append(float(r["Counter_Value"]))
+    out = {"commands": ["cd /tmp && export TMPDIR=/tmp && cd <repo> && timeout -s KILL 120 rocprofv3 --pmc "
+                        f"{p} --output-format csv -d gpurun_out/pmc_<tag> -o <tag> -- python3 bench.py <workload>"
+                        for p in COUNTER_PASSES],
+           "workloads": {"edit": "--warmup 1 --profile-steps 0 --steps 1 --no-cpu --no-primitives --no-batched "
+                                 "--no-serving --no-svo --no-config0 --no-voxel-edits --copy-calibration",
+                         "primitives": "--primitives-only --copy-calibration"},
+           "kernels": {}}
+    for key, counters in acc.items():
+        avg = {k: sum(v) / len(v) for k, v in counters.items()}
+        d = {"dispatches": max(len(v) for v in counters.values()), "avg": avg}
+        w = avg.get("SQ_WAVES")
+        if w:
+            for k in ("SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR",
+                      "SQ_INSTS_BRANCH"):
+                if k in avg:
+                    d.setdefault("per_wave", {})[k] = avg[k] / w
+        wc = avg.get("SQ_WAVE_CYCLES")
+        if wc:
+            for k in ("SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_WAIT_INST_LDS", "SQ_ACTIVE_INST_VALU",
+                      "SQ_ACTIVE_INST_LDS", "SQ_ACTIVE_INST_VMEM", "SQ_ACTIVE_INST_SCA"):
+                if k in avg:
+                    d.setdefault("fraction_of_wave_cycles", {})[k] = avg[k] / wc
+        if avg.get("SQ_BUSY_CYCLES") and avg.get("SQ_LEVEL_WAVES"):
+            d["mean_waves_in_flight"] = avg["SQ_LEVEL_WAVES"] / avg["SQ_BUSY_CYCLES"]
+        out["kernels"][key] = d
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps({k: {kk: v[kk] for kk in v if kk != "avg"} for k, v in out["kernels"].items()}, indent=1))
+
+
 def main():
     args = parse()
+    if args.counters:
+        counter_summary(args.counters[0], args.counters[1:])
+        return
     if args.traffic:
         traffic_summary(args.traffic[0], args.traffic[1], TRAFFIC_FILE)
         return
@@ -899,6 +970,16 @@ def main():
 
     import voxel_amd.primitives as P
     from voxel_amd import dag as V
+
+    if args.primitives_only:
+        prims = primitives(torch, P)
+        if args.copy_calibration:
+            src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+            dst = torch.empty_like(src)
+            torch.bitwise_not(src, out=dst)
+            torch.cuda.synchronize()
+        print(json.dumps(prims))
+        return
 
     world = 4 << args.inner_depth
     # Batch order: warmup, then the profiled pass, then the timed steps.

@@ -134,7 +134,7 @@ vx_status vx_sort_by_key(void* keys, uint32_t key_bytes, uint32_t n, uint32_t ke
 /* Sort tuning (the reference selects these by template arguments,
  * cuda/sort/radix.cuh:55-68). radix_bits in {4, 8} with keys_per_thread in
  * {8, 15, 23} and block in {256, 512, 1024}, or radix_bits 9 with
- * (keys_per_thread, block) in {(8, 512), (8, 1024), (15, 512)};
+ * (keys_per_thread, block) in {(8, 512), (8, 1024), (15, 512), (23, 512), (15, 1024)};
  * combinations whose tile does not fit LDS are VX_UNSUPPORTED. Takes effect
  * for later calls of this process. key_bytes selects the table row: 4 or 8
  * for (key, value) sorts, or'ed with VX_SORT_KEYS_ONLY for sorts without

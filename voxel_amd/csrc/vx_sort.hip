@@ -41,7 +41,13 @@ struct Cfg {
   static constexpr uint32_t R = 1u << RB;
   static constexpr uint32_t WAVES = BLOCK / kWave;
   static constexpr uint32_t TILE = BLOCK * KPT;
-  static constexpr size_t LDS = size_t(TILE) * (sizeof(K) + (VALS ? 4 : 0)) + size_t(WAVES + 3) * R * 4 + 64;
+  // The staged tile (keys, values) and the per-wave digit histograms share
+  // one LDS region: the histograms are dead once every key's position is in
+  // registers, before the first key is staged.
+  static constexpr size_t KEYS_BYTES = size_t(TILE) * (sizeof(K) + (VALS ? 4 : 0));
+  static constexpr size_t HIST_BYTES = size_t(WAVES) * R * 4;
+  static constexpr size_t SHARED = ((KEYS_BYTES > HIST_BYTES ? KEYS_BYTES : HIST_BYTES) + 15) / 16 * 16;
+  static constexpr size_t LDS = SHARED + size_t(3) * R * 4 + 64;
   static constexpr bool kFeasible = LDS <= kMaxLds && R <= BLOCK;
 };
 
@@ -50,57 +56,79 @@ __device__ __forceinline__ uint32_t count_of(const uint32_t* n_ptr, uint32_t n_a
 }
 
 // ---- pass 0: histograms of every digit position in one read ----------------
+// Adds the wave's digits of one pass (lanes at and above `valid_lanes` hold
+// no key) into the block's LDS histogram h.
+// COHERENT (keys that arrive in spatially coherent runs, like the edit's
+// brick records in brush order): equal digits then come in runs over
+// consecutive lanes, and same-address LDS atomics from one instruction
+// serialise, so the first lane of each run adds the run's length; a wave
+// with many runs (the low digits) adds one per lane instead. Random keys
+// always add one per lane.
+template <bool COHERENT>
+__device__ __forceinline__ void count_digits(uint32_t* h, uint32_t d, uint32_t lane, uint32_t valid_lanes) {
+  const bool valid = lane < valid_lanes;
+  if constexpr (COHERENT) {
+    const uint32_t key = valid ? d : 0xffffffffu;  // a run of its own, never added
+    // The previous lane's digit (lane 0 sees one that differs from its own).
+    const uint32_t prev = uint32_t(__builtin_amdgcn_update_dpp(int(~key), int(key), 0x138 /* wave_shr:1 */, 0xf, 0xf,
+                                                               false));
+    const uint64_t heads = ballot(key != prev);
+    if (__popcll(heads) <= 16) {
+      if (valid && key != prev) {
+        const uint64_t above = (heads >> lane) >> 1;  // heads after this lane
+        const uint32_t len = above ? uint32_t(__builtin_ctzll(above)) + 1u : kWave - lane;
+        atomicAdd(&h[d], len);
+      }
+      return;
+    }
+  }
+  if (valid) atomicAdd(&h[d], 1u);
+}
+
+// Each wave reads runs of kHistKpt * 64 consecutive keys, all loads of a run
+// issued before any is counted; two 1024-thread blocks per CU (eight waves
+// per SIMD) keep the final flush of the per-block histograms into the global
+// ones to a few hundred atomics per counter.
+constexpr uint32_t kHistKpt = 8;
+constexpr uint32_t kHistBlock = 1024;
+
 template <class K, uint32_t RB, bool COHERENT>
-__global__ __launch_bounds__(256) void histogram_kernel(const K* __restrict__ keys, const uint32_t* n_ptr,
-                                                        uint32_t n_arg, uint32_t begin_bit, uint32_t key_bits,
-                                                        uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kHistBlock) void histogram_kernel(const K* __restrict__ keys, const uint32_t* n_ptr,
+                                                               uint32_t n_arg, uint32_t begin_bit, uint32_t key_bits,
+                                                               uint32_t* __restrict__ hist) {
   constexpr uint32_t R = 1u << RB;
   constexpr uint32_t MAXP = (sizeof(K) * 8 + RB - 1) / RB;
+  constexpr uint32_t CHUNK = kHistBlock * kHistKpt;
   __shared__ uint32_t s[MAXP * R];
   const uint32_t n = count_of(n_ptr, n_arg);
   const uint32_t passes = (key_bits + RB - 1) / RB;
-  for (uint32_t i = threadIdx.x; i < passes * R; i += blockDim.x) s[i] = 0;
+  const uint32_t lane = lane_id();
+  for (uint32_t i = threadIdx.x; i < passes * R; i += kHistBlock) s[i] = 0;
   __syncthreads();
-  // COHERENT (keys that arrive in spatially coherent runs, like the edit's
-  // brick records in brush order): lanes whose digit equals the first active
-  // lane's add it with one atomic (the high digits are then often the same
-  // across a wave, and 64 same-address LDS atomics would serialise); the
-  // others add their own. Random keys skip the extra ballots.
-  const uint32_t stride = gridDim.x * blockDim.x;
-  if constexpr (!COHERENT) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-      const K key = keys[i];
+  for (uint64_t c0 = uint64_t(blockIdx.x) * CHUNK; c0 < n; c0 += uint64_t(gridDim.x) * CHUNK) {
+    const uint32_t first = wave_id() * (kWave * kHistKpt);  // the wave's first key in the chunk
+    const K* run = keys + c0 + first;
+    const uint32_t left = uint32_t(min<uint64_t>(CHUNK, n - c0));
+    const uint32_t wave_keys = left > first ? left - first : 0u;
+    K key[kHistKpt];
+#pragma unroll
+    for (uint32_t j = 0; j < kHistKpt; ++j) key[j] = j * kWave + lane < wave_keys ? run[j * kWave + lane] : K(0);
+#pragma unroll
+    for (uint32_t j = 0; j < kHistKpt; ++j) {
+      const uint32_t valid_lanes = wave_keys > j * kWave ? min(wave_keys - j * kWave, kWave) : 0u;
+      if (valid_lanes == 0) break;  // uniform across the wave
 #pragma unroll
       for (uint32_t p = 0; p < MAXP; ++p) {
         if (p < passes) {
           const uint32_t bits = min(RB, key_bits - p * RB);
-          atomicAdd(&s[p * R + digit_of(key, begin_bit + p * RB, (1u << bits) - 1)], 1u);
-        }
-      }
-    }
-  }
-  for (uint32_t i0 = blockIdx.x * blockDim.x; COHERENT && i0 < n; i0 += stride) {
-    const uint32_t i = i0 + threadIdx.x;
-    const bool valid = i < n;
-    const K key = valid ? keys[i] : K(0);
-#pragma unroll
-    for (uint32_t p = 0; p < MAXP; ++p) {
-      if (p < passes) {
-        const uint32_t bits = min(RB, key_bits - p * RB);
-        const uint32_t d = digit_of(key, begin_bit + p * RB, (1u << bits) - 1);
-        const uint64_t act = ballot(valid);
-        const uint32_t lead = __shfl(d, act ? int(__builtin_ctzll(act)) : 0, kWave);
-        const uint64_t same = ballot(valid && d == lead);
-        if (valid && d == lead) {
-          if (lanes_below(same) == 0) atomicAdd(&s[p * R + d], uint32_t(__popcll(same)));
-        } else if (valid) {
-          atomicAdd(&s[p * R + d], 1u);
+          count_digits<COHERENT>(s + p * R, digit_of(key[j], begin_bit + p * RB, (1u << bits) - 1), lane,
+                                 valid_lanes);
         }
       }
     }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < passes * R; i += blockDim.x)
+  for (uint32_t i = threadIdx.x; i < passes * R; i += kHistBlock)
     if (s[i]) atomicAdd(&hist[i], s[i]);
 }
 
@@ -121,7 +149,56 @@ __global__ __launch_bounds__((1u << RB) > 256 ? (1u << RB) : 256) void hist_scan
   if (t < R) h[t] = before + inc - v;
 }
 
+// Lanes of the wave whose digit equals this lane's, among the lanes of
+// `init`: one ballot per digit bit, each narrowing the match with one
+// three-input bit operation per 32-lane half (m &= ~(ballot ^ my bit)).
+template <uint32_t RB>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t init) {
+  uint32_t lo = uint32_t(init), hi = uint32_t(init >> 32);
+#pragma unroll
+  for (uint32_t b = 0; b < RB; ++b) {
+    const uint32_t bm = uint32_t(__builtin_amdgcn_sbfe(int(d), b, 1));  // all ones iff bit b is set
+    const uint64_t bal = ballot(bm != 0u);
+    lo &= ~(uint32_t(bal) ^ bm);
+    hi &= ~(uint32_t(bal >> 32) ^ bm);
+  }
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// Loads of a wave's KPT keys (lane-contiguous rounds) and their ranks among
+// the wave's earlier keys of the same digit, kept in the wave's LDS
+// histogram `wh`. FULL: the whole tile lies below n (no per-key guards).
+template <bool FULL, class K, bool VALS, uint32_t RB, uint32_t KPT>
+__device__ __forceinline__ void load_and_rank(const K* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t base,
+                                              uint32_t n, uint32_t lane, uint32_t shift, uint32_t dmask, uint32_t* wh,
+                                              K (&key)[KPT], uint32_t (&val)[KPT], uint32_t (&dig)[KPT],
+                                              uint32_t (&rank)[KPT]) {
+#pragma unroll
+  for (uint32_t j = 0; j < KPT; ++j) {
+    const uint64_t i = base + j * kWave + lane;
+    const bool valid = FULL || i < n;
+    key[j] = valid ? kin[i] : K(0);
+    if constexpr (VALS) val[j] = valid ? vin[i] : 0u;
+    dig[j] = digit_of(key[j], shift, dmask);
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < KPT; ++j) {
+    const bool valid = FULL || base + j * kWave + lane < n;
+    const uint64_t peers = match_digit<RB>(dig[j], FULL ? ~0ull : ballot(valid));
+    const uint32_t below = lanes_below(peers);
+    uint32_t cnt = 0;
+    if (valid) cnt = wh[dig[j]];
+    rank[j] = cnt + below;
+    if (valid && below == 0) wh[dig[j]] = cnt + uint32_t(__popcll(peers));
+  }
+}
+
 // ---- partition pass -------------------------------------------------------
+// One tile per claim: load, rank (wave64 multisplit), per-digit wave scan and
+// publication of the tile aggregate, then the first look-back window is
+// issued before the digit scan and the LDS staging, so its round trip to the
+// predecessors' records overlaps them; the walk is finished afterwards and
+// the staged keys are scattered to their digits' output runs.
 template <class K, bool VALS, uint32_t RB, uint32_t BLOCK, uint32_t KPT>
 __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ kin, K* __restrict__ kout,
                                                           const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
@@ -130,11 +207,12 @@ __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ 
                                                           uint32_t* tile_counter, uint32_t* rec, uint32_t* rec_next) {
   using C = Cfg<K, VALS, RB, BLOCK, KPT>;
   constexpr uint32_t R = C::R, WAVES = C::WAVES, TILE = C::TILE;
+  constexpr int kLookWindow = 8;
   extern __shared__ __align__(16) unsigned char smem[];
   K* s_keys = reinterpret_cast<K*>(smem);
   uint32_t* s_vals = reinterpret_cast<uint32_t*>(smem + size_t(TILE) * sizeof(K));
-  uint32_t* s_hist = reinterpret_cast<uint32_t*>(smem + size_t(TILE) * (sizeof(K) + (VALS ? 4 : 0)));  // [WAVES][R]
-  uint32_t* s_cnt = s_hist + WAVES * R;      // tile count per digit
+  uint32_t* s_hist = reinterpret_cast<uint32_t*>(smem);  // [WAVES][R], aliases the staged tile
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(smem + C::SHARED);  // tile count per digit
   uint32_t* s_start = s_cnt + R;             // tile-local start per digit
   uint32_t* s_global = s_start + R;          // global base - tile start per digit
   uint32_t* s_misc = s_global + R;
@@ -155,47 +233,34 @@ __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ 
 
     K key[KPT];
     uint32_t val[KPT], dig[KPT], rank[KPT];
-#pragma unroll
-    for (uint32_t j = 0; j < KPT; ++j) {
-      const uint64_t i = base + j * kWave + lane;
-      const bool valid = i < n;
-      key[j] = valid ? kin[i] : K(0);
-      if constexpr (VALS) val[j] = valid ? vin[i] : 0u;
-      dig[j] = digit_of(key[j], shift, dmask);
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < KPT; ++j) {
-      const uint64_t i = base + j * kWave + lane;
-      const bool valid = i < n;
-      uint64_t peers = ballot(valid);
-#pragma unroll
-      for (uint32_t b = 0; b < RB; ++b) {
-        const bool bit = (dig[j] >> b) & 1u;
-        const uint64_t bal = ballot(bit);
-        peers &= bit ? bal : ~bal;
-      }
-      const uint32_t below = lanes_below(peers);
-      uint32_t cnt = 0;
-      if (valid) cnt = wh[dig[j]];
-      rank[j] = cnt + below;
-      if (valid && below == 0) wh[dig[j]] = cnt + uint32_t(__popcll(peers));
-    }
+    const bool full = tile_base + TILE <= n;
+    if (full)
+      load_and_rank<true, K, VALS, RB, KPT>(kin, vin, base, n, lane, shift, dmask, wh, key, val, dig, rank);
+    else
+      load_and_rank<false, K, VALS, RB, KPT>(kin, vin, base, n, lane, shift, dmask, wh, key, val, dig, rank);
     __syncthreads();
-    // Per digit: exclusive scan over waves (wave order = input order) and
-    // publication of the tile aggregate.
-    for (uint32_t d = t; d < R; d += BLOCK) {
+    // Per digit (one per thread, R <= BLOCK): exclusive scan over waves (wave
+    // order = input order), publication of the tile aggregate, and the first
+    // look-back window over the predecessors' records.
+    uint32_t look[kLookWindow];
+    if (t < R) {
       uint32_t sum = 0;
 #pragma unroll
       for (uint32_t w = 0; w < WAVES; ++w) {
-        const uint32_t c = s_hist[w * R + d];
-        s_hist[w * R + d] = sum;
+        const uint32_t c = s_hist[w * R + t];
+        s_hist[w * R + t] = sum;
         sum += c;
       }
-      s_cnt[d] = sum;
-      st_agent(&rec[size_t(tile) * R + d], (tile == 0 ? kStatusPrefix : kStatusAgg) | sum);
+      s_cnt[t] = sum;
+      st_agent(&rec[size_t(tile) * R + t], (tile == 0 ? kStatusPrefix : kStatusAgg) | sum);
+#pragma unroll
+      for (int q = 0; q < kLookWindow; ++q) {
+        const int64_t jq = int64_t(tile) - 1 - q;
+        look[q] = jq >= 0 ? ld_agent(&rec[size_t(jq) * R + t]) : kStatusPrefix;
+      }
     }
     __syncthreads();
-    // Exclusive scan over digits of the tile counts (wave 0, R <= 256).
+    // Exclusive scan over digits of the tile counts (wave 0, R <= 512).
     if (wave == 0) {
       constexpr uint32_t PER = (R + kWave - 1) / kWave;
       uint32_t loc[PER];
@@ -216,30 +281,32 @@ __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ 
       }
     }
     __syncthreads();
+    // Tile positions into registers, then (the histograms dead) the staging.
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; ++j) rank[j] += s_start[dig[j]] + wh[dig[j]];
+    __syncthreads();
 #pragma unroll
     for (uint32_t j = 0; j < KPT; ++j) {
-      const uint64_t i = base + j * kWave + lane;
-      if (i < n) {
-        const uint32_t p = s_start[dig[j]] + wh[dig[j]] + rank[j];
-        s_keys[p] = key[j];
-        if constexpr (VALS) s_vals[p] = val[j];
+      if (full || base + j * kWave + lane < n) {
+        s_keys[rank[j]] = key[j];
+        if constexpr (VALS) s_vals[rank[j]] = val[j];
       }
     }
-    // Per-digit look-back over predecessor tiles, kLookWindow records per
-    // round trip: the loads of one window are independent, so a walk across
-    // tiles that have only published aggregates costs one memory latency per
-    // window instead of one per tile.
-    for (uint32_t d = t; d < R; d += BLOCK) {
+    // Finish the per-digit look-back, kLookWindow records per round trip: the
+    // loads of one window are independent, so a walk across tiles that have
+    // only published aggregates costs one memory latency per window instead
+    // of one per tile; an unpublished record makes the walk re-read from it.
+    if (t < R) {
       uint32_t excl = 0;
       if (tile > 0) {
-        constexpr int kLookWindow = 8;
         int64_t j = int64_t(tile) - 1;
-        bool done = false;
+        bool done = false, first = true;
         while (!done) {
           uint32_t r[kLookWindow];
 #pragma unroll
           for (int q = 0; q < kLookWindow; ++q)
-            r[q] = j - q >= 0 ? ld_agent(&rec[size_t(j - q) * R + d]) : kStatusPrefix;
+            r[q] = first ? look[q] : (j - q >= 0 ? ld_agent(&rec[size_t(j - q) * R + t]) : kStatusPrefix);
+          first = false;
           int consumed = 0;
 #pragma unroll
           for (int q = 0; q < kLookWindow; ++q) {
@@ -251,10 +318,10 @@ __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ 
           }
           j -= consumed;
         }
-        st_agent(&rec[size_t(tile) * R + d], kStatusPrefix | (excl + s_cnt[d]));
+        st_agent(&rec[size_t(tile) * R + t], kStatusPrefix | (excl + s_cnt[t]));
       }
-      s_global[d] = pass_offsets[d] + excl - s_start[d];
-      rec_next[size_t(tile) * R + d] = 0u;
+      s_global[t] = pass_offsets[t] + excl - s_start[t];
+      rec_next[size_t(tile) * R + t] = 0u;
     }
     __syncthreads();
     const uint32_t tile_n = uint32_t(min<uint64_t>(TILE, n - tile_base));
@@ -351,6 +418,8 @@ bool pick9(Launcher<K, VALS>& L, uint32_t block, uint32_t kpt) {
   if (block == 512 && kpt == 8) return pick3<K, VALS, 9, 512, 8>(L);
   if (block == 1024 && kpt == 8) return pick3<K, VALS, 9, 1024, 8>(L);
   if (block == 512 && kpt == 15) return pick3<K, VALS, 9, 512, 15>(L);
+  if (block == 512 && kpt == 23) return pick3<K, VALS, 9, 512, 23>(L);
+  if (block == 1024 && kpt == 15) return pick3<K, VALS, 9, 1024, 15>(L);
   return false;
 }
 
@@ -405,15 +474,16 @@ vx_status drive(K* keys, uint32_t* vals, K* alt_keys, uint32_t* alt_vals, const 
   const size_t clear_words = (reinterpret_cast<char*>(lay.rec0 + size_t(lay.tiles) * R) - reinterpret_cast<char*>(lay.hist)) / 4;
   vx_status st = fill_u32(lay.hist, clear_words, 0, s);
   if (st != VX_OK) return st;
-  const uint32_t hblocks = std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(n, 256 * 16), 2048));
+  const uint32_t hblocks =
+      std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(n, kHistBlock * kHistKpt), 2 * limits().cus));
   {
     TimedLaunch tl(VX_K_SORT_HISTOGRAM, s);
     if (tn.rb == 8)
-      histogram_kernel<K, 8, COHERENT><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
+      histogram_kernel<K, 8, COHERENT><<<hblocks, kHistBlock, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
     else if (tn.rb == 9)
-      histogram_kernel<K, 9, COHERENT><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
+      histogram_kernel<K, 9, COHERENT><<<hblocks, kHistBlock, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
     else
-      histogram_kernel<K, 4, COHERENT><<<hblocks, 256, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
+      histogram_kernel<K, 4, COHERENT><<<hblocks, kHistBlock, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
   }
   if ((st = last_launch()) != VX_OK) return st;
   {
@@ -425,7 +495,13 @@ vx_status drive(K* keys, uint32_t* vals, K* alt_keys, uint32_t* alt_vals, const 
   if ((st = last_launch()) != VX_OK) return st;
 
   hipFuncSetAttribute(reinterpret_cast<const void*>(L.kernel), hipFuncAttributeMaxDynamicSharedMemorySize, int(L.lds));
-  const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>(uint32_t(160 * 1024 / L.lds), 2048 / L.block));
+  // Persistent blocks: as many as can be resident at once (LDS, registers
+  // and waves per CU all count).
+  int resident = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, reinterpret_cast<const void*>(L.kernel), int(L.block),
+                                                   L.lds) != hipSuccess || resident < 1)
+    resident = 1;
+  const uint32_t per_cu = uint32_t(resident);
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(lay.tiles, limits().cus * per_cu));
 
   K* src = keys;
