@@ -33,9 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # Kernel kinds of the edit pipeline -> the device symbol rocprofv3 reports.
 KERNEL_SYMBOL = {
     "records": "fused_dev_kernel<PairSumOp, RowLoad, RowStore> + brush_records_kernel",
-    "sort_partition": "partition_kernel<unsigned long, true, ...>",
+    "sort_partition": "partition_kernel<unsigned long, false, ...> (packed 8-byte records, keys only)",
     "leaf_build": "leaf_build_kernel<16u>",
-    "leaf_probe": "probe_kernel<16u>",
+    "leaf_probe": "probe_slices_kernel<16u>",
     "leaf_commit": "fused_dev_kernel<SegSumOp, CommitLoad, CommitStore>",
     "node_build": "node_build_kernel",
 }
@@ -707,13 +707,19 @@ def leg_serving(V, torch, n_requests=2000, seed=20, load=0.7):
 def kernel_bytes(prof, leaf_words):
     """Algorithmic HBM bytes of every launch of each kernel kind, summed over
     the measured edits, from the edit's sizes: R brick records, L distinct
-    leaves, N distinct inner nodes, P radix passes, lw leaf payload bytes.
+    leaves, N distinct inner nodes (E1 of them at height 1, the leaves'
+    parents), P radix passes, Q pending leaves, lw leaf payload bytes.
     Lower bounds: work that only some entries do (reading a non-sentinel old
-    leaf, probing a non-sentinel item, copying a new item) is counted only
-    where every entry does it; random 4..8-byte reads count their own size
-    (the line-granular cost is what the hardware byte counters report)."""
+    leaf, probing a non-sentinel item) is counted only where every entry
+    does it; data shared by many entries (a parent's old node, the old and
+    the interned leaf items, most of which are the same few uniform leaves)
+    counts once per distinct owner or not at all; random 4..8-byte reads
+    count their own size (the line-granular cost is what the hardware byte
+    counters report, and a kernel whose counter bytes sit far above this
+    figure is re-reading)."""
     R, L, N, P = prof.records, prof.leaves, prof.nodes, prof.sort_passes
     Q = prof.pending_leaves  # leaves the build's lookup did not find: the probe, commit and place run over these
+    E1 = int(prof.height_entries[1])  # inner entries at height 1 (the leaves' parents)
     lw = 4 * leaf_words
     return {
         "records": (prof.sort_record_bytes + 12) * R,   # sort record + mask 8 + material 4 written (the touched
@@ -722,16 +728,20 @@ def kernel_bytes(prof, leaf_words):
         "sort_partition": 2 * prof.sort_record_bytes * R * P,  # per pass: each record read and written
         "runs": 8 * R + 16 * L,                         # sorted keys read; leaf keys, run starts, parent links
         "up": 24 * N,                                   # heights >= 1: keys read + parent links; parent keys, starts
-        "topdown": 48 * N,                              # parent index, parent ref, parent item, own ref
-        "leaf_build": L * (48 + lw) + (prof.sort_record_bytes + 12) * R,  # parent index, parent ref, parent
-                                                        # item word, range, slot, result, and the item (written
-                                                        # when new, else the interned one compared); per record:
-                                                        # its sorted entry, mask and material
+        "topdown": 12 * (N - E1) + 12 * E1,             # heights >= 2: key read, old ref written; height 1:
+                                                        # parent link read, old ref written, and the parent's
+                                                        # old node word (once per parent)
+        "leaf_build": 16 * L + (prof.sort_record_bytes + 12) * R + (lw + 12) * Q + 12 * E1,
+                                                        # per leaf: record range, parent link, result; per
+                                                        # record: sorted entry, mask, material; per pending
+                                                        # leaf: item and hash written, list entry; per parent:
+                                                        # its old ref and node word
         "leaf_probe": (40 + lw) * Q + L // 16,          # pending entry, list entry, item, hash, result read and
                                                         # written; slice counts and bases
         "leaf_commit": 28 * Q,                          # list entry, result, hash read; slot value written
-        "leaf_place": 20 * Q,                           # list entry, result, slot value (copies of new items:
-                                                        # not counted)
+        "leaf_place": (20 + 2 * lw) * Q,                # list entry, result, slot value; the new item read from
+                                                        # the batch and written to the pool (Q also holds the
+                                                        # batch's duplicates of a new item, which are not copied)
         "node_build": 64 * N + 20 * (L + N),            # old ref, starts, item, hash, result; child entries
         "node_probe": 16 * N,
         "node_commit": 16 * N,

@@ -139,6 +139,20 @@ __device__ __forceinline__ T ld_stream1(const T* p) { return __builtin_nontempor
 template <class T>
 __device__ __forceinline__ void st_stream1(T* p, T v) { __builtin_nontemporal_store(v, p); }
 
+// Plain load from device (global) memory through a pointer whose address
+// space the compiler cannot see (one read out of a descriptor): a generic
+// pointer becomes a flat access, which also occupies the LDS wait counter,
+// so every wait for it drains both counters.
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+__device__ __forceinline__ uint4 gld4(const uint32_t* p) {  // 16-byte aligned
+  typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+  const v4u32 v = *(const __attribute__((address_space(1))) v4u32*)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Device-coherent (agent scope) relaxed atomics: every XCD sees the value,
 // which plain loads through a per-XCD L2 would not guarantee.
 template <class T>

@@ -116,7 +116,7 @@ __device__ __forceinline__ void load_item(const uint32_t* p, uint32_t n, uint32_
     if (n == kMax && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
 #pragma unroll
       for (uint32_t k = 0; k < kMax; k += 4) {
-        const uint4 v = *reinterpret_cast<const uint4*>(p + k);
+        const uint4 v = gld4(p + k);
         w[k] = v.x;
         w[k + 1] = v.y;
         w[k + 2] = v.z;
@@ -126,7 +126,7 @@ __device__ __forceinline__ void load_item(const uint32_t* p, uint32_t n, uint32_
     }
   }
 #pragma unroll
-  for (uint32_t k = 0; k < kMax; ++k) w[k] = k < n ? p[k] : 0u;
+  for (uint32_t k = 0; k < kMax; ++k) w[k] = k < n ? gld(p + k) : 0u;
 }
 
 template <uint32_t kMax>
@@ -229,7 +229,7 @@ __device__ __forceinline__ uint32_t find_interned(const Table& t, const uint32_t
   const uint32_t fp = uint32_t(h >> 32);
   uint64_t idx = h & t.mask;
   for (uint32_t p = 0; p < kFindProbes; ++p) {
-    const uint64_t s = t.slots[idx];
+    const uint64_t s = gld(t.slots + idx);
     if (s == 0) return 0;
     if (uint32_t(s >> 32) == fp) {
       const uint32_t v = uint32_t(s);
@@ -613,7 +613,12 @@ struct SortedRecs {
   uint32_t index_bits;
   __device__ __forceinline__ uint64_t key(uint32_t i) const { return index_bits ? keys[i] >> index_bits : keys[i]; }
   __device__ __forceinline__ uint32_t index(uint32_t i) const {
-    return index_bits ? uint32_t(keys[i] & ((uint64_t(1) << index_bits) - 1)) : vals[i];
+    // One load in either layout, no branch (a branch would make the loads
+    // issued before it wait at the join): the packed key's low word holds
+    // the index (index_bits <= 32), else the value array does.
+    const uint32_t* src = index_bits ? reinterpret_cast<const uint32_t*>(keys) + 2 * size_t(i) : vals + i;
+    const uint32_t m = index_bits ? uint32_t((uint64_t(1) << index_bits) - 1) : 0xffffffffu;
+    return gld(src) & m;
   }
 };
 
@@ -1013,6 +1018,120 @@ struct SliceStore {
   }
 };
 
+// child_of with the node's header and its eight possible child words read
+// together (one memory round trip instead of two; the reads past the node's
+// end are clamped to the pool and unused). The child is picked with masks,
+// not a select, so the compiler cannot fold the nine loads back into a
+// header read followed by one dependent child read.
+__device__ __forceinline__ uint32_t child_of_wide(const uint32_t* pool, uint64_t cap, uint32_t ref, uint32_t c) {
+  const uint64_t base = ref > kFull ? ref : 0u;
+  uint32_t v[9];
+#pragma unroll
+  for (uint32_t k = 0; k < 9; ++k) v[k] = gld(pool + min<uint64_t>(base + k, cap - 1));
+  const uint32_t cm = v[0] & 0xffu, fm = (v[0] >> 8) & 0xffu, bit = 1u << c;
+  const uint32_t at = 1u + __popc(cm & (bit - 1));
+  uint32_t r = 0;
+#pragma unroll
+  for (uint32_t k = 1; k < 9; ++k) r |= v[k] & (0u - uint32_t(at == k));
+  if (ref <= kFull) return ref;
+  if (cm & bit) return r;
+  return (fm & bit) ? kFull : kEmpty;
+}
+
+// One leaf entry j of the build: its old leaf (the parent's old node, one
+// child), the entry's records applied in order, then the lookup among the
+// interned leaves. The reads run in three rounds of independent loads
+// (indices are clamped instead of branched on, so no round waits for a
+// conditional region): the record range, parent link and scene root; the
+// first record's index and the parent's old ref; the first record's mask and
+// material with the parent's old node. Most leaves have one record, so the
+// records cost no round trip of their own. d is the scene's descriptor by
+// value, so its fields are plain loads (kernel arguments in the single-scene
+// case). Returns whether the entry is left PENDING (its item and hash
+// written).
+template <uint32_t LW>
+__device__ __forceinline__ bool build_leaf(const Dev d, uint32_t j, const uint32_t* __restrict__ parent,
+                                           const uint32_t* __restrict__ old_up, const uint32_t* __restrict__ first,
+                                           const SortedRecs& recs, const uint64_t* __restrict__ rmask,
+                                           const uint32_t* __restrict__ rmat, uint32_t* __restrict__ items,
+                                           uint64_t* __restrict__ hashes, uint64_t* __restrict__ res) {
+  const bool top = d.depth == 0;  // the leaf is the scene's root
+  const uint32_t r0 = first[j], r1 = first[j + 1];
+  const uint32_t link = parent[j];
+  const uint32_t root = gld(d.ctl + kCtlRoot);
+  // n > 0 implies at least one record: record 0 stands in, unused, for an
+  // entry without records; entry 0 of the level above stands in for the
+  // root's missing parent.
+  const uint32_t up = old_up[top ? 0u : link_parent(link)];
+  const uint32_t idx0 = recs.index(r0 < r1 ? r0 : 0u);
+  const uint64_t m0 = rmask[idx0];
+  const uint32_t mat0 = rmat[idx0];
+  const uint32_t child = child_of_wide(d.node_pool, d.node_cap, up, link_slot(link));
+  const uint32_t ref = top ? root : child;
+  uint32_t w[LW];
+  if (ref == kEmpty) {
+#pragma unroll
+    for (uint32_t k = 0; k < LW; ++k) w[k] = 0;
+  } else if (ref == kFull) {
+#pragma unroll
+    for (uint32_t k = 0; k < LW; ++k) w[k] = 0xffffffffu;
+  } else {
+    load_item<LW>(d.leaf_pool + ref, LW, w);
+  }
+  // Records in order: the first from the registers loaded above, the rest
+  // (rare) read here.
+  auto apply = [&w](uint64_t m, uint32_t mat) {
+    if constexpr (LW == 2) {
+      const uint32_t lo = uint32_t(m), hi = uint32_t(m >> 32);
+      if (mat) {
+        w[0] |= lo;
+        w[1] |= hi;
+      } else {
+        w[0] &= ~lo;
+        w[1] &= ~hi;
+      }
+    } else {
+      const uint32_t fill = mat * 0x01010101u;
+#pragma unroll
+      for (uint32_t k = 0; k < LW; ++k) {
+        const uint32_t bm = nibble_bytes(uint32_t(m >> (4 * k)) & 15u);
+        w[k] = (w[k] & ~bm) | (fill & bm);
+      }
+    }
+  };
+  if (r0 < r1) apply(m0, mat0);
+  for (uint32_t r = r0 + 1; r < r1; ++r) {
+    const uint32_t idx = recs.index(r);
+    apply(rmask[idx], rmat[idx]);
+  }
+  uint32_t any = 0, all = 0xffffffffu;
+#pragma unroll
+  for (uint32_t k = 0; k < LW; ++k) {
+    any |= w[k];
+    all &= w[k];
+  }
+  if (!any) {
+    res[j] = res_make(kResFinal, kEmpty);
+    return false;
+  }
+  if (LW == 2 && all == 0xffffffffu) {
+    res[j] = res_make(kResFinal, kFull);
+    return false;
+  }
+  const uint64_t h = hash_words(w, LW);
+  const uint32_t v = find_interned<LW>(table_of(d, true), w, LW, h);
+  if (v) {
+    res[j] = res_make(kResFinal, v);
+    return false;
+  }
+  uint32_t* it = items + size_t(j) * LW;
+#pragma unroll
+  for (uint32_t k = 0; k < LW; ++k) it[k] = w[k];
+  hashes[j] = h;
+  res[j] = res_make(kResPending, 0);
+  return true;
+}
+
 template <uint32_t LW>
 __global__ __launch_bounds__(kLeafBlock, 8) void leaf_build_kernel(
     Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ parent, const uint32_t* __restrict__ old_up,
@@ -1027,67 +1146,11 @@ __global__ __launch_bounds__(kLeafBlock, 8) void leaf_build_kernel(
     const uint32_t j = sl * kWave + lane_id();
     bool pending = false;
     if (j < n) {
-      const Dev& d = sc.of(j);
-      const uint32_t ref =
-          d.depth == 0 ? d.ctl[kCtlRoot]
-                       : child_of(d.node_pool, old_up[link_parent(parent[j])], link_slot(parent[j]));
-      uint32_t w[LW];
-      if (ref == kEmpty) {
-#pragma unroll
-        for (uint32_t k = 0; k < LW; ++k) w[k] = 0;
-      } else if (ref == kFull) {
-#pragma unroll
-        for (uint32_t k = 0; k < LW; ++k) w[k] = 0xffffffffu;
-      } else {
-        load_item<LW>(sc.of(j).leaf_pool + ref, LW, w);
-      }
-      const uint32_t r1 = first[j + 1];
-      for (uint32_t r = first[j]; r < r1; ++r) {
-        const uint32_t idx = recs.index(r);
-        const uint64_t m = rmask[idx];
-        const uint32_t mat = rmat[idx];
-        if constexpr (LW == 2) {
-          const uint32_t lo = uint32_t(m), hi = uint32_t(m >> 32);
-          if (mat) {
-            w[0] |= lo;
-            w[1] |= hi;
-          } else {
-            w[0] &= ~lo;
-            w[1] &= ~hi;
-          }
-        } else {
-          const uint32_t fill = mat * 0x01010101u;
-#pragma unroll
-          for (uint32_t k = 0; k < LW; ++k) {
-            const uint32_t bm = nibble_bytes(uint32_t(m >> (4 * k)) & 15u);
-            w[k] = (w[k] & ~bm) | (fill & bm);
-          }
-        }
-      }
-      uint32_t any = 0, all = 0xffffffffu;
-#pragma unroll
-      for (uint32_t k = 0; k < LW; ++k) {
-        any |= w[k];
-        all &= w[k];
-      }
-      if (!any) {
-        res[j] = res_make(kResFinal, kEmpty);
-      } else if (LW == 2 && all == 0xffffffffu) {
-        res[j] = res_make(kResFinal, kFull);
-      } else {
-        const uint64_t h = hash_words(w, LW);
-        const uint32_t v = find_interned<LW>(table_of(d, true), w, LW, h);
-        if (v) {
-          res[j] = res_make(kResFinal, v);
-        } else {
-          uint32_t* it = items + size_t(j) * LW;
-#pragma unroll
-          for (uint32_t k = 0; k < LW; ++k) it[k] = w[k];
-          hashes[j] = h;
-          res[j] = res_make(kResPending, 0);
-          pending = true;
-        }
-      }
+      if (sc.table)
+        pending = build_leaf<LW>(sc.table[sc.codec.scene(sc.keys[j])], j, parent, old_up, first, recs, rmask, rmat,
+                                 items, hashes, res);
+      else
+        pending = build_leaf<LW>(sc.single, j, parent, old_up, first, recs, rmask, rmat, items, hashes, res);
     }
     const uint64_t pm = ballot(pending);
     if (pending) ls.pend[sl * kWave + lanes_below(pm)] = j;
