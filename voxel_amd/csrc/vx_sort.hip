@@ -88,9 +88,12 @@ __device__ __forceinline__ void count_digits(uint32_t* h, uint32_t d, uint32_t l
 // Each wave reads runs of kHistKpt * 64 consecutive keys, all loads of a run
 // issued before any is counted; two 1024-thread blocks per CU (eight waves
 // per SIMD) keep the final flush of the per-block histograms into the global
-// ones to a few hundred atomics per counter.
+// ones to a few hundred atomics per counter. The block keeps kHistCopies
+// copies of its histograms (wave w counts into copy w % kHistCopies), so at
+// most four waves contend for one LDS counter; the flush sums them.
 constexpr uint32_t kHistKpt = 8;
 constexpr uint32_t kHistBlock = 1024;
+constexpr uint32_t kHistCopies = 4;
 
 template <class K, uint32_t RB, bool COHERENT>
 __global__ __launch_bounds__(kHistBlock) void histogram_kernel(const K* __restrict__ keys, const uint32_t* n_ptr,
@@ -99,12 +102,15 @@ __global__ __launch_bounds__(kHistBlock) void histogram_kernel(const K* __restri
   constexpr uint32_t R = 1u << RB;
   constexpr uint32_t MAXP = (sizeof(K) * 8 + RB - 1) / RB;
   constexpr uint32_t CHUNK = kHistBlock * kHistKpt;
-  __shared__ uint32_t s[MAXP * R];
+  constexpr uint32_t COPY = MAXP * R;
+  static_assert(size_t(kHistCopies) * COPY * 4 <= 64 * 1024, "histogram copies exceed static LDS");
+  __shared__ uint32_t s_all[kHistCopies * COPY];
   const uint32_t n = count_of(n_ptr, n_arg);
   const uint32_t passes = (key_bits + RB - 1) / RB;
   const uint32_t lane = lane_id();
-  for (uint32_t i = threadIdx.x; i < passes * R; i += kHistBlock) s[i] = 0;
+  for (uint32_t i = threadIdx.x; i < kHistCopies * COPY; i += kHistBlock) s_all[i] = 0;
   __syncthreads();
+  uint32_t* s = s_all + (wave_id() % kHistCopies) * COPY;
   for (uint64_t c0 = uint64_t(blockIdx.x) * CHUNK; c0 < n; c0 += uint64_t(gridDim.x) * CHUNK) {
     const uint32_t first = wave_id() * (kWave * kHistKpt);  // the wave's first key in the chunk
     const K* run = keys + c0 + first;
@@ -128,8 +134,12 @@ __global__ __launch_bounds__(kHistBlock) void histogram_kernel(const K* __restri
     }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < passes * R; i += kHistBlock)
-    if (s[i]) atomicAdd(&hist[i], s[i]);
+  for (uint32_t i = threadIdx.x; i < passes * R; i += kHistBlock) {
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < kHistCopies; ++c) v += s_all[c * COPY + i];
+    if (v) atomicAdd(&hist[i], v);
+  }
 }
 
 // Exclusive scan of each pass's R counters (one block of max(R, 256)
