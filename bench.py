@@ -722,7 +722,7 @@ def kernel_bytes(prof, leaf_words):
     E1 = int(prof.height_entries[1])  # inner entries at height 1 (the leaves' parents)
     lw = 4 * leaf_words
     return {
-        "records": (prof.sort_record_bytes + 12) * R,   # sort record + mask 8 + material 4 written (the touched
+        "records": (prof.sort_record_bytes + 16) * R,   # sort record + {mask 8, material 4, pad 4} written (the touched
                                                         # rows, 16 bytes each and ~R/16 of them: not counted)
         "sort_histogram": 8 * R,                        # keys read once for every pass's histogram
         "sort_partition": 2 * prof.sort_record_bytes * R * P,  # per pass: each record read and written
@@ -731,11 +731,11 @@ def kernel_bytes(prof, leaf_words):
         "topdown": 12 * (N - E1) + 12 * E1,             # heights >= 2: key read, old ref written; height 1:
                                                         # parent link read, old ref written, and the parent's
                                                         # old node word (once per parent)
-        "leaf_build": 16 * L + (prof.sort_record_bytes + 12) * R + (lw + 12) * Q + 12 * E1,
+        "leaf_build": 16 * L + (prof.sort_record_bytes + 16) * R + (lw + 12) * Q + 12 * E1,
                                                         # per leaf: record range, parent link, result; per
-                                                        # record: sorted entry, mask, material; per pending
-                                                        # leaf: item and hash written, list entry; per parent:
-                                                        # its old ref and node word
+                                                        # record: sorted entry, its 16-byte mask and material;
+                                                        # per pending leaf: item and hash written, list entry;
+                                                        # per parent: its old ref and node word
         "leaf_probe": (40 + lw) * Q + L // 16,          # pending entry, list entry, item, hash, result read and
                                                         # written; slice counts and bases
         "leaf_commit": 28 * Q,                          # list entry, result, hash read; slot value written
@@ -901,7 +901,7 @@ COUNTER_PASSES = [
 ]
 
 
-COUNTER_KERNELS = ("partition_kernel", "histogram_kernel", "leaf_build_kernel")
+COUNTER_KERNELS = ("partition_kernel", "histogram_kernel", "leaf_build_kernel", "scan_vec_kernel")
 
 
 def counter_summary(out_path, csv_paths):

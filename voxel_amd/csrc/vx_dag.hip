@@ -592,10 +592,12 @@ __device__ __forceinline__ uint64_t brick_mask(const vx_brush& b, int32_t X, int
 struct RecordOut {
   uint64_t* key;
   uint32_t* val;
-  uint64_t* mask;
-  uint32_t* mat;
+  uint4* pay;  // {mask low, mask high, material, 0}: one 16-byte read per record later
   uint32_t* n_rec;
   uint32_t index_bits;
+  __device__ __forceinline__ void put_payload(uint32_t pos, uint64_t mask, uint32_t mat) const {
+    pay[pos] = make_uint4(uint32_t(mask), uint32_t(mask >> 32), mat, 0u);
+  }
   __device__ __forceinline__ void put_key(uint32_t pos, uint64_t k) const {
     if (index_bits) {
       key[pos] = (k << index_bits) | pos;
@@ -790,8 +792,8 @@ __global__ void brush_records_kernel(BrushCands c, TouchedRows rows, RecordOut o
       const uint32_t gx = j.bx0 + rows.x0[my_row] + (i - my_start);
       const uint32_t gy = j.by0 + (yz & 0xffffu), gz = j.bz0 + (yz >> 16);
       out.put_key(i, (uint64_t(j.scene) << c.codec.shift) | morton3(gx, gy, gz));
-      out.mask[i] = brick_mask(j.b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4));
-      out.mat[i] = c.materials ? j.b.material : (j.b.material ? 1u : 0u);
+      out.put_payload(i, brick_mask(j.b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4)),
+                      c.materials ? j.b.material : (j.b.material ? 1u : 0u));
     }
     row += uint32_t(__builtin_amdgcn_readlane(int(k), kWave - 1));
   }
@@ -818,8 +820,7 @@ struct RecordStore {
                              uint32_t n) const {
     if (v) {
       out.put_key(excl, rec.key);
-      out.mask[excl] = rec.mask;
-      out.mat[excl] = rec.mat;
+      out.put_payload(excl, rec.mask, rec.mat);
     }
     if (i == n - 1) *out.n_rec = incl;
   }
@@ -1044,16 +1045,16 @@ __device__ __forceinline__ uint32_t child_of_wide(const uint32_t* pool, uint64_t
 // (indices are clamped instead of branched on, so no round waits for a
 // conditional region): the record range, parent link and scene root; the
 // first record's index and the parent's old ref; the first record's mask and
-// material with the parent's old node. Most leaves have one record, so the
-// records cost no round trip of their own. d is the scene's descriptor by
-// value, so its fields are plain loads (kernel arguments in the single-scene
-// case). Returns whether the entry is left PENDING (its item and hash
-// written).
+// material (one 16-byte read) with the parent's old node. Most leaves have
+// one record, so the records cost no round trip of their own. d is the
+// scene's descriptor by value, so its fields are plain loads (kernel
+// arguments in the single-scene case). Returns whether the entry is left
+// PENDING (its item and hash written).
 template <uint32_t LW>
 __device__ __forceinline__ bool build_leaf(const Dev d, uint32_t j, const uint32_t* __restrict__ parent,
                                            const uint32_t* __restrict__ old_up, const uint32_t* __restrict__ first,
-                                           const SortedRecs& recs, const uint64_t* __restrict__ rmask,
-                                           const uint32_t* __restrict__ rmat, uint32_t* __restrict__ items,
+                                           const SortedRecs& recs, const uint4* __restrict__ rpay,
+                                           uint32_t* __restrict__ items,
                                            uint64_t* __restrict__ hashes, uint64_t* __restrict__ res) {
   const bool top = d.depth == 0;  // the leaf is the scene's root
   const uint32_t r0 = first[j], r1 = first[j + 1];
@@ -1064,8 +1065,7 @@ __device__ __forceinline__ bool build_leaf(const Dev d, uint32_t j, const uint32
   // root's missing parent.
   const uint32_t up = old_up[top ? 0u : link_parent(link)];
   const uint32_t idx0 = recs.index(r0 < r1 ? r0 : 0u);
-  const uint64_t m0 = rmask[idx0];
-  const uint32_t mat0 = rmat[idx0];
+  const uint4 pay0 = rpay[idx0];
   const uint32_t child = child_of_wide(d.node_pool, d.node_cap, up, link_slot(link));
   const uint32_t ref = top ? root : child;
   uint32_t w[LW];
@@ -1099,10 +1099,11 @@ __device__ __forceinline__ bool build_leaf(const Dev d, uint32_t j, const uint32
       }
     }
   };
-  if (r0 < r1) apply(m0, mat0);
+  auto mask_of = [](const uint4& q) { return (uint64_t(q.y) << 32) | q.x; };
+  if (r0 < r1) apply(mask_of(pay0), pay0.z);
   for (uint32_t r = r0 + 1; r < r1; ++r) {
-    const uint32_t idx = recs.index(r);
-    apply(rmask[idx], rmat[idx]);
+    const uint4 q = rpay[recs.index(r)];
+    apply(mask_of(q), q.z);
   }
   uint32_t any = 0, all = 0xffffffffu;
 #pragma unroll
@@ -1135,8 +1136,8 @@ __device__ __forceinline__ bool build_leaf(const Dev d, uint32_t j, const uint32
 template <uint32_t LW>
 __global__ __launch_bounds__(kLeafBlock, 8) void leaf_build_kernel(
     Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ parent, const uint32_t* __restrict__ old_up,
-    const uint32_t* __restrict__ first, SortedRecs recs, const uint64_t* __restrict__ rmask,
-    const uint32_t* __restrict__ rmat, uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
+    const uint32_t* __restrict__ first, SortedRecs recs, const uint4* __restrict__ rpay,
+    uint32_t* __restrict__ items, uint64_t* __restrict__ hashes,
     uint64_t* __restrict__ res, LeafSlices ls) {
   const uint32_t n = *n_ptr;
   const uint32_t slices = (n + kWave - 1) / kWave;
@@ -1147,10 +1148,10 @@ __global__ __launch_bounds__(kLeafBlock, 8) void leaf_build_kernel(
     bool pending = false;
     if (j < n) {
       if (sc.table)
-        pending = build_leaf<LW>(sc.table[sc.codec.scene(sc.keys[j])], j, parent, old_up, first, recs, rmask, rmat,
-                                 items, hashes, res);
+        pending = build_leaf<LW>(sc.table[sc.codec.scene(sc.keys[j])], j, parent, old_up, first, recs, rpay, items,
+                                 hashes, res);
       else
-        pending = build_leaf<LW>(sc.single, j, parent, old_up, first, recs, rmask, rmat, items, hashes, res);
+        pending = build_leaf<LW>(sc.single, j, parent, old_up, first, recs, rpay, items, hashes, res);
     }
     const uint64_t pm = ballot(pending);
     if (pending) ls.pend[sl * kWave + lanes_below(pm)] = j;
@@ -1294,8 +1295,9 @@ struct EditWs {
   uint32_t* root_pos;
   uint32_t* status;  // per-scene first-error words of an executor's edits
   uint32_t* base;    // per-scene pool snapshots of an executor's edits (2 words each)
-  uint64_t *rkey, *rkey_alt, *rmask;
-  uint32_t *rval, *rval_alt, *rmat;
+  uint64_t *rkey, *rkey_alt;
+  uint4* rpay;  // per record: brick mask and material (RecordOut::pay)
+  uint32_t *rval, *rval_alt;
   void* sort_ws;
   sort::Tuning sort_tn;  // the tuning sort_ws was sized for
   uint32_t index_bits;  // > 0: records sort as packed u64 keys (RecordOut)
@@ -1557,10 +1559,9 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
   w.index_bits = !pairs && std::max(p.key_bits, 1u) + ib <= 64 ? ib : 0u;
   w.rkey = c.take<uint64_t>(R);
   w.rkey_alt = c.take<uint64_t>(R);
-  w.rmask = c.take<uint64_t>(R);
+  w.rpay = c.take<uint4>(R);
   w.rval = w.index_bits ? nullptr : c.take<uint32_t>(R);
   w.rval_alt = w.index_bits ? nullptr : c.take<uint32_t>(R);
-  w.rmat = c.take<uint32_t>(R);
   w.sort_tn = w.index_bits ? sort::keys_u64_dev_tuning(std::max(p.key_bits, 1u)) : sort::pairs_u64_dev_tuning();
   w.sort_ws = c.take<char>(w.index_bits ? sort::keys_u64_dev_workspace_bytes(R, std::max(p.key_bits, 1u), w.sort_tn)
                                         : sort::pairs_u64_dev_workspace_bytes(R, p.key_bits, w.sort_tn));
@@ -1700,11 +1701,11 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
       TimedLaunch tl(VX_K_LEAF_BUILD, s);
       if (p.leaf_words == 2)
         leaf_build_kernel<2><<<grid_for(L.cap[0]), kLeafBlock, 0, s>>>(sh, &cnt[1], L.parent[0], L.old[1],
-                                                                       L.first[0], recs, w.rmask, w.rmat, w.items,
+                                                                       L.first[0], recs, w.rpay, w.items,
                                                                        w.hashes, L.res[0], ls);
       else
         leaf_build_kernel<16><<<grid_for(L.cap[0]), kLeafBlock, 0, s>>>(sh, &cnt[1], L.parent[0], L.old[1],
-                                                                        L.first[0], recs, w.rmask, w.rmat, w.items,
+                                                                        L.first[0], recs, w.rpay, w.items,
                                                                         w.hashes, L.res[0], ls);
     }
     if ((st = last_launch()) != VX_OK) return st;
@@ -1899,7 +1900,7 @@ vx_status upload(EditWs& w, const std::vector<Dev>& devs, const std::vector<Brus
 
 // The record writer that matches each candidate kind.
 inline RecordOut record_out(EditWs& w) {
-  return RecordOut{w.rkey, w.rval, w.rmask, w.rmat, &w.counts[0], w.index_bits};
+  return RecordOut{w.rkey, w.rval, w.rpay, &w.counts[0], w.index_bits};
 }
 inline RecordStore store_for(const VoxelLoad&, EditWs& w) {
   return RecordStore{record_out(w)};
