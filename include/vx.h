@@ -500,6 +500,24 @@ vx_status vx_dag_collect_garbage(vx_dag* dag, vx_gc_report* report, void* stream
 vx_status vx_dag_read_pools(const vx_dag* dag, uint32_t* node_out, uint64_t node_cap, uint64_t* node_n,
                             uint32_t* leaf_out, uint64_t leaf_cap, uint64_t* leaf_n);
 
+/* Device-side check of the tables' invariant (no counterpart in the
+ * reference, which relies on it without checking: insert_unique_unchecked
+ * requires uniqueness, cuda/hash_table/acceleration_hash/set/common.cuh:51-56,
+ * guaranteed per depth by svt/hash_dag_gpu/detail/edit/temp_dedup.cuh:29-34):
+ * no two live slots hold items of equal content. Synchronous. out[0..2] for
+ * the node table, out[3..5] for the leaf table: live slots, duplicates, and
+ * broken slots (a tag left over, an offset outside the used words, a
+ * fingerprint that is not the item's, or a slot a lookup cannot reach). */
+vx_status vx_dag_check_unique(const vx_dag* dag, uint64_t out[6], void* stream);
+
+/* The same check over a caller's table: n_slots (a power of two) u64 slots
+ * {fingerprint:32 | pool offset:32} (0 empty, 1 dead) over a u32 pool whose
+ * words [0, pool_words_used) are in use; items of item_words words, or DAG
+ * node items (1 + popcount of the header's low byte) when item_words is 0.
+ * out[3]: live slots, duplicates, broken slots. */
+vx_status vx_table_check_unique(const uint64_t* d_slots, uint64_t n_slots, const uint32_t* d_pool,
+                                uint64_t pool_words_used, uint32_t item_words, uint64_t out[3], void* stream);
+
 /* ---- pointer SVO voxel edits (algo::svt::cuda) -------------------------------
  * The mutable sparse voxel octree of svt/cuda: 4096^3 world, ten inner
  * levels of 2x2x2 (config.cuh:16-36); nodes of eight u32 slots {child

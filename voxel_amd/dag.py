@@ -63,6 +63,8 @@ register({
     "vx_dag_export_canonical_device": (_i32, [_P, _P, _u64, C.POINTER(_u64), _P]),
     "vx_dag_collect_garbage": (_i32, [_P, _P, _P]),
     "vx_dag_read_pools": (_i32, [_P, _P, _u64, C.POINTER(_u64), _P, _u64, C.POINTER(_u64)]),
+    "vx_dag_check_unique": (_i32, [_P, C.POINTER(_u64), _P]),
+    "vx_table_check_unique": (_i32, [_P, _u64, _P, _u64, _u32, C.POINTER(_u64), _P]),
     "vx_brush_voxel_ops": (_i32, [_P, _u32, _u32, C.POINTER(_u64)]),
     "vx_gen_brushes": (_i32, [_u64, _u64, _u32, _u32, _u32, _P]),
     "vx_gen_sized_batch": (_i32, [_u64, _u64, _u64, _u32, _u32, _u32, _u32, _u32, _P, C.POINTER(_u32),
@@ -282,6 +284,14 @@ class HashDagGpu:
         check(lib().vx_dag_read_pools(self.h, npool.ctypes.data_as(_P), nn.value, C.byref(nn),
                                       lpool.ctypes.data_as(_P), ln.value, C.byref(ln)), "vx_dag_read_pools")
         return npool, lpool
+
+    def check_unique(self, stream=None) -> dict:
+        """Device-side invariant check of both interning tables: live slots,
+        duplicate items and broken slots per table."""
+        out = (_u64 * 6)()
+        check(lib().vx_dag_check_unique(self.h, out, _stream(stream)), "vx_dag_check_unique")
+        return {"nodes": {"live": out[0], "duplicates": out[1], "broken": out[2]},
+                "leaves": {"live": out[3], "duplicates": out[4], "broken": out[5]}}
 
     def export(self) -> np.ndarray:
         """Canonical export (host array of u32 words)."""
