@@ -543,7 +543,8 @@ typedef struct vx_svo_view {
   uint32_t max_node_count, max_leaf_count;
 } vx_svo_view;
 /* GpuSvo<MaxNodeCount, MaxLeafCount> (svt/cuda/storage.cuh:12-122): capacities
- * are runtime values here. Allocation only; call vx_svo_reset before use. */
+ * are runtime values here. The tree starts reset (as after vx_svo_reset),
+ * where the reference's constructor leaves it unusable until reset_svo. */
 vx_status vx_svo_create(uint32_t max_node_count, uint32_t max_leaf_count, vx_svo** out);
 vx_status vx_svo_destroy(vx_svo* svo);
 /* reset_svo (svt/cuda/svo.cuh:14-22, detail/reset.cuh:9-21): empty root,

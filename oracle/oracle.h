@@ -55,6 +55,14 @@ void vxo_sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint64_t n,
                         uint32_t key_bits);
 void vxo_sort_pairs_u64(uint64_t* keys, uint32_t* vals, uint64_t n,
                         uint32_t key_bits);
+/* u64 scans (wrapping sums, maxima). */
+void vxo_exclusive_sum_u64(const uint64_t* in, uint64_t n, uint64_t* out);
+void vxo_inclusive_sum_u64(const uint64_t* in, uint64_t n, uint64_t* out);
+void vxo_exclusive_max_u64(const uint64_t* in, uint64_t n, uint64_t* out);
+void vxo_inclusive_max_u64(const uint64_t* in, uint64_t n, uint64_t* out);
+/* Stable order of n keys of `words` u32 words (word 0 least significant),
+ * compared on their low key_bits bits: order[i] = index of the i-th key. */
+void vxo_sort_order(const uint32_t* keys, uint32_t words, uint64_t n, uint32_t key_bits, uint32_t* order);
 
 /* ---- hash set truth (pinned by construction) ---------------------------- */
 /* out[i] = 1 if queries[i] is one of keys[0..nk). */
@@ -81,6 +89,7 @@ typedef struct vxo_brush {
 } vxo_brush;
 
 int32_t vxo_terrain_height(const vxo_terrain* t, uint32_t x, uint32_t z);
+void vxo_terrain_heights(const vxo_terrain* t, const uint32_t* xs, const uint32_t* zs, uint64_t n, int32_t* out);
 /* Brush mixes: 0 = 512^3 reference case (spheres r[4,16], 50/50 add);
  * 1 = large mix (50% spheres r[8,64], 50% boxes h[4,48], 60/40 add). */
 void vxo_gen_brushes(uint64_t seed, uint64_t stream, uint32_t n,

@@ -127,6 +127,23 @@ def test_invalid_arguments(gpu):
     assert s.counters() == (1, 0, 0, 0)
 
 
+def test_fresh_svo_is_reset_without_reset_call(gpu):
+    """A tree straight from vx_svo_create is already reset (the reference's
+    constructor only allocates): counters {1, 0, 0, 0}, and edits on it
+    equal to the oracle's."""
+    from voxel_amd.svo import GpuSvo
+    g = GpuSvo(1 << 16, 1 << 14, reset=False)
+    assert g.counters() == (1, 0, 0, 0)
+    rng = np.random.default_rng(12)
+    xyz = rng.integers(0, 4096, (4000, 3)).astype(np.uint32)
+    ref = O.Svo(1 << 16, 1 << 14)
+    g.place_voxel_edits(xyz)
+    ref.apply(xyz, 0)
+    g.destroy_voxel_edits(xyz[:2000])
+    ref.apply(xyz[:2000], 1)
+    assert g.counters() == ref.counters() and np.array_equal(g.export(), ref.export())
+
+
 def test_reference_sphere_workload(gpu):
     """bench_cuda_svt.cu sphere case: r = 128 at the world centre, 8.78M voxels
     / 144k leaves, placed on a reset SVO of the benchmark's capacity

@@ -26,9 +26,19 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+# Host-sanitized variant (tooling: tests/test_sanitizers.py): the host half
+# of every translation unit with AddressSanitizer and UndefinedBehavior-
+# Sanitizer, each flag behind -Xarch_host so the device code is the normal
+# build's. The sanitizer runtimes come from the executable that loads it.
+ASAN_DIR = os.path.join(OUT_DIR, "asan")
+ASAN_LIB = os.path.join(ASAN_DIR, "libvoxel_amd_asan.so")
+HOST_SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def _compile(src: str, out_dir: str = OUT_DIR, extra: tuple = ()) -> str:
+    obj = os.path.join(out_dir, os.path.basename(src) + ".o")
+    cmd = [HIPCC, *FLAGS, *extra, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
@@ -52,5 +62,24 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+def build_host_sanitized(verbose: bool = True) -> str:
+    os.makedirs(ASAN_DIR, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)))
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(lambda s: _compile(s, ASAN_DIR, tuple(HOST_SANITIZE)), srcs))
+    tmp = ASAN_LIB + ".tmp"
+    r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, ASAN_LIB)
+    if verbose:
+        print(f"built {ASAN_LIB} (host code under ASan/UBSan) from {len(srcs)} sources", file=sys.stderr)
+    return ASAN_LIB
+
+
 if __name__ == "__main__":
     build()
+    if "--host-sanitizers" in sys.argv:
+        build_host_sanitized()

@@ -524,6 +524,15 @@ extern "C" vx_status vx_svo_create(uint32_t max_node_count, uint32_t max_leaf_co
     vx_svo_destroy(g);
     return VX_OUT_OF_CAPACITY;
   }
+  // The tree starts reset (empty root, counters {1, 0, 0, 0}): the
+  // reference's constructor only allocates and leaves the counters and free
+  // stacks as the allocation found them until reset_svo, and an edit on such
+  // a tree pops whatever indices those words hold.
+  reset_kernel<<<1, 64>>>(dev_of(g));
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    vx_svo_destroy(g);
+    return VX_DEVICE_ERROR;
+  }
   *out = g;
   return VX_OK;
 }
