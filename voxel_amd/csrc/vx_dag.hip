@@ -1068,8 +1068,11 @@ __device__ __forceinline__ bool build_leaf(const Dev d, uint32_t j, const uint32
   const uint4 pay0 = rpay[idx0];
   const uint32_t child = child_of_wide(d.node_pool, d.node_cap, up, link_slot(link));
   const uint32_t ref = top ? root : child;
+  // A first record whose mask covers the whole brick (a brush interior)
+  // overwrites every voxel, so the old leaf is not read.
+  const bool covered = r0 < r1 && (pay0.x & pay0.y) == 0xffffffffu;
   uint32_t w[LW];
-  if (ref == kEmpty) {
+  if (ref == kEmpty || covered) {
 #pragma unroll
     for (uint32_t k = 0; k < LW; ++k) w[k] = 0;
   } else if (ref == kFull) {

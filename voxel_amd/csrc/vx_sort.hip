@@ -56,16 +56,51 @@ __device__ __forceinline__ uint32_t count_of(const uint32_t* n_ptr, uint32_t n_a
 }
 
 // ---- pass 0: histograms of every digit position in one read ----------------
+// Lanes of the wave whose digit equals this lane's, among the lanes of
+// `init`: one ballot per digit bit, each narrowing the match with one
+// three-input bit operation per 32-lane half (m &= ~(ballot ^ my bit)).
+template <uint32_t RB>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t init) {
+  uint32_t lo = uint32_t(init), hi = uint32_t(init >> 32);
+#pragma unroll
+  for (uint32_t b = 0; b < RB; ++b) {
+    const uint32_t bm = uint32_t(__builtin_amdgcn_sbfe(int(d), b, 1));  // all ones iff bit b is set
+    const uint64_t bal = ballot(bm != 0u);
+    lo &= ~(uint32_t(bal) ^ bm);
+    hi &= ~(uint32_t(bal >> 32) ^ bm);
+  }
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// The same match over a digit of `bits` bits (a wave-uniform run-time value).
+__device__ __forceinline__ uint64_t match_digit_n(uint32_t d, uint32_t bits, uint64_t init) {
+  uint32_t lo = uint32_t(init), hi = uint32_t(init >> 32);
+  for (uint32_t b = 0; b < bits; ++b) {
+    const uint32_t bm = uint32_t(__builtin_amdgcn_sbfe(int(d), b, 1));
+    const uint64_t bal = ballot(bm != 0u);
+    lo &= ~(uint32_t(bal) ^ bm);
+    hi &= ~(uint32_t(bal >> 32) ^ bm);
+  }
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// A digit of at most this many bits is counted once per distinct value in
+// the wave: with 64 lanes over 2^bits values, same-address LDS atomics from
+// one instruction (which serialise) would otherwise dominate, as for the
+// 2-bit top digit of 42-bit keys in 8-bit digits.
+constexpr uint32_t kMatchCountBits = 6;
+
 // Adds the wave's digits of one pass (lanes at and above `valid_lanes` hold
 // no key) into the block's LDS histogram h.
 // COHERENT (keys that arrive in spatially coherent runs, like the edit's
 // brick records in brush order): equal digits then come in runs over
 // consecutive lanes, and same-address LDS atomics from one instruction
 // serialise, so the first lane of each run adds the run's length; a wave
-// with many runs (the low digits) adds one per lane instead. Random keys
-// always add one per lane.
+// with many runs (the low digits) adds one per lane instead. Random keys add
+// one per lane, or, for narrow digits, one per distinct value.
 template <bool COHERENT>
-__device__ __forceinline__ void count_digits(uint32_t* h, uint32_t d, uint32_t lane, uint32_t valid_lanes) {
+__device__ __forceinline__ void count_digits(uint32_t* h, uint32_t d, uint32_t bits, uint32_t lane,
+                                             uint32_t valid_lanes) {
   const bool valid = lane < valid_lanes;
   if constexpr (COHERENT) {
     const uint32_t key = valid ? d : 0xffffffffu;  // a run of its own, never added
@@ -81,6 +116,12 @@ __device__ __forceinline__ void count_digits(uint32_t* h, uint32_t d, uint32_t l
       }
       return;
     }
+  }
+  if (bits <= kMatchCountBits) {  // wave-uniform
+    const uint64_t live = valid_lanes >= kWave ? ~0ull : ((1ull << valid_lanes) - 1ull);
+    const uint64_t peers = match_digit_n(d, bits, live);
+    if (valid && lanes_below(peers) == 0) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+    return;
   }
   if (valid) atomicAdd(&h[d], 1u);
 }
@@ -127,7 +168,7 @@ __global__ __launch_bounds__(kHistBlock) void histogram_kernel(const K* __restri
       for (uint32_t p = 0; p < MAXP; ++p) {
         if (p < passes) {
           const uint32_t bits = min(RB, key_bits - p * RB);
-          count_digits<COHERENT>(s + p * R, digit_of(key[j], begin_bit + p * RB, (1u << bits) - 1), lane,
+          count_digits<COHERENT>(s + p * R, digit_of(key[j], begin_bit + p * RB, (1u << bits) - 1), bits, lane,
                                  valid_lanes);
         }
       }
@@ -157,22 +198,6 @@ __global__ __launch_bounds__((1u << RB) > 256 ? (1u << RB) : 256) void hist_scan
   uint32_t before = 0;
   for (uint32_t w = 0; w < wave_id(); ++w) before += s_wave[w];
   if (t < R) h[t] = before + inc - v;
-}
-
-// Lanes of the wave whose digit equals this lane's, among the lanes of
-// `init`: one ballot per digit bit, each narrowing the match with one
-// three-input bit operation per 32-lane half (m &= ~(ballot ^ my bit)).
-template <uint32_t RB>
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t init) {
-  uint32_t lo = uint32_t(init), hi = uint32_t(init >> 32);
-#pragma unroll
-  for (uint32_t b = 0; b < RB; ++b) {
-    const uint32_t bm = uint32_t(__builtin_amdgcn_sbfe(int(d), b, 1));  // all ones iff bit b is set
-    const uint64_t bal = ballot(bm != 0u);
-    lo &= ~(uint32_t(bal) ^ bm);
-    hi &= ~(uint32_t(bal >> 32) ^ bm);
-  }
-  return (uint64_t(hi) << 32) | lo;
 }
 
 // Loads of a wave's KPT keys (lane-contiguous rounds) and their ranks among
@@ -504,7 +529,8 @@ vx_status drive(K* keys, uint32_t* vals, K* alt_keys, uint32_t* alt_vals, const 
   }
   if ((st = last_launch()) != VX_OK) return st;
 
-  hipFuncSetAttribute(reinterpret_cast<const void*>(L.kernel), hipFuncAttributeMaxDynamicSharedMemorySize, int(L.lds));
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(L.kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            int(L.lds));
   // Persistent blocks: as many as can be resident at once (LDS, registers
   // and waves per CU all count).
   int resident = 0;
