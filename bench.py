@@ -40,6 +40,7 @@ KERNEL_SYMBOL = {
     "node_build": "node_build_kernel",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_edit_16384.json")
+PRIM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_primitives.json")
 VENDOR_SORT_BIN = os.path.join(ROOT, "voxel_amd", "_build", "bench_vendor_sort")
 
 
@@ -94,6 +95,9 @@ def parse():
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarise two rocprofv3 --pmc counter CSVs of this benchmark (FETCH_SIZE, WRITE_SIZE) "
                          "into the per-kernel traffic file used for roofline.traffic, then exit")
+    ap.add_argument("--traffic-primitives", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarise FETCH_SIZE / WRITE_SIZE counter CSVs of a --primitives-only --copy-calibration "
+                         "run per kernel instantiation into the primitives traffic file, then exit")
     ap.add_argument("--kernel-trace", metavar="TRACE",
                     help="summarise a rocprofv3 --kernel-trace output (CSV or rocpd .db) of this benchmark per edit "
                          "kernel kind and exit")
@@ -890,6 +894,62 @@ def traffic_summary(fetch_csv, write_csv, out_path):
     print(json.dumps(out["kernels"], indent=1))
 
 
+PRIM_PMC_COMMAND = ("cd /tmp && export TMPDIR=/tmp && cd <repo> && "
+                    "rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmcp_{counter} -o prim -- "
+                    "python3 bench.py --primitives-only --copy-calibration --no-vendor-sort")
+
+
+def primitive_traffic_summary(fetch_csv, write_csv, out_path):
+    """HBM bytes per dispatch of every sort and scan kernel instantiation of
+    the primitives leg (sorts of 2^28 u32 and 2^27 u64 pairs, 2^28 scan), from
+    FETCH_SIZE / WRITE_SIZE counter CSVs scaled as in traffic_summary by the
+    calibration copy, beside their algorithmic bytes per dispatch."""
+    import csv
+
+    def name_of(n):
+        if "bitwise_not" in n:
+            return "bitwise_not"
+        n = n.replace("(anonymous namespace)::", "")
+        return n.split("(")[0].replace("void ", "")
+
+    def load(path, counter):
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                acc.setdefault(name_of(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+        return acc
+
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    cal_f, cal_w = fetch["bitwise_not"][-1], write["bitwise_not"][-1]
+    f_scale, w_scale = (1 << 30) / (cal_f * 1024), (1 << 30) / (cal_w * 1024)
+    n32, n64 = 1 << 28, 1 << 27
+    algo = {"partition_kernel<unsigned int, true": 16 * n32, "partition_kernel<unsigned long, true": 24 * n64,
+            "histogram_kernel<unsigned int,": 4 * n32, "histogram_kernel<unsigned long,": 8 * n64,
+            "scan_vec_kernel": 8 * n32}
+    kernels = {}
+    for name, fv in fetch.items():
+        if not (name.startswith("vx::sort::") or name.startswith("vx::scan::")) or name not in write:
+            continue
+        f = sum(fv) / len(fv) * 1024 * f_scale
+        w = sum(write[name]) / len(write[name]) * 1024 * w_scale
+        row = {"dispatches": len(fv), "fetch_bytes": f, "write_bytes": w, "bytes": f + w}
+        a = next((v for k, v in algo.items() if k in name), None)
+        if a:
+            row["algorithmic_bytes"] = a
+            row["traffic_over_algorithmic"] = (f + w) / a
+        kernels[name] = row
+    out = {"workload": "primitives leg: stable sort of 2^28 u32 (key, index) pairs and of 2^27 42-bit Morton "
+                       "u64 (key, index) pairs, exclusive scan of 2^28 u32",
+           "commands": [PRIM_PMC_COMMAND.format(counter=c) for c in ("FETCH_SIZE", "WRITE_SIZE")],
+           "calibration": {"kernel": "bitwise_not over 2^28 int32 (1 GiB read + 1 GiB write)",
+                           "fetch_kib": cal_f, "write_kib": cal_w, "fetch_scale": f_scale, "write_scale": w_scale},
+           "kernels": kernels}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    for k, v in kernels.items():
+        print(f"{v['bytes'] / 1e6:10.1f} MB/dispatch  x{v.get('traffic_over_algorithmic', 0):.2f}  {k}")
+
+
 COUNTER_PASSES = [
     "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_INSTS_BRANCH SQ_INSTS_SMEM",
     "SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_WAIT_INST_LDS SQ_ACTIVE_INST_VALU "
@@ -960,6 +1020,9 @@ def main():
         return
     if args.traffic:
         traffic_summary(args.traffic[0], args.traffic[1], TRAFFIC_FILE)
+        return
+    if args.traffic_primitives:
+        primitive_traffic_summary(args.traffic_primitives[0], args.traffic_primitives[1], PRIM_TRAFFIC_FILE)
         return
     if args.kernel_trace:
         kernel_trace_summary(args.kernel_trace)

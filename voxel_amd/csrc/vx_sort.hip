@@ -137,7 +137,7 @@ constexpr uint32_t kHistBlock = 1024;
 constexpr uint32_t kHistCopies = 4;
 
 template <class K, uint32_t RB, bool COHERENT>
-__global__ __launch_bounds__(kHistBlock) void histogram_kernel(const K* __restrict__ keys, const uint32_t* n_ptr,
+__global__ __launch_bounds__(kHistBlock, 8) void histogram_kernel(const K* __restrict__ keys, const uint32_t* n_ptr,
                                                                uint32_t n_arg, uint32_t begin_bit, uint32_t key_bits,
                                                                uint32_t* __restrict__ hist) {
   constexpr uint32_t R = 1u << RB;
