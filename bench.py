@@ -422,7 +422,8 @@ def primitives(torch, P):
 
     copy_ms = timed(lambda: (k.copy_(src_k), v.copy_(src_v)))
     ms = timed(sort32) - copy_ms
-    out["sort_u32_pairs_2^28"] = {"ms": ms, "keys_per_s": n / ms * 1e3, "gbs_algorithmic": 68 * n / ms / 1e6}
+    out["sort_u32_pairs_2^28"] = {"ms": ms, "keys_per_s": n / ms * 1e3, "gbs_algorithmic": 68 * n / ms / 1e6,
+                                  "verified": sorted_pairs_ok(torch, src_k, k, v, unsigned32=True)}
     del src_k, src_v, k, v, ws
     n = 1 << 27
     src_k = P.gen_morton(2, 1, n, 14)
@@ -437,7 +438,8 @@ def primitives(torch, P):
 
     copy_ms = timed(lambda: (k.copy_(src_k), v.copy_(src_v)))
     ms = timed(sort64) - copy_ms
-    out["sort_u64_morton42_pairs_2^27"] = {"ms": ms, "keys_per_s": n / ms * 1e3, "gbs_algorithmic": 152 * n / ms / 1e6}
+    out["sort_u64_morton42_pairs_2^27"] = {"ms": ms, "keys_per_s": n / ms * 1e3, "gbs_algorithmic": 152 * n / ms / 1e6,
+                                           "verified": sorted_pairs_ok(torch, src_k, k, v, unsigned32=False)}
     del src_k, src_v, k, v, ws
     n = 1 << 28
     src = P.gen_u32(2, 3, n, 0, 15)
@@ -445,8 +447,35 @@ def primitives(torch, P):
     ws = torch.empty(max(1, P.required_workspace_size(n)), dtype=torch.uint8, device="cuda")
     copy_ms = timed(lambda: a.copy_(src))
     ms = timed(lambda: (a.copy_(src), P.exclusive_sum(a, workspace=ws))) - copy_ms
-    out["exclusive_scan_u32_2^28"] = {"ms": ms, "elements_per_s": n / ms * 1e3, "gbs_algorithmic": 8 * n / ms / 1e6}
+    # The timed runs leave the scan of src in a: against torch's int64 cumsum, mod 2^32.
+    want = (torch.cumsum(src.to(torch.int64), 0) - src.to(torch.int64)) & 0xFFFFFFFF
+    scan_ok = bool(torch.equal(a.to(torch.int64) & 0xFFFFFFFF, want))
+    del want
+    out["exclusive_scan_u32_2^28"] = {"ms": ms, "elements_per_s": n / ms * 1e3, "gbs_algorithmic": 8 * n / ms / 1e6,
+                                      "verified": {"check": "equals torch.cumsum (int64) - input, mod 2^32",
+                                                   "ok": scan_ok}}
     return out
+
+
+def sorted_pairs_ok(torch, src_k, k, v, unsigned32):
+    """Size-independent checks of a stable (key, index) sort of src_k with
+    values 0..n-1, on the device: the values are a permutation of 0..n-1,
+    every key is the input key its value indexes, the keys are non-decreasing
+    (u32 keys compared as unsigned) and equal keys keep their values in
+    increasing order. Together these fix the stable sort."""
+    n = k.numel()
+    seen = torch.zeros(n, dtype=torch.int8, device=k.device)
+    seen[v.long()] = 1
+    perm = bool(seen.all())
+    del seen
+    paired = bool(torch.equal(k, src_k[v.long()]))
+    kk = k.to(torch.int64) & 0xFFFFFFFF if unsigned32 else k
+    ordered = bool((kk[1:] >= kk[:-1]).all())
+    ties = kk[1:] == kk[:-1]
+    stable = bool((v[1:][ties] > v[:-1][ties]).all())
+    del kk, ties
+    return {"check": "values a permutation of 0..n-1, keys[i] == input[values[i]], keys non-decreasing, "
+                     "equal keys in increasing value order", "ok": perm and paired and ordered and stable}
 
 
 def ray_leg(torch, scene, world, dev, n=1 << 20):

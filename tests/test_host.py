@@ -1,5 +1,7 @@
 """Host-side logic of the C-ABI library (no GPU needed): workload generators
 and voxel-op counting agree with the oracle's definitions."""
+import os
+
 import numpy as np
 
 from oracle import oracle as O
@@ -55,3 +57,35 @@ def test_sized_batches_match_workload_definition():
     want = [lo + 0.1 * (hi - lo), (lo + hi) / 2, lo + 0.9 * (hi - lo)]
     assert all(abs(a - b) < 1.0 for a, b in zip(q, want)), q
     assert logs.min() > lo - 2 and logs.max() < hi + 1
+
+
+def test_full_size_sort_check_rejects_planted_mistakes():
+    """The bench's size-independent check of a stable (key, index) sort, run
+    on CPU tensors: it accepts numpy's stable argsort and rejects an output
+    with two tied values swapped (unstable), a key that does not match its
+    index, keys out of order, and a repeated value."""
+    import sys
+
+    import torch
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    rng = np.random.default_rng(5)
+    src = rng.integers(0, 1 << 32, 5000, dtype=np.uint64).astype(np.uint32)
+    src[::7] = src[3]  # ties
+    order = np.argsort(src, kind="stable").astype(np.int32)
+    tk = torch.from_numpy(src.view(np.int32).copy())
+    good_k, good_v = tk[torch.from_numpy(order).long()].clone(), torch.from_numpy(order.copy())
+    assert bench.sorted_pairs_ok(torch, tk, good_k, good_v, unsigned32=True)["ok"]
+    tied = np.nonzero(src[order][1:] == src[order][:-1])[0][0]
+    v = good_v.clone()
+    v[tied], v[tied + 1] = good_v[tied + 1], good_v[tied]
+    assert not bench.sorted_pairs_ok(torch, tk, good_k, v, unsigned32=True)["ok"]
+    k = good_k.clone()
+    k[10] ^= 1
+    assert not bench.sorted_pairs_ok(torch, tk, k, good_v, unsigned32=True)["ok"]
+    k, v = good_k.clone(), good_v.clone()
+    k[[0, -1]], v[[0, -1]] = good_k[[-1, 0]], good_v[[-1, 0]]
+    assert not bench.sorted_pairs_ok(torch, tk, k, v, unsigned32=True)["ok"]
+    v = good_v.clone()
+    v[1] = v[0]
+    assert not bench.sorted_pairs_ok(torch, tk, good_k, v, unsigned32=True)["ok"]
