@@ -98,7 +98,7 @@ constexpr uint32_t kMatchCountBits = 6;
 // serialise, so the first lane of each run adds the run's length; a wave
 // with many runs (the low digits) adds one per lane instead. Random keys add
 // one per lane, or, for narrow digits, one per distinct value.
-template <bool COHERENT>
+template <bool COHERENT, bool NARROW = false>
 __device__ __forceinline__ void count_digits(uint32_t* h, uint32_t d, uint32_t bits, uint32_t lane,
                                              uint32_t valid_lanes) {
   const bool valid = lane < valid_lanes;
@@ -117,7 +117,7 @@ __device__ __forceinline__ void count_digits(uint32_t* h, uint32_t d, uint32_t b
       return;
     }
   }
-  if (bits <= kMatchCountBits) {  // wave-uniform
+  if (NARROW && bits <= kMatchCountBits) {  // wave-uniform
     const uint64_t live = valid_lanes >= kWave ? ~0ull : ((1ull << valid_lanes) - 1ull);
     const uint64_t peers = match_digit_n(d, bits, live);
     if (valid && lanes_below(peers) == 0) atomicAdd(&h[d], uint32_t(__popcll(peers)));
@@ -148,6 +148,7 @@ __global__ __launch_bounds__(kHistBlock, 8) void histogram_kernel(const K* __res
   __shared__ uint32_t s_all[kHistCopies * COPY];
   const uint32_t n = count_of(n_ptr, n_arg);
   const uint32_t passes = (key_bits + RB - 1) / RB;
+  const uint32_t full = key_bits / RB;  // passes of all RB bits
   const uint32_t lane = lane_id();
   for (uint32_t i = threadIdx.x; i < kHistCopies * COPY; i += kHistBlock) s_all[i] = 0;
   __syncthreads();
@@ -164,13 +165,17 @@ __global__ __launch_bounds__(kHistBlock, 8) void histogram_kernel(const K* __res
     for (uint32_t j = 0; j < kHistKpt; ++j) {
       const uint32_t valid_lanes = wave_keys > j * kWave ? min(wave_keys - j * kWave, kWave) : 0u;
       if (valid_lanes == 0) break;  // uniform across the wave
+      // Passes of RB bits, then the narrower last pass, if any (only it can
+      // take the per-distinct-value count).
 #pragma unroll
       for (uint32_t p = 0; p < MAXP; ++p) {
-        if (p < passes) {
-          const uint32_t bits = min(RB, key_bits - p * RB);
-          count_digits<COHERENT>(s + p * R, digit_of(key[j], begin_bit + p * RB, (1u << bits) - 1), bits, lane,
-                                 valid_lanes);
-        }
+        if (p < full)
+          count_digits<COHERENT>(s + p * R, digit_of(key[j], begin_bit + p * RB, R - 1), RB, lane, valid_lanes);
+      }
+      if (passes > full) {
+        const uint32_t bits = key_bits - full * RB;
+        count_digits<COHERENT, true>(s + full * R, digit_of(key[j], begin_bit + full * RB, (1u << bits) - 1), bits,
+                                     lane, valid_lanes);
       }
     }
   }
