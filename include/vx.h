@@ -494,6 +494,22 @@ typedef struct vx_gc_report {
 } vx_gc_report;
 vx_status vx_dag_collect_garbage(vx_dag* dag, vx_gc_report* report, void* stream);
 
+/* Growth of a scene's pools and slot tables to at least `cap` (a field at or
+ * below the current size keeps it; slot counts round up to powers of two).
+ * The used pool words move to the larger pools unchanged, so every ref and
+ * slot value stays valid; a table that grows is rebuilt by re-inserting each
+ * live slot at its item's home position in the larger table (the hash
+ * recomputed from the item). Counterpart of the reference's growing GPU
+ * tables, whose bump allocator hands out further slabs as buckets fill
+ * (cuda/hash_table/bump_allocator.cuh:118-121, slab_hash/set/host.cuh:107);
+ * an open-addressing table grows by rebuilding instead. Synchronous on
+ * `stream`; no other call may use the scene meanwhile (executors that hold
+ * the scene take its new buffers at their next submission and capture their
+ * graph again). Allocation failure: VX_OUT_OF_CAPACITY, a slot whose item does
+ * not hash to its fingerprint: VX_DEVICE_ERROR; either way the scene is left
+ * as it was. */
+vx_status vx_dag_grow(vx_dag* dag, const vx_dag_capacity* cap, void* stream);
+
 /* Host copy of the used words [0, used) of both pools (inspection and
  * tests; no counterpart in the reference). *node_n / *leaf_n receive the
  * sizes; a pool's words are written only if its cap suffices. */

@@ -69,6 +69,8 @@ static void validation() {
   REQUIRE(vx_slab_alloc_destroy(nullptr) == VX_INVALID_VALUE);
   REQUIRE(vx_svo_destroy(nullptr) == VX_INVALID_VALUE);
   REQUIRE(vx_table_check_unique(nullptr, 64, nullptr, 16, 2, nullptr, nullptr) == VX_INVALID_VALUE);
+  const vx_dag_capacity grow_cap{1 << 20, 1 << 20, 1 << 16, 1 << 16};
+  REQUIRE(vx_dag_grow(nullptr, &grow_cap, nullptr) == VX_INVALID_VALUE);
 }
 
 static void primitives() {
@@ -241,6 +243,15 @@ static void dags() {
   REQUIRE(after.size() == words && after == before);
   vx_dag_stats stats{};
   OK(vx_dag_stats_get(g[0], &stats));
+  // Growth: pools and tables twice as large, the same canonical DAG.
+  const vx_dag_capacity bigger{2 * (stats.node_words_used + 4096), 2 * (stats.leaf_words_used + 4096),
+                               2 * stats.node_slots, 2 * stats.leaf_slots};
+  REQUIRE(vx_dag_grow(g[0], nullptr, nullptr) == VX_INVALID_VALUE);
+  OK(vx_dag_grow(g[0], &bigger, nullptr));
+  OK(vx_dag_stats_get(g[0], &stats));
+  REQUIRE(stats.node_slots == bigger.node_slots && stats.leaf_slots == bigger.leaf_slots);
+  OK(vx_dag_export_canonical(g[0], after.data(), words, &words));
+  REQUIRE(after == before);
   uint64_t nn = 0, ln = 0;
   OK(vx_dag_read_pools(g[0], nullptr, 0, &nn, nullptr, 0, &ln));
   for (void* p : {static_cast<void*>(dv), static_cast<void*>(got), static_cast<void*>(dr), static_cast<void*>(dh),

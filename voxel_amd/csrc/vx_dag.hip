@@ -2313,6 +2313,7 @@ struct vx_batcher {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<Dev> devs;
+  std::vector<uint64_t> generations;  // of each scene when its `devs` entry was copied
   std::vector<BrushJob> jobs;
   HostStage stage;
   // Asynchronous submissions: one in flight at a time, completed by query.
@@ -2348,6 +2349,7 @@ extern "C" vx_status vx_batcher_create(vx_dag* const* scenes, uint32_t n_scenes,
     }
     b->scenes.push_back(scenes[i]);
     b->devs.push_back(scenes[i]->dev);
+    b->generations.push_back(scenes[i]->generation);
   }
   b->plan = make_plan(std::vector<const vx_dag*>(b->scenes.begin(), b->scenes.end()));
   if (b->plan.key_bits > 64) {
@@ -2417,6 +2419,21 @@ static vx_status batcher_submit(vx_batcher* b, const uint32_t* scene_of_batch, c
         cand += make_jobs(brushes[i], counts[i], 4u << b->scenes[s]->depth, s, cand, &b->jobs, &rows);
   if (candidates) *candidates = cand;
   if (cand > b->cap_cand || b->jobs.size() > b->cap_jobs) return VX_INVALID_VALUE;
+  // A scene whose buffers moved since it was copied (vx_dag_grow): take its
+  // new descriptor, and capture the pipeline again (the graph holds the old
+  // one in its kernel arguments).
+  for (uint32_t i = 0; i < b->scenes.size(); ++i) {
+    if (b->generations[i] == b->scenes[i]->generation) continue;
+    Dev d = b->scenes[i]->dev;
+    d.status = b->w.status + i;
+    d.base = b->w.base + 2 * i;
+    b->devs[i] = d;
+    b->generations[i] = b->scenes[i]->generation;
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
   hipStream_t s = b->stream;
   vx_status st;
   const uint32_t n_scenes = uint32_t(b->scenes.size());

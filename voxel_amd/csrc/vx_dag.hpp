@@ -209,6 +209,9 @@ __host__ __device__ __forceinline__ uint64_t morton3(uint32_t x, uint32_t y, uin
 
 inline uint32_t grid_for(uint64_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(ceil_div(n, 256), 4096)); }
 
+// Smallest power of two >= v (slot counts).
+uint64_t pow2_at_least(uint64_t v);
+
 // A level's items: fixed stride; width from the header for node items.
 struct ItemSet {
   const uint32_t* items;
@@ -283,6 +286,10 @@ struct vx_dag {
   uint64_t* node_slots = nullptr;
   uint64_t* leaf_slots = nullptr;
   uint32_t* ctl = nullptr;
+  // Bumped whenever the scene's buffers move (vx_dag_grow): an executor that
+  // holds a copy of `dev` refreshes it, and re-captures its graph, when the
+  // generation it copied is older.
+  uint64_t generation = 0;
   // Optional per-stage and per-kernel timing of edits (hipEvents on the
   // edit's stream).
   bool profile = false;

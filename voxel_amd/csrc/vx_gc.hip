@@ -17,6 +17,8 @@
 // device pointer into the scene (captured graphs, batchers) stays valid, and
 // nothing of the scene changes before that copy: any failure leaves it as it
 // was.
+#include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "vx_dag.hpp"
@@ -307,6 +309,121 @@ extern "C" vx_status vx_dag_collect_garbage(vx_dag* g, vx_gc_report* report, voi
   rep.node_words_after = nu;
   rep.leaf_words_after = lu;
   if (report) *report = rep;
+  return VX_OK;
+}
+
+// ---- growth ----------------------------------------------------------------
+// Every live slot of a table (a pool offset, not empty, dead or tagged)
+// re-inserted into a larger table at its item's home position: the hash is
+// recomputed from the item in the pool (its fingerprint must equal the
+// slot's), the slot value itself is unchanged because the pool offsets are.
+// Linear probing from the home position keeps every slot reachable from it.
+__global__ void rehash_kernel(const uint64_t* __restrict__ old_slots, uint64_t n_old, const uint32_t* __restrict__ pool,
+                              uint64_t pool_used, uint32_t leaf_words, uint64_t* new_slots, uint64_t new_mask,
+                              uint32_t* status) {
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n_old; i += stride) {
+    const uint64_t sv = old_slots[i];
+    const uint32_t v = uint32_t(sv);
+    if (v < kPoolStart || (v & kTag)) continue;
+    if (v >= pool_used) {  // no item there: the header read below would be out of the used pool
+      atomicCAS(status, 0u, 1u);
+      continue;
+    }
+    const uint32_t w = leaf_words ? leaf_words : 1u + uint32_t(__popc(pool[v] & 0xffu));
+    if (uint64_t(v) + w > pool_used) {
+      atomicCAS(status, 0u, 1u);
+      continue;
+    }
+    const uint64_t h = hash_words(pool + v, w);
+    if (uint32_t(h >> 32) != uint32_t(sv >> 32)) {
+      atomicCAS(status, 0u, 1u);
+      continue;
+    }
+    uint64_t idx = h & new_mask;
+    bool placed = false;
+    for (uint64_t p = 0; p <= new_mask && p < kMaxProbe; ++p) {
+      const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&new_slots[idx]), 0ull,
+                                                static_cast<unsigned long long>(sv));
+      if (prev == 0ull) {
+        placed = true;
+        break;
+      }
+      idx = (idx + 1) & new_mask;
+    }
+    if (!placed) atomicCAS(status, 0u, 1u);
+  }
+}
+
+extern "C" vx_status vx_dag_grow(vx_dag* g, const vx_dag_capacity* cap, void* stream) {
+  if (!g || !cap) return VX_INVALID_VALUE;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // Targets: never below the current sizes; slot counts rounded up to powers of two.
+  vx_dag_capacity c = g->cap;
+  const uint64_t nw = std::max<uint64_t>(c.node_words, std::min<uint64_t>(cap->node_words, kTag));
+  const uint64_t lw = std::max<uint64_t>(c.leaf_words, std::min<uint64_t>(cap->leaf_words, kTag));
+  const uint64_t ns = std::max<uint64_t>(c.node_slots, cap->node_slots ? pow2_at_least(cap->node_slots) : 0);
+  const uint64_t ls = std::max<uint64_t>(c.leaf_slots, cap->leaf_slots ? pow2_at_least(cap->leaf_slots) : 0);
+  if (ns > (1ull << 32) || ls > (1ull << 32)) return VX_INVALID_VALUE;
+  const bool grow_np = nw > c.node_words, grow_lp = lw > c.leaf_words;
+  const bool grow_ns = ns > c.node_slots, grow_ls = ls > c.leaf_slots;
+  if (!grow_np && !grow_lp && !grow_ns && !grow_ls) return VX_OK;
+  if (hipStreamSynchronize(s) != hipSuccess) return VX_DEVICE_ERROR;
+  uint32_t ctl[kCtlWords];
+  if (hipMemcpy(ctl, g->ctl, sizeof(ctl), hipMemcpyDeviceToHost) != hipSuccess) return VX_DEVICE_ERROR;
+  const uint64_t nu = std::min<uint64_t>(ctl[kCtlNodeUsed], c.node_words);
+  const uint64_t lu = std::min<uint64_t>(ctl[kCtlLeafUsed], c.leaf_words);
+  // New buffers first; the scene is untouched until every step succeeded.
+  DevBuf npool, lpool, nslots, lslots, status;
+  vx_status st;
+  if (grow_np && (st = dev_alloc(npool, nw * 4)) != VX_OK) return st;
+  if (grow_lp && (st = dev_alloc(lpool, lw * 4)) != VX_OK) return st;
+  if (grow_ns && (st = dev_alloc(nslots, ns * 8)) != VX_OK) return st;
+  if (grow_ls && (st = dev_alloc(lslots, ls * 8)) != VX_OK) return st;
+  if ((st = dev_alloc(status, 4)) != VX_OK) return st;
+  if ((st = fill_u32(status.get<uint32_t>(), 1, 0, s)) != VX_OK) return st;
+  if (grow_np && hipMemcpyAsync(npool.p, g->node_pool, nu * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return VX_DEVICE_ERROR;
+  if (grow_lp && hipMemcpyAsync(lpool.p, g->leaf_pool, lu * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return VX_DEVICE_ERROR;
+  auto rehash = [&](DevBuf& to, uint64_t n_new, const uint64_t* from, uint64_t n_old, const uint32_t* pool,
+                    uint64_t used, uint32_t leaf_words) -> vx_status {
+    vx_status r = fill_u32(to.get<uint32_t>(), n_new * 2, 0, s);
+    if (r != VX_OK) return r;
+    const uint32_t blocks = uint32_t(std::min<uint64_t>(std::max<uint64_t>(1, (n_old + 255) / 256), 8192));
+    rehash_kernel<<<blocks, 256, 0, s>>>(from, n_old, pool, used, leaf_words, to.get<uint64_t>(), n_new - 1,
+                                         status.get<uint32_t>());
+    return last_launch();
+  };
+  if (grow_ns && (st = rehash(nslots, ns, g->node_slots, c.node_slots, g->node_pool, nu, 0)) != VX_OK) return st;
+  if (grow_ls && (st = rehash(lslots, ls, g->leaf_slots, c.leaf_slots, g->leaf_pool, lu, g->dev.leaf_words)) != VX_OK)
+    return st;
+  uint32_t bad = 0;
+  if (hipMemcpyAsync(&bad, status.p, 4, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return VX_DEVICE_ERROR;
+  if (bad) return VX_DEVICE_ERROR;  // a slot whose item does not hash to its fingerprint: the table is not intact
+  // Swap in the new buffers.
+  auto swap_in = [](DevBuf& b, auto*& field) {
+    using T = std::remove_reference_t<decltype(*field)>;
+    (void)hipFree(field);
+    field = b.get<T>();
+    b.p = nullptr;
+  };
+  if (grow_np) swap_in(npool, g->node_pool);
+  if (grow_lp) swap_in(lpool, g->leaf_pool);
+  if (grow_ns) swap_in(nslots, g->node_slots);
+  if (grow_ls) swap_in(lslots, g->leaf_slots);
+  g->cap = vx_dag_capacity{nw, lw, ns, ls};
+  Dev& d = g->dev;
+  d.node_pool = g->node_pool;
+  d.leaf_pool = g->leaf_pool;
+  d.node_slots = g->node_slots;
+  d.leaf_slots = g->leaf_slots;
+  d.node_mask = ns - 1;
+  d.leaf_mask = ls - 1;
+  d.node_cap = nw;
+  d.leaf_cap = lw;
+  ++g->generation;
   return VX_OK;
 }
 

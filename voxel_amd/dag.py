@@ -62,6 +62,7 @@ register({
     "vx_dag_export_canonical": (_i32, [_P, _P, _u64, C.POINTER(_u64)]),
     "vx_dag_export_canonical_device": (_i32, [_P, _P, _u64, C.POINTER(_u64), _P]),
     "vx_dag_collect_garbage": (_i32, [_P, _P, _P]),
+    "vx_dag_grow": (_i32, [_P, _P, _P]),
     "vx_dag_read_pools": (_i32, [_P, _P, _u64, C.POINTER(_u64), _P, _u64, C.POINTER(_u64)]),
     "vx_dag_check_unique": (_i32, [_P, C.POINTER(_u64), _P]),
     "vx_table_check_unique": (_i32, [_P, _u64, _P, _u64, _u32, C.POINTER(_u64), _P]),
@@ -275,6 +276,18 @@ class HashDagGpu:
         r = GcReport()
         check(lib().vx_dag_collect_garbage(self.h, C.byref(r), _stream(stream)), "vx_dag_collect_garbage")
         return r
+
+    def grow(self, node_words: int = 0, leaf_words: int = 0, node_slots: int = 0, leaf_slots: int = 0,
+             stream=None) -> None:
+        """Grow the pools and slot tables to at least these sizes (0, or a size
+        at or below the current one, keeps it); every ref stays valid
+        (vx_dag_grow)."""
+        cap = Capacity(node_words, leaf_words, node_slots, leaf_slots)
+        check(lib().vx_dag_grow(self.h, C.byref(cap), _stream(stream)), "vx_dag_grow")
+        st = self.stats()
+        word_cap = 1 << 31  # pool offsets stay below the tag bit
+        self._cap = Capacity(max(self._cap.node_words, min(node_words, word_cap)),
+                             max(self._cap.leaf_words, min(leaf_words, word_cap)), st.node_slots, st.leaf_slots)
 
     def read_pools(self) -> tuple[np.ndarray, np.ndarray]:
         """Host copies of the used words of the node and leaf pools."""

@@ -23,7 +23,10 @@ workspace, pinned staging buffer, stream and captured hipGraph):
   are marked rejected;
 - a request that does not fit its scene even with nothing in flight triggers
   a garbage collection of that scene (vx_dag_collect_garbage) after the
-  executors drain; if it still does not fit it is rejected.
+  executors drain; if it still does not fit, the scene's pools (and slot
+  tables, in proportion) grow to twice their size or to what the request
+  needs (vx_dag_grow), provided the device's free HBM holds the growth plus
+  a reserve; if it still does not fit it is rejected.
 
 Per scene the requests apply in arrival order, so every scene's final DAG
 equals replaying its requests one by one, and a request's effect is the same
@@ -101,7 +104,8 @@ class Server:
     each tick is one synchronous multi-scene call (vx_dag_apply_batches)."""
 
     def __init__(self, scenes: list[V.HashDagGpu], max_candidates: int | None = None, max_jobs: int = 8192,
-                 use_graph: bool = True, executors: int = 2, collect_on_full: bool = True):
+                 use_graph: bool = True, executors: int = 2, collect_on_full: bool = True,
+                 grow_on_full: bool = True, hbm_reserve: int = 1 << 30):
         if not 1 <= executors <= 2:
             raise ValueError("one or two executors")
         self.scenes = scenes
@@ -115,6 +119,9 @@ class Server:
         self.overlapped = 0  # ticks submitted while another tick was still in flight
         self.concurrent = 0  # ... of them, with no scene in common (no device wait)
         self.collections = 0
+        self.growths = 0
+        self.grow_on_full = grow_on_full
+        self.hbm_reserve = hbm_reserve  # device bytes a growth must leave free
         self.max_candidates = max_candidates
         self.max_jobs = max_jobs
         self.collect_on_full = collect_on_full
@@ -254,17 +261,54 @@ class Server:
         self.drain()
 
     def make_room(self, r: Request) -> bool:
-        """Garbage-collect r's scene (after the executors drain) and report
-        whether r fits it now."""
-        if not self.collect_on_full:
+        """After the executors drain: garbage-collect r's scene, then, if r
+        still does not fit, grow the scene within the free HBM; report whether
+        r fits it now."""
+        if not (self.collect_on_full or self.grow_on_full):
             return False
         self.drain()
         d = self.scenes[r.scene]
-        d.collect_garbage()
-        self.collections += 1
+        if self.collect_on_full:
+            d.collect_garbage()
+            self.collections += 1
+            st = d.stats()
+            self.used[r.scene] = [st.node_words_used, st.leaf_words_used]
+            if self.admit([r]):
+                return True
+        return self.grow_on_full and self._grow_for(r) and bool(self.admit([r]))
+
+    def _grow_for(self, r: Request) -> bool:
+        """Grow r's scene so that r's worst-case growth fits: each short pool
+        to twice its size or to what r needs, the slot tables in the same
+        proportion; only if the device's free HBM keeps `hbm_reserve` after."""
+        import torch
+        d = self.scenes[r.scene]
+        nw, lw = growth_bound(d, r.brushes)
+        u = self.used[r.scene]
+        cap_n, cap_l = self.caps[r.scene]
+        word_cap = 1 << 31
+        want_n = cap_n if u[0] + nw <= cap_n else min(word_cap, max(2 * cap_n, u[0] + nw))
+        want_l = cap_l if u[1] + lw <= cap_l else min(word_cap, max(2 * cap_l, u[1] + lw))
+        if (want_n, want_l) == (cap_n, cap_l) or u[0] + nw > want_n or u[1] + lw > want_l:
+            return False
         st = d.stats()
-        self.used[r.scene] = [st.node_words_used, st.leaf_words_used]
-        return bool(self.admit([r]))
+        slots_n = pow2_at_least(-(-st.node_slots * want_n // cap_n))
+        slots_l = pow2_at_least(-(-st.leaf_slots * want_l // cap_l))
+        need = 4 * (want_n - cap_n + want_l - cap_l) + 8 * (slots_n - st.node_slots + slots_l - st.leaf_slots)
+        free, _ = torch.cuda.mem_get_info()
+        if free < need + self.hbm_reserve:
+            return False
+        d.grow(want_n, want_l, slots_n, slots_l)
+        self.caps[r.scene] = (d._cap.node_words, d._cap.leaf_words)
+        self.growths += 1
+        return True
+
+
+def pow2_at_least(v: int) -> int:
+    p = 1
+    while p < v:
+        p <<= 1
+    return p
 
 
 def make_requests(n: int, n_scenes: int, worlds: list[int], seed: int, lo_log2: int = 6, hi_log2: int = 18,
@@ -347,4 +391,5 @@ def serve(server: Server, reqs: list[Request], arrivals, max_tick: int = 1 << 30
         "overlapped_ticks": server.overlapped,
         "concurrent_ticks": server.concurrent,
         "collections": server.collections,
+        "growths": server.growths,
     }
