@@ -1,7 +1,12 @@
-"""The batching and serving workloads at their full sizes, bit-exact against
-the CPU oracle: every scene's canonical export (computed on the GPU) has the
-FNV-1a digest the oracle recorded for the same terrain and batches
-(tests/golden/workloads.json, written by tests/golden/make_workload_golden.py).
+"""The plain-edit, batching and serving workloads at their full sizes,
+bit-exact against the CPU oracle: every scene's canonical export (computed on
+the GPU) has the FNV-1a digest the oracle recorded for the same terrain and
+batches (tests/golden/workloads.json, written by
+tests/golden/make_workload_golden.py).
+
+plain: the 16384^3 materials heightfield (inner depth 12, seed 3) after each
+of the benchmark's first two batches of 4096 brushes, edited by the plain
+per-batch entry point with the scene sizes the benchmark uses.
 
 batching: 8 materials heightfields of inner depth 8..12 (1024^3..16384^3,
 seeds 10..17), 2 x 256 batches of 2^8..2^20 voxel ops (1..16 brushes each),
@@ -48,6 +53,24 @@ def _check(scenes, golden_scenes, what):
         got = {"words": int(e.size), "nodes": int(e[4]), "leaves": int(e[5]), "digest": "%016x" % O.digest(e)}
         exp = {k: want[k] for k in got}
         assert got == exp, f"{what} scene {s} (inner depth {want['depth']})"
+
+
+@pytest.mark.timeout(300)
+def test_plain_edit_workload_matches_oracle(gpu):
+    from voxel_amd import dag as V
+    G = _golden()["plain"]
+    depth, world = G["depth"], 4 << G["depth"]
+    d = V.HashDagGpu(depth, True, node_words=1 << 30, leaf_words=1 << 30, node_slots=1 << 27, leaf_slots=1 << 27)
+    d.build_terrain(V.terrain_for_world(world, G["terrain_seed"]))
+    for want in G["digests"]:
+        k = want["after_batch"]
+        d.apply_brushes(V.gen_brushes(G["brush_seed"], V.STREAM_BRUSH + (k << 8), G["brushes_per_batch"], world,
+                                      G["mix"]))
+        e = d.export_device().cpu().numpy().view(np.uint32)
+        got = {"words": int(e.size), "nodes": int(e[4]), "leaves": int(e[5]), "digest": "%016x" % O.digest(e)}
+        del e
+        assert got == {key: want[key] for key in got}, f"after batch {k}"
+    d.close()
 
 
 @pytest.mark.timeout(600)
