@@ -270,7 +270,10 @@ struct CommitPayload {
 };
 
 // Position t of the scan is entry list[t] (entry t without a list); a list
-// is in entry order, so scene segments stay contiguous.
+// is in entry order, so scene segments stay contiguous. A split load: the
+// reads of all of a thread's items go out together, round by round (the
+// entries; their results, fingerprints, node headers and scene ids; the
+// scenes' pool words in use), none behind a branch on a loaded value.
 struct CommitLoad {
   using Payload = CommitPayload;
   const uint64_t* res;
@@ -280,19 +283,73 @@ struct CommitLoad {
   ItemSet set;
   Scenes sc;
   bool leaves;
-  __device__ uint32_t entry(uint32_t t) const { return list ? list[t] : t; }
-  __device__ uint64_t operator()(uint32_t t, CommitPayload& p) const {
-    const uint32_t i = entry(t);
-    p.i = i;
-    p.r = res[i];
-    const bool rep = res_kind(p.r) == kResRep;
-    p.fp = rep ? uint32_t(hashes[i] >> 32) : 0u;
-    const uint32_t w = rep ? set.width(set.items + size_t(i) * set.stride) : 0u;
-    const uint32_t id = sc.id(i);
-    p.base = *table_of(sc.of(i), leaves).base;
-    p.last = t == *n_ptr - 1 || sc.id(entry(t + 1)) != id;
-    const bool head = t == 0 || sc.id(entry(t - 1)) != id;
-    return scan::SegSumOp::make(w, head);
+  struct Raw {
+    uint64_t r;
+    uint32_t i, fp, w, base;
+    bool head, last;
+  };
+  template <uint32_t N>
+  __device__ void fetch(const uint32_t (&t)[N], uint32_t n, Raw (&x)[N]) const {
+    uint32_t ip[N], in[N], hdr[N];
+    if (list) {
+#pragma unroll
+      for (uint32_t k = 0; k < N; ++k) {
+        x[k].i = gld(list + t[k]);
+        ip[k] = gld(list + (t[k] ? t[k] - 1 : 0u));
+        in[k] = gld(list + min(t[k] + 1, n - 1));
+      }
+    } else {
+#pragma unroll
+      for (uint32_t k = 0; k < N; ++k) {
+        x[k].i = t[k];
+        ip[k] = t[k] ? t[k] - 1 : 0u;
+        in[k] = min(t[k] + 1, n - 1);
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) {
+      x[k].r = gld(res + x[k].i);
+      x[k].fp = gld(reinterpret_cast<const uint32_t*>(hashes + x[k].i) + 1);  // the hash's high word
+      hdr[k] = 0;
+    }
+    if (!set.fixed_width) {
+#pragma unroll
+      for (uint32_t k = 0; k < N; ++k) hdr[k] = gld(set.items + size_t(x[k].i) * set.stride);
+    }
+    if (sc.table) {
+      uint32_t id[N];
+#pragma unroll
+      for (uint32_t k = 0; k < N; ++k) {  // every load unconditional (no short-circuit)
+        id[k] = sc.codec.scene(gld(sc.keys + x[k].i));
+        const uint32_t before = sc.codec.scene(gld(sc.keys + ip[k])), after = sc.codec.scene(gld(sc.keys + in[k]));
+        x[k].head = (t[k] == 0) | (before != id[k]);
+        x[k].last = (t[k] == n - 1) | (after != id[k]);
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < N; ++k) x[k].base = gld(gld(&sc.table[id[k]].base) + (leaves ? 1 : 0));
+    } else {
+      const uint32_t b = gld(table_of(sc.single, leaves).base);
+#pragma unroll
+      for (uint32_t k = 0; k < N; ++k) {
+        x[k].head = t[k] == 0;
+        x[k].last = t[k] == n - 1;
+        x[k].base = b;
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) {
+      const bool rep = res_kind(x[k].r) == kResRep;
+      x[k].w = rep ? (set.fixed_width ? set.fixed_width : 1u + __popc(hdr[k] & 0xffu)) : 0u;
+      if (!rep) x[k].fp = 0;
+    }
+  }
+  __device__ uint64_t finish(uint32_t, const Raw& x, CommitPayload& p) const {
+    p.i = x.i;
+    p.r = x.r;
+    p.fp = x.fp;
+    p.base = x.base;
+    p.last = x.last;
+    return scan::SegSumOp::make(x.w, x.head);
   }
 };
 
@@ -428,6 +485,7 @@ struct alignas(16) BrushJob {
   uint32_t pad;
 };
 static_assert(sizeof(BrushJob) == 80, "BrushJob layout");
+static_assert(offsetof(BrushJob, bx0) == 32 && offsetof(BrushJob, scene) == 60, "BrushJob words read by the records");
 
 // a / d and a % d for a < 2^22 and d >= 1 from inv = float(1/d): a is exact
 // in float and the product carries a relative error below 2^-23, so it is
@@ -722,13 +780,18 @@ struct RowLoad {
   }
 };
 
-// The touched rows, compacted: record start, job, brick y|z, first brick.
+// Records per chunk of the record kernel (a multiple of the wave width).
+constexpr uint32_t kRecChunk = 1024;
+
+// The touched rows, compacted: record start, job, brick y|z, first brick;
+// and, per chunk of kRecChunk records, the row that holds its first record.
 struct TouchedRows {
   uint32_t* rec;
   uint32_t* job;
   uint32_t* yz;
   uint32_t* x0;
   uint32_t* n_rows;
+  uint32_t* chunk_row;
 };
 
 struct RowStore {
@@ -739,10 +802,14 @@ struct RowStore {
     using PS = scan::PairSumOp;
     if (PS::lo(v)) {
       const uint32_t k = PS::lo(excl);
-      rows.rec[k] = PS::hi(excl);
+      const uint64_t e = PS::hi(excl), len = PS::hi(v);
+      rows.rec[k] = uint32_t(e);
       rows.job[k] = p.job;
       rows.yz[k] = p.yz;
       rows.x0[k] = p.x0;
+      // The chunk starts among this row's records [e, e + len).
+      for (uint64_t ch = (e + kRecChunk - 1) / kRecChunk; ch * kRecChunk < e + len; ++ch)
+        rows.chunk_row[ch] = k;
     }
     if (r == n - 1) {
       *rows.n_rows = PS::lo(incl);
@@ -751,51 +818,88 @@ struct RowStore {
   }
 };
 
-// One thread per record. A wave takes a contiguous chunk of records; 64
-// consecutive records span at most 64 rows (no row is empty), so each step
-// loads the 64 row starts from the wave's current row in one coalesced load
-// and every lane finds its row among them (a 6-step search over the lanes).
+// What a record needs of its row: the brick x of record 0 (gx = gx0 + i), the
+// row's brick y and z, the scene and the brush.
+struct RowFields {
+  uint32_t gx0, gy, gz, scene;
+  vx_brush b;
+};
+
+__device__ __forceinline__ RowFields shfl_row(const RowFields& f, uint32_t k) {
+  auto sh = [k](uint32_t v) { return uint32_t(__shfl(v, int(k), kWave)); };
+  auto shi = [k](int32_t v) { return int32_t(__shfl(v, int(k), kWave)); };
+  RowFields o;
+  o.gx0 = sh(f.gx0);
+  o.gy = sh(f.gy);
+  o.gz = sh(f.gz);
+  o.scene = sh(f.scene);
+  o.b.cx = shi(f.b.cx);
+  o.b.cy = shi(f.b.cy);
+  o.b.cz = shi(f.b.cz);
+  o.b.hx = shi(f.b.hx);
+  o.b.hy = shi(f.b.hy);
+  o.b.hz = shi(f.b.hz);
+  o.b.kind = sh(f.b.kind);
+  o.b.material = sh(f.b.material);
+  return o;
+}
+
+// One thread per record. A wave takes chunks of kRecChunk consecutive records
+// (grid stride); the row scan noted the row holding each chunk's first record.
+// The wave holds a window of 64 consecutive rows in its lanes (lane l: row
+// w + l, its first record and its fields, its brush's read with it) and walks
+// the window's records in steps of at most 64: each lane finds its record's
+// row among the lanes (a 6-step search; the first records ascend over the
+// lanes) and takes that lane's fields. The memory round trips (rows, then
+// brushes) come once per window, not once per step; the next window starts
+// at the record where this one ends.
 __global__ void brush_records_kernel(BrushCands c, TouchedRows rows, RecordOut out) {
   const uint32_t n = *out.n_rec, nr = *rows.n_rows;
   const uint32_t lane = lane_id();
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
-  const uint32_t w = blockIdx.x * (blockDim.x / kWave) + wave_id();
-  const uint32_t chunk = (n + waves * kWave - 1) / (waves * kWave) * kWave;
-  const uint32_t r0 = w * chunk, r1 = min(n, r0 + chunk);
-  if (r0 >= r1) return;
-  // The row of the chunk's first record: the last row starting at or before it.
-  uint32_t row = 0;
-  {
-    uint32_t lo = 0, hi = nr;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (rows.rec[mid] <= r0) lo = mid;
-      else hi = mid;
-    }
-    row = lo;
-  }
-  for (uint32_t i0 = r0; i0 < r1; i0 += kWave) {
-    const uint32_t i = i0 + lane;
-    const uint32_t start = row + lane < nr ? rows.rec[row + lane] : 0xffffffffu;  // ascending over the lanes
-    // k = the number of lanes whose row starts at or before record i, minus one.
-    uint32_t k = 0;
+  const uint32_t chunks = (n + kRecChunk - 1) / kRecChunk;
+  for (uint32_t ch = blockIdx.x * (blockDim.x / kWave) + wave_id(); ch < chunks; ch += waves) {
+    const uint32_t r1 = uint32_t(min<uint64_t>(n, uint64_t(ch + 1) * kRecChunk));
+    uint32_t i0 = ch * kRecChunk;
+    uint32_t wrow = rows.chunk_row[ch];
+    while (i0 < r1) {
+      const uint32_t rr = wrow + lane;
+      const bool have = rr < nr;
+      const uint32_t start = have ? rows.rec[rr] : 0xffffffffu;  // ascending over the lanes
+      const uint32_t job = have ? rows.job[rr] : 0u;
+      const uint32_t yz = have ? rows.yz[rr] : 0u;
+      const uint32_t x0 = have ? rows.x0[rr] : 0u;
+      const uint32_t wend = wrow + kWave < nr ? rows.rec[wrow + kWave] : n;  // the window's end
+      const uint32_t* jp = reinterpret_cast<const uint32_t*>(c.jobs + job);
+      const uint4 q0 = gld4(jp), q1 = gld4(jp + 4), q2 = gld4(jp + 8), q3 = gld4(jp + 12);
+      RowFields f;
+      f.b = vx_brush{int32_t(q0.x), int32_t(q0.y), int32_t(q0.z), int32_t(q0.w),
+                     int32_t(q1.x), int32_t(q1.y), q1.z, q1.w};
+      f.gx0 = q2.x + x0 - start;  // BrushJob: bx0, by0, bz0 at words 8..10, scene at word 15
+      f.gy = q2.y + (yz & 0xffffu);
+      f.gz = q2.z + (yz >> 16);
+      f.scene = q3.w;
+      const uint32_t stop = min(wend, r1);
+      for (uint32_t s0 = i0; s0 < stop; s0 += kWave) {
+        const uint32_t i = s0 + lane;
+        // k = the number of lanes whose row starts at or before record i, minus one.
+        uint32_t k = 0;
 #pragma unroll
-    for (uint32_t step = 32; step >= 1; step >>= 1) {
-      const uint32_t probe = uint32_t(__shfl(start, int(k + step), kWave));
-      if (k + step < kWave && probe <= i) k += step;
+        for (uint32_t step = 32; step >= 1; step >>= 1) {
+          const uint32_t probe = uint32_t(__shfl(start, int(k + step), kWave));
+          if (k + step < kWave && probe <= i) k += step;
+        }
+        const RowFields g = shfl_row(f, k);
+        if (i < stop) {
+          const uint32_t gx = g.gx0 + i;
+          out.put_key(i, (uint64_t(g.scene) << c.codec.shift) | morton3(gx, g.gy, g.gz));
+          out.put_payload(i, brick_mask(g.b, int32_t(gx * 4), int32_t(g.gy * 4), int32_t(g.gz * 4)),
+                          c.materials ? g.b.material : (g.b.material ? 1u : 0u));
+        }
+      }
+      i0 = max(i0, stop);  // [i0, stop) done
+      wrow += kWave;
     }
-    const uint32_t my_row = row + k;
-    const uint32_t my_start = uint32_t(__shfl(start, int(k), kWave));
-    if (i < r1) {
-      const BrushJob j = c.jobs[rows.job[my_row]];
-      const uint32_t yz = rows.yz[my_row];
-      const uint32_t gx = j.bx0 + rows.x0[my_row] + (i - my_start);
-      const uint32_t gy = j.by0 + (yz & 0xffffu), gz = j.bz0 + (yz >> 16);
-      out.put_key(i, (uint64_t(j.scene) << c.codec.shift) | morton3(gx, gy, gz));
-      out.put_payload(i, brick_mask(j.b, int32_t(gx * 4), int32_t(gy * 4), int32_t(gz * 4)),
-                      c.materials ? j.b.material : (j.b.material ? 1u : 0u));
-    }
-    row += uint32_t(__builtin_amdgcn_readlane(int(k), kWave - 1));
   }
 }
 
@@ -838,16 +942,44 @@ struct RunUpPayload {
   bool inc;
 };
 
+// The depth of the scene of each key (reads of the scene table, all issued
+// together, or the single scene's).
+template <uint32_t N, class Raw>
+__device__ __forceinline__ void fetch_depths(const Scenes& sc, Raw (&r)[N]) {
+  if (sc.table) {
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) r[k].depth = gld(&sc.table[sc.codec.scene(r[k].key)].depth);
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) r[k].depth = sc.single.depth;
+  }
+}
+
+// Split load (the reads of all of a thread's items first): the record's key
+// and its predecessor's (index clamped, not branched on), then the scenes'
+// depths.
 struct RunUpLoad {
   using Payload = RunUpPayload;
   SortedRecs recs;
   Scenes sc;  // keys unused: scene from the record key
-  __device__ uint64_t operator()(uint32_t i, RunUpPayload& p) const {
-    p.key = recs.key(i);
-    const uint64_t prev = i ? recs.key(i - 1) : 0;
-    p.leaf = i == 0 || prev != p.key;
-    p.inc = sc.at(sc.codec.scene(p.key)).depth > 0;
-    const bool par = p.leaf && p.inc && (i == 0 || sc.codec.parent(prev) != sc.codec.parent(p.key));
+  struct Raw {
+    uint64_t key, prev;
+    uint32_t depth;
+  };
+  template <uint32_t N>
+  __device__ void fetch(const uint32_t (&i)[N], uint32_t, Raw (&r)[N]) const {
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) {
+      r[k].key = recs.key(i[k]);
+      r[k].prev = recs.key(i[k] ? i[k] - 1 : 0u);
+    }
+    fetch_depths<N>(sc, r);
+  }
+  __device__ uint64_t finish(uint32_t i, const Raw& r, RunUpPayload& p) const {
+    p.key = r.key;
+    p.leaf = i == 0 || r.prev != p.key;
+    p.inc = r.depth > 0;
+    const bool par = p.leaf && p.inc && (i == 0 || sc.codec.parent(r.prev) != sc.codec.parent(p.key));
     return scan::PairSumOp::make(p.leaf, par);
   }
 };
@@ -893,17 +1025,30 @@ struct UpPayload {
   bool inc;
 };
 
-struct UpLoad {
+struct UpLoad {  // split load, as RunUpLoad
   using Payload = UpPayload;
   const uint64_t* keys;
   Scenes sc;
   uint32_t h;
-  __device__ uint32_t operator()(uint32_t i, UpPayload& p) const {
-    p.key = keys[i];
+  struct Raw {
+    uint64_t key, prev;
+    uint32_t depth;
+  };
+  template <uint32_t N>
+  __device__ void fetch(const uint32_t (&i)[N], uint32_t, Raw (&r)[N]) const {
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) {
+      r[k].key = gld(keys + i[k]);
+      r[k].prev = gld(keys + (i[k] ? i[k] - 1 : 0u));
+    }
+    fetch_depths<N>(sc, r);
+  }
+  __device__ uint32_t finish(uint32_t i, const Raw& r, UpPayload& p) const {
+    p.key = r.key;
     p.scene = sc.codec.scene(p.key);
-    p.inc = sc.at(p.scene).depth > h;
+    p.inc = r.depth > h;
     if (!p.inc) return 0u;
-    return i == 0 || sc.codec.parent(keys[i - 1]) != sc.codec.parent(p.key);
+    return i == 0 || sc.codec.parent(r.prev) != sc.codec.parent(p.key);
   }
 };
 
@@ -1317,6 +1462,7 @@ struct EditWs {
   void* commit_ws[kMaxHeights + 1];
   uint32_t *pend, *pend_list, *slice_count, *slice_base;  // pending leaves (LeafSlices)
   uint32_t *row_rec, *row_job, *row_yz, *row_x0;          // touched brick rows (TouchedRows)
+  uint32_t* chunk_row;                                     // per kRecChunk records (TouchedRows)
   void* slice_ws;
   BrushJob* jobs;
   Dev* scenes;
@@ -1595,6 +1741,7 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
   w.row_job = c.take<uint32_t>(rows_ub);
   w.row_yz = c.take<uint32_t>(rows_ub);
   w.row_x0 = c.take<uint32_t>(rows_ub);
+  w.chunk_row = c.take<uint32_t>(R / kRecChunk + 2);
   w.slice_count = c.take<uint32_t>(slices);
   w.slice_base = c.take<uint32_t>(slices);
   w.hashes = c.take<uint64_t>(max_cap);
@@ -1930,11 +2077,12 @@ vx_status device_pipeline(vx_dag* prof_dag, const Plan& p, const Scenes& sc, Edi
     if constexpr (std::is_same_v<Load, BrushLoad>) {
       // Brushes: a scan over the brick rows (counts[kCntCandidates] of them),
       // then one thread per touched brick.
-      const TouchedRows rows{w.row_rec, w.row_job, w.row_yz, w.row_x0, &w.counts[kCntRows]};
+      const TouchedRows rows{w.row_rec, w.row_job, w.row_yz, w.row_x0, &w.counts[kCntRows], w.chunk_row};
       st = scan::fused_dev_m<scan::PairSumOp>(&w.counts[kCntCandidates], cand_cap, RowLoad{load.c},
                                               RowStore{rows, &w.counts[0]}, w.rec_ws, s, false);
       if (st == VX_OK) {
-        brush_records_kernel<<<grid_for(cand_cap), 256, 0, s>>>(load.c, rows, record_out(w));
+        // About one wave per chunk of records (cand_cap bounds the records).
+        brush_records_kernel<<<grid_for(cand_cap / (kRecChunk / kWave)), 256, 0, s>>>(load.c, rows, record_out(w));
         st = last_launch();
       }
     } else {

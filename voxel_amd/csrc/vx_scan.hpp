@@ -147,6 +147,15 @@ __device__ typename M::T lookback_window(const uint64_t* rec, uint32_t tile) {
   return excl;
 }
 
+// A Load may split its reads from its arithmetic: `Raw`, a
+// `fetch<N>(idx, n, raw)` that reads the N items of a thread (indices clamped
+// to the count n, so it reads without guards, round by round for all items
+// at once), and `finish(i, raw, payload)` that does the rest without memory
+// reads. A load written as one call per item waits for each item's reads
+// before the next item's are issued whenever the call branches.
+template <class L>
+concept SplitLoad = requires { typename L::Raw; };
+
 // One tile: kFusedBlock threads x ITEMS elements, wave-segment order
 // (lane-contiguous per round, so every round of loads is coalesced). Grid
 // kernels use kFusedItems; a single-workgroup pass over at most kFusedBlock
@@ -163,10 +172,29 @@ __device__ __forceinline__ void fused_tile(uint32_t tile, uint32_t n, bool multi
   P pl[ITEMS];
   T carry = M::identity();
   Load lc = ld;  // per-thread copy: a Load may keep state from call to call
+  // Every item's loads first, then the scans: a wave scan between two items'
+  // loads would make each item wait for its own round trip.
+  if constexpr (SplitLoad<Load>) {
+    uint32_t idx[ITEMS];
+    typename Load::Raw raw[ITEMS];
+    const uint32_t last = n ? n - 1 : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < ITEMS; ++k) idx[k] = min(seg + k * kWave + lane, last);
+    if (n) lc.template fetch<ITEMS>(idx, n, raw);
+#pragma unroll
+    for (uint32_t k = 0; k < ITEMS; ++k) {
+      const uint32_t i = seg + k * kWave + lane;
+      v[k] = i < n ? T(lc.finish(i, raw[k], pl[k])) : M::identity();
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < ITEMS; ++k) {
+      const uint32_t i = seg + k * kWave + lane;
+      v[k] = i < n ? T(lc(i, pl[k])) : M::identity();
+    }
+  }
 #pragma unroll
   for (uint32_t k = 0; k < ITEMS; ++k) {
-    const uint32_t i = seg + k * kWave + lane;
-    v[k] = i < n ? T(lc(i, pl[k])) : M::identity();
     const T inc = wave_inclusive<M>(v[k]);
     ex[k] = M::combine(carry, wave_shift_up1(inc));
     carry = M::combine(carry, wave_last(inc));
