@@ -89,3 +89,20 @@ def test_full_size_sort_check_rejects_planted_mistakes():
     v = good_v.clone()
     v[1] = v[0]
     assert not bench.sorted_pairs_ok(torch, tk, good_k, v, unsigned32=True)["ok"]
+
+
+def test_brick_masks_match_voxel_by_voxel_evaluation():
+    """The brick voxel masks of the record kernel (vx_brick_mask.hpp, built for
+    the host) against the brush definitions evaluated voxel by voxel in 64-bit
+    arithmetic: exhaustive for small extents, seeded samples over each form's
+    whole range (tests/csrc/brick_mask_check.cpp)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "tests", "_build", "brick_mask_check")
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    subprocess.check_call(["/opt/rocm/llvm/bin/clang++", "-O2", "-std=c++20", "-D__HIP_PLATFORM_AMD__",
+                           "-I/opt/rocm/include", "-I" + os.path.join(root, "include"),
+                           "-I" + os.path.join(root, "voxel_amd", "csrc"),
+                           os.path.join(root, "tests", "csrc", "brick_mask_check.cpp"), "-o", exe])
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "brick masks: ok" in r.stdout, r.stdout[-3000:]

@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "vx_dag.hpp"
+#include "vx_brick_mask.hpp"
 #include "vx_scan.hpp"
 #include "vx_sort.hpp"
 
@@ -515,134 +516,6 @@ struct Record {
   uint32_t mat;
 };
 
-// Sphere voxels of one brick, row by row: voxel (lx, ly, lz) is inside iff
-// dx^2 <= r^2 - dy^2 - dz^2. Every |d| <= r + 3 for a brick inside the
-// brush's bounding box, so for r < 2^14 all sums stay below 2^30 in int32.
-__device__ __forceinline__ uint64_t sphere_mask32(int32_t dx0, int32_t dy0, int32_t dz0, int32_t r2) {
-  int32_t qx[4], qy[4], qz[4];
-#pragma unroll
-  for (int l = 0; l < 4; ++l) {
-    qx[l] = (dx0 + l) * (dx0 + l);
-    qy[l] = (dy0 + l) * (dy0 + l);
-    qz[l] = (dz0 + l) * (dz0 + l);
-  }
-  // Outside iff rem - dx^2 < 0: collect sign bits (plain ALU work, no
-  // compare-to-mask round trips), then invert.
-  uint32_t out[2] = {0u, 0u};
-#pragma unroll
-  for (int lz = 0; lz < 4; ++lz)
-#pragma unroll
-    for (int ly = 0; ly < 4; ++ly) {
-      const int32_t rem = r2 - qy[ly] - qz[lz];
-#pragma unroll
-      for (int lx = 0; lx < 4; ++lx)
-        out[lz >> 1] |= (uint32_t(rem - qx[lx]) >> 31) << (lx + 4 * ly + 16 * (lz & 1));
-    }
-  return ~((uint64_t(out[1]) << 32) | out[0]);
-}
-
-// Same test on packed 16-bit lanes, for radii up to kSphere16MaxR: every
-// |d| <= r + 3 = 103, so r^2 - dx^2 - dy^2 - dz^2 lies in [-21827, 10000] and
-// fits int16. One packed subtract tests a voxel and its neighbour in z (lz,
-// lz + 1); its two sign bits (outside) land 16 bits apart, which is exactly
-// the distance between the two voxels' mask bits (bit lx + 4 ly + 16 lz), so
-// an arithmetic shift and one and-or place both. 3 ALU operations per voxel
-// pair instead of 3-4 per voxel.
-constexpr int32_t kSphere16MaxR = 100;
-typedef short v2i16 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ uint64_t sphere_mask16(int32_t dx0, int32_t dy0, int32_t dz0, int32_t r2) {
-  v2i16 t[2];  // (r^2 - dz^2) of lz = 2P (low half) and 2P + 1 (high half)
-#pragma unroll
-  for (int P = 0; P < 2; ++P) {
-    const int32_t z0 = dz0 + 2 * P, z1 = z0 + 1;
-    t[P] = v2i16{short(r2 - z0 * z0), short(r2 - z1 * z1)};
-  }
-  v2i16 qx[4], qy[4];
-#pragma unroll
-  for (int l = 0; l < 4; ++l) {
-    const short x = short((dx0 + l) * (dx0 + l)), y = short((dy0 + l) * (dy0 + l));
-    qx[l] = v2i16{x, x};
-    qy[l] = v2i16{y, y};
-  }
-  uint32_t out[2] = {0u, 0u};
-#pragma unroll
-  for (int P = 0; P < 2; ++P)
-#pragma unroll
-    for (int ly = 0; ly < 4; ++ly) {
-      const v2i16 ty = t[P] - qy[ly];
-#pragma unroll
-      for (int lx = 0; lx < 4; ++lx) {
-        const v2i16 sgn = (ty - qx[lx]) >> short(15);  // all ones where outside
-        const uint32_t bit = 1u << (lx + 4 * ly);
-        out[P] |= __builtin_bit_cast(uint32_t, sgn) & (bit | (bit << 16));
-      }
-    }
-  return ~((uint64_t(out[1]) << 32) | out[0]);
-}
-
-// Same test in 64-bit arithmetic, for radii of 2^14 and more (out of line:
-// it is rare and would otherwise cost every caller its registers).
-__device__ __noinline__ uint64_t sphere_mask64(int64_t dx0, int64_t dy0, int64_t dz0, int64_t r2) {
-  uint64_t m = 0;
-  for (int lz = 0; lz < 4; ++lz)
-    for (int ly = 0; ly < 4; ++ly) {
-      const int64_t rem = r2 - (dy0 + ly) * (dy0 + ly) - (dz0 + lz) * (dz0 + lz);
-      for (int lx = 0; lx < 4; ++lx)
-        if ((dx0 + lx) * (dx0 + lx) <= rem) m |= 1ull << (lx + 4 * ly + 16 * lz);
-    }
-  return m;
-}
-
-// Box voxels of one brick in 64-bit arithmetic, for half extents of 2^30 and
-// more (out of line, like sphere_mask64).
-// (Arguments by value: a reference would force the caller's brush copy into
-// scratch memory.)
-__device__ __noinline__ uint64_t box_mask64(int64_t dx, int64_t dy, int64_t dz, int64_t hx, int64_t hy, int64_t hz) {
-  uint32_t xm = 0, ym = 0, zm = 0;
-  for (int l = 0; l < 4; ++l) {
-    xm |= uint32_t(llabs(dx + l) <= hx) << l;
-    ym |= uint32_t(llabs(dy + l) <= hy) << l;
-    zm |= uint32_t(llabs(dz + l) <= hz) << l;
-  }
-  uint64_t m = 0;
-  for (int lz = 0; lz < 4; ++lz)
-    for (int ly = 0; ly < 4; ++ly)
-      if (((zm >> lz) & 1u) && ((ym >> ly) & 1u)) m |= uint64_t(xm) << (4 * ly + 16 * lz);
-  return m;
-}
-
-// Voxel mask of brick (X, Y, Z) (its minimum corner) under brush b; bit
-// lx + 4 ly + 16 lz (the reference's leaf bit order, detail/node.cuh:90-94).
-// The brick lies inside the brush's clipped bounding box, so every offset
-// from the centre is at most the half extent + 3: 32-bit arithmetic is exact
-// below the stated extents.
-__device__ __forceinline__ uint64_t brick_mask(const vx_brush& b, int32_t X, int32_t Y, int32_t Z) {
-  if (b.kind == 0) {
-    if (b.hx <= kSphere16MaxR) return sphere_mask16(X - b.cx, Y - b.cy, Z - b.cz, b.hx * b.hx);
-    if (b.hx < (1 << 14)) return sphere_mask32(X - b.cx, Y - b.cy, Z - b.cz, b.hx * b.hx);
-    return sphere_mask64(int64_t(X) - b.cx, int64_t(Y) - b.cy, int64_t(Z) - b.cz, int64_t(b.hx) * b.hx);
-  }
-  if (b.hx >= (1 << 30) || b.hy >= (1 << 30) || b.hz >= (1 << 30))
-    return box_mask64(int64_t(X) - b.cx, int64_t(Y) - b.cy, int64_t(Z) - b.cz, b.hx, b.hy, b.hz);
-  const int32_t dx = X - b.cx, dy = Y - b.cy, dz = Z - b.cz;
-  uint32_t xm = 0, ym = 0, zm = 0;
-#pragma unroll
-  for (int l = 0; l < 4; ++l) {
-    xm |= uint32_t(abs(dx + l) <= b.hx) << l;
-    ym |= uint32_t(abs(dy + l) <= b.hy) << l;
-    zm |= uint32_t(abs(dz + l) <= b.hz) << l;
-  }
-  // Spread the z and y bits to one nibble per (ly, lz) row, then replicate
-  // the x row into every selected nibble.
-  uint32_t rows = 0;
-#pragma unroll
-  for (int lz = 0; lz < 4; ++lz) rows |= (((zm >> lz) & 1u) * ym) << (4 * lz);
-  uint64_t m = 0;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) m |= uint64_t(((rows >> r) & 1u) * xm) << (4 * r);
-  return m;
-}
 
 // Record outputs. With index_bits > 0 the sort key is packed above the
 // record index (key << index_bits | index) and sorted by its upper bits, so
@@ -818,42 +691,34 @@ struct RowStore {
   }
 };
 
-// What a record needs of its row: the brick x of record 0 (gx = gx0 + i), the
-// row's brick y and z, the scene and the brush.
-struct RowFields {
-  uint32_t gx0, gy, gz, scene;
-  vx_brush b;
+// What a record needs of its row, kept in LDS for the 64 rows of a window
+// (four 16-byte words per row): the brick x of the batch's record 0 (record
+// i has gx = gx0 + i), the row's brick y and z, the key bits of its scene, y
+// and z (key = those | spread3(gx)), and the brush.
+struct alignas(16) RowFields {
+  uint32_t gx0, gy, gz, key_lo;
+  uint32_t key_hi;
+  int32_t cx, cy, cz;
+  int32_t hx, hy, hz;
+  uint32_t kind;
+  uint32_t material, pad[3];
 };
+static_assert(sizeof(RowFields) == 64, "RowFields layout");
 
-__device__ __forceinline__ RowFields shfl_row(const RowFields& f, uint32_t k) {
-  auto sh = [k](uint32_t v) { return uint32_t(__shfl(v, int(k), kWave)); };
-  auto shi = [k](int32_t v) { return int32_t(__shfl(v, int(k), kWave)); };
-  RowFields o;
-  o.gx0 = sh(f.gx0);
-  o.gy = sh(f.gy);
-  o.gz = sh(f.gz);
-  o.scene = sh(f.scene);
-  o.b.cx = shi(f.b.cx);
-  o.b.cy = shi(f.b.cy);
-  o.b.cz = shi(f.b.cz);
-  o.b.hx = shi(f.b.hx);
-  o.b.hy = shi(f.b.hy);
-  o.b.hz = shi(f.b.hz);
-  o.b.kind = sh(f.b.kind);
-  o.b.material = sh(f.b.material);
-  return o;
-}
+constexpr uint32_t kRecBlock = 256;
 
 // One thread per record. A wave takes chunks of kRecChunk consecutive records
 // (grid stride); the row scan noted the row holding each chunk's first record.
-// The wave holds a window of 64 consecutive rows in its lanes (lane l: row
-// w + l, its first record and its fields, its brush's read with it) and walks
-// the window's records in steps of at most 64: each lane finds its record's
-// row among the lanes (a 6-step search; the first records ascend over the
-// lanes) and takes that lane's fields. The memory round trips (rows, then
-// brushes) come once per window, not once per step; the next window starts
-// at the record where this one ends.
-__global__ void brush_records_kernel(BrushCands c, TouchedRows rows, RecordOut out) {
+// The wave loads a window of 64 consecutive rows (lane l: row w + l, and its
+// brush), puts each row's fields in LDS and keeps the rows' first records in
+// its lanes, then walks the window's records in steps of at most 64: each
+// lane finds its record's row among the lanes (a 6-step search; the first
+// records ascend over the lanes) and reads that row's fields from LDS. The
+// memory round trips (rows, then brushes) come once per window, not once per
+// step; the next window starts at the record where this one ends.
+__global__ __launch_bounds__(kRecBlock) void brush_records_kernel(BrushCands c, TouchedRows rows, RecordOut out) {
+  __shared__ RowFields s_rows[kRecBlock / kWave][kWave];
+  RowFields* const wrows = s_rows[wave_id()];
   const uint32_t n = *out.n_rec, nr = *rows.n_rows;
   const uint32_t lane = lane_id();
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
@@ -870,15 +735,27 @@ __global__ void brush_records_kernel(BrushCands c, TouchedRows rows, RecordOut o
       const uint32_t yz = have ? rows.yz[rr] : 0u;
       const uint32_t x0 = have ? rows.x0[rr] : 0u;
       const uint32_t wend = wrow + kWave < nr ? rows.rec[wrow + kWave] : n;  // the window's end
-      const uint32_t* jp = reinterpret_cast<const uint32_t*>(c.jobs + job);
-      const uint4 q0 = gld4(jp), q1 = gld4(jp + 4), q2 = gld4(jp + 8), q3 = gld4(jp + 12);
-      RowFields f;
-      f.b = vx_brush{int32_t(q0.x), int32_t(q0.y), int32_t(q0.z), int32_t(q0.w),
-                     int32_t(q1.x), int32_t(q1.y), q1.z, q1.w};
-      f.gx0 = q2.x + x0 - start;  // BrushJob: bx0, by0, bz0 at words 8..10, scene at word 15
-      f.gy = q2.y + (yz & 0xffffu);
-      f.gz = q2.z + (yz >> 16);
-      f.scene = q3.w;
+      {
+        const uint32_t* jp = reinterpret_cast<const uint32_t*>(c.jobs + job);
+        const uint4 q0 = gld4(jp), q1 = gld4(jp + 4), q2 = gld4(jp + 8), q3 = gld4(jp + 12);
+        RowFields f;
+        f.gx0 = q2.x + x0 - start;  // BrushJob: bx0, by0, bz0 at words 8..10, scene at word 15
+        f.gy = q2.y + (yz & 0xffffu);
+        f.gz = q2.z + (yz >> 16);
+        const uint64_t key = (uint64_t(q3.w) << c.codec.shift) | (spread3(f.gy) << 1) | (spread3(f.gz) << 2);
+        f.key_lo = uint32_t(key);
+        f.key_hi = uint32_t(key >> 32);
+        f.cx = int32_t(q0.x);
+        f.cy = int32_t(q0.y);
+        f.cz = int32_t(q0.z);
+        f.hx = int32_t(q0.w);
+        f.hy = int32_t(q1.x);
+        f.hz = int32_t(q1.y);
+        f.kind = q1.z;
+        f.material = c.materials ? q1.w : (q1.w ? 1u : 0u);
+        f.pad[0] = f.pad[1] = f.pad[2] = 0u;
+        wrows[lane] = f;  // read by other lanes of this wave only (LDS is in order within a wave)
+      }
       const uint32_t stop = min(wend, r1);
       for (uint32_t s0 = i0; s0 < stop; s0 += kWave) {
         const uint32_t i = s0 + lane;
@@ -889,12 +766,12 @@ __global__ void brush_records_kernel(BrushCands c, TouchedRows rows, RecordOut o
           const uint32_t probe = uint32_t(__shfl(start, int(k + step), kWave));
           if (k + step < kWave && probe <= i) k += step;
         }
-        const RowFields g = shfl_row(f, k);
+        const RowFields g = wrows[k];
         if (i < stop) {
           const uint32_t gx = g.gx0 + i;
-          out.put_key(i, (uint64_t(g.scene) << c.codec.shift) | morton3(gx, g.gy, g.gz));
-          out.put_payload(i, brick_mask(g.b, int32_t(gx * 4), int32_t(g.gy * 4), int32_t(g.gz * 4)),
-                          c.materials ? g.b.material : (g.b.material ? 1u : 0u));
+          out.put_key(i, ((uint64_t(g.key_hi) << 32) | g.key_lo) | spread3(gx));
+          const vx_brush b{g.cx, g.cy, g.cz, g.hx, g.hy, g.hz, g.kind, g.material};
+          out.put_payload(i, brick_mask(b, int32_t(gx * 4), int32_t(g.gy * 4), int32_t(g.gz * 4)), g.material);
         }
       }
       i0 = max(i0, stop);  // [i0, stop) done
@@ -2082,7 +1959,8 @@ vx_status device_pipeline(vx_dag* prof_dag, const Plan& p, const Scenes& sc, Edi
                                               RowStore{rows, &w.counts[0]}, w.rec_ws, s, false);
       if (st == VX_OK) {
         // About one wave per chunk of records (cand_cap bounds the records).
-        brush_records_kernel<<<grid_for(cand_cap / (kRecChunk / kWave)), 256, 0, s>>>(load.c, rows, record_out(w));
+        brush_records_kernel<<<grid_for(cand_cap / (kRecChunk / kWave)), kRecBlock, 0, s>>>(load.c, rows,
+                                                                                             record_out(w));
         st = last_launch();
       }
     } else {
