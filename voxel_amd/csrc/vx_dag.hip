@@ -126,8 +126,19 @@ __device__ __forceinline__ void load_item(const uint32_t* p, uint32_t n, uint32_
       return;
     }
   }
+  if constexpr (kMax % 4 == 0) {
+    // (Not taken by aligned full-width items; kept small in registers.)
 #pragma unroll
-  for (uint32_t k = 0; k < kMax; ++k) w[k] = k < n ? gld(p + k) : 0u;
+    for (uint32_t k = 0; k < kMax; ++k) w[k] = k < n ? gld(p + k) : 0u;
+  } else {
+    // Words past the item's width read its first word again (no branch on
+    // n, which would make each load wait for the one before; no line past
+    // the item), then read as 0.
+#pragma unroll
+    for (uint32_t k = 0; k < kMax; ++k) w[k] = gld(p + (k < n ? k : 0u));
+#pragma unroll
+    for (uint32_t k = 0; k < kMax; ++k) w[k] &= k < n ? 0xffffffffu : 0u;
+  }
 }
 
 template <uint32_t kMax>
@@ -160,15 +171,20 @@ __device__ __forceinline__ void store_item(uint32_t* p, uint32_t n, const uint32
 template <uint32_t kMax>
 __device__ __forceinline__ void probe_entry(const ItemSet& set, const uint64_t* __restrict__ hashes, uint64_t* res,
                                             const Scenes& sc, bool leaves, uint32_t i) {
-  if (res_kind(res[i]) != kResPending) return;
+  // One round of reads: the entry's result, its hash and its whole item slot
+  // (a slot holds set.stride == kMax words), the width taken from the loaded
+  // header and the words past it cleared afterwards.
+  const uint64_t r0 = res[i];
+  const uint64_t h = hashes[i];
+  uint32_t mine[kMax];
+  load_item<kMax>(set.items + size_t(i) * set.stride, kMax, mine);
+  if (res_kind(r0) != kResPending) return;
   {
     const Dev& d = sc.of(i);
     const Table t = table_of(d, leaves);
-    const uint32_t* it = set.items + size_t(i) * set.stride;
-    const uint32_t w = set.width(it);
-    uint32_t mine[kMax];
-    load_item<kMax>(it, w, mine);
-    const uint64_t h = hashes[i];
+    const uint32_t w = set.fixed_width ? set.fixed_width : 1u + __popc(mine[0] & 0xffu);
+#pragma unroll
+    for (uint32_t k = 0; k < kMax; ++k) mine[k] &= k < w ? 0xffffffffu : 0u;
     const uint32_t fp = uint32_t(h >> 32);
     uint64_t idx = h & t.mask;
     const uint64_t limit = t.mask < kMaxProbe ? t.mask + 1 : kMaxProbe;
@@ -1232,8 +1248,15 @@ __global__ void probe_slices_kernel(ItemSet set, const uint64_t* __restrict__ ha
 // into per-parent LDS slots (a per-thread walk over each parent's children
 // touches ~6 scattered entries per lane per step, and the lines it drags in
 // are evicted before the next step), then each thread merges its slots into
-// the old node's children.
+// the old node's children. Every read goes out in rounds without branches on
+// loaded values (a load under such a branch makes the next one wait for it):
+// the old ref and the old node's header before the child entries; the child
+// entries 8 per thread at a time (links and results, then the results of
+// duplicates); the old node's child words, one load per child slot (a slot
+// without a child reads the header's word again, so no line outside the
+// node is touched).
 constexpr uint32_t kBuildBlock = 256;
+constexpr uint32_t kBuildBatch = 8;
 
 __global__ __launch_bounds__(kBuildBlock) void node_build_kernel(
     Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ old, const uint32_t* __restrict__ first,
@@ -1244,19 +1267,57 @@ __global__ __launch_bounds__(kBuildBlock) void node_build_kernel(
   const uint32_t t = threadIdx.x;
   for (uint32_t j0 = blockIdx.x * kBuildBlock; j0 < n; j0 += gridDim.x * kBuildBlock) {
     const uint32_t jn = min(j0 + kBuildBlock, n);
+    const uint32_t j = j0 + t;
+    const bool live = j < n;
+    const uint32_t* pool = sc.of(live ? j : j0).node_pool;
+    const uint32_t o = live ? old[j] : kEmpty;
+    const uint32_t hdr = gld(pool + (o > kFull ? o : 0u));
 #pragma unroll
     for (uint32_t c = 0; c < 8; ++c) s_ch[c * kBuildBlock + t] = kNone;
     __syncthreads();
-    const uint32_t k1 = first[jn];
-    for (uint32_t k = first[j0] + t; k < k1; k += kBuildBlock) {
-      const uint32_t link = cparent[k], pj = link_parent(link);
-      if (pj >= j0 && pj < jn) s_ch[link_slot(link) * kBuildBlock + (pj - j0)] = resolve(cres, k);
+    const uint32_t kb = first[j0], k1 = first[jn];
+    for (uint32_t k0 = kb; k0 < k1; k0 += kBuildBatch * kBuildBlock) {
+      uint32_t link[kBuildBatch];
+      uint64_t r[kBuildBatch];
+#pragma unroll
+      for (uint32_t u = 0; u < kBuildBatch; ++u) {
+        const uint32_t k = k0 + u * kBuildBlock + t;
+        const uint32_t kk = k < k1 ? k : kb;
+        link[u] = cparent[kk];
+        r[u] = cres[kk];
+      }
+      uint64_t rep[kBuildBatch];  // a duplicate's representative (the same line otherwise)
+#pragma unroll
+      for (uint32_t u = 0; u < kBuildBatch; ++u) {
+        const uint32_t k = k0 + u * kBuildBlock + t;
+        rep[u] = cres[res_kind(r[u]) == kResDup ? uint32_t(r[u]) : (k < k1 ? k : kb)];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kBuildBatch; ++u) r[u] = res_kind(r[u]) == kResDup ? rep[u] : r[u];
+#pragma unroll
+      for (uint32_t u = 0; u < kBuildBatch; ++u) {
+        const uint32_t k = k0 + u * kBuildBlock + t;
+        const uint32_t pj = link_parent(link[u]);
+        if (k < k1 && pj >= j0 && pj < jn) s_ch[link_slot(link[u]) * kBuildBlock + (pj - j0)] = uint32_t(r[u]);
+      }
     }
     __syncthreads();
-    const uint32_t j = j0 + t;
-    if (j < n) {
+    if (live) {
       uint32_t ch[8];
-      children_of(sc.of(j).node_pool, old[j], ch);
+      {
+        const uint32_t cm = o > kFull ? hdr & 0xffu : 0u, fm = (hdr >> 8) & 0xffu;
+        uint32_t w[8];
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+          const uint32_t bit = 1u << c;
+          w[c] = gld(pool + (o > kFull ? o + ((cm & bit) ? 1u + __popc(cm & (bit - 1u)) : 0u) : 0u));
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+          const uint32_t bit = 1u << c;
+          ch[c] = o <= kFull ? o : (cm & bit) ? w[c] : (fm & bit) ? kFull : kEmpty;
+        }
+      }
 #pragma unroll
       for (uint32_t c = 0; c < 8; ++c) {
         const uint32_t v = s_ch[c * kBuildBlock + t];
