@@ -405,15 +405,20 @@ struct CommitStore {
 template <uint32_t kMax>
 __device__ __forceinline__ void place_entry(const ItemSet& set, const uint64_t* __restrict__ slot_vals, uint64_t* res,
                                             const Scenes& sc, bool leaves, uint32_t i) {
+  // Node items (up to 9 words): one round of reads (result, slot value, the
+  // whole item slot: set.stride == kMax words), as in probe_entry. Leaf
+  // items (16-byte vectors): the result first, so that the batch's
+  // duplicates, which are not copied, read no item.
   const uint64_t r = res[i];
-  if (res_kind(r) != kResRep) return;
-  const Table t = table_of(sc.of(i), leaves);
   const uint64_t sv = slot_vals[i];
-  const uint32_t off = uint32_t(sv);
-  const uint32_t* it = set.items + size_t(i) * set.stride;
-  const uint32_t w = set.width(it);
   uint32_t words[kMax];
-  load_item<kMax>(it, w, words);
+  constexpr bool kVec = kMax % 4 == 0;
+  if (!kVec) load_item<kMax>(set.items + size_t(i) * set.stride, kMax, words);
+  if (res_kind(r) != kResRep) return;
+  if (kVec) load_item<kMax>(set.items + size_t(i) * set.stride, kMax, words);
+  const Table t = table_of(sc.of(i), leaves);
+  const uint32_t off = uint32_t(sv);
+  const uint32_t w = set.fixed_width ? set.fixed_width : 1u + __popc(words[0] & 0xffu);
   store_item<kMax>(t.pool + off, w, words);
   st_agent(&t.slots[uint32_t(r)], sv);
   res[i] = res_make(kResFinal, off);
@@ -1243,6 +1248,33 @@ __global__ void probe_slices_kernel(ItemSet set, const uint64_t* __restrict__ ha
   }
 }
 
+// The children of old node `ref` in two rounds of reads: its header, then one
+// load per child slot (a slot without a child, and a sentinel ref, read the
+// header's word or pool word 0 again: no branch on a loaded value, which
+// would make each load wait for the one before, and no line outside the
+// node). children_of reads the same words one after another.
+__device__ __forceinline__ uint32_t old_header(const uint32_t* pool, uint32_t ref) {
+  return gld(pool + (ref > kFull ? ref : 0u));
+}
+__device__ __forceinline__ void old_children_of_header(const uint32_t* pool, uint32_t ref, uint32_t hdr,
+                                                       uint32_t (&ch)[8]) {
+  const uint32_t cm = ref > kFull ? hdr & 0xffu : 0u, fm = (hdr >> 8) & 0xffu;
+  uint32_t w[8];
+#pragma unroll
+  for (uint32_t c = 0; c < 8; ++c) {
+    const uint32_t bit = 1u << c;
+    w[c] = gld(pool + (ref > kFull ? ref + ((cm & bit) ? 1u + __popc(cm & (bit - 1u)) : 0u) : 0u));
+  }
+#pragma unroll
+  for (uint32_t c = 0; c < 8; ++c) {
+    const uint32_t bit = 1u << c;
+    ch[c] = ref <= kFull ? ref : (cm & bit) ? w[c] : (fm & bit) ? kFull : kEmpty;
+  }
+}
+__device__ __forceinline__ void old_children(const uint32_t* pool, uint32_t ref, uint32_t (&ch)[8]) {
+  old_children_of_header(pool, ref, old_header(pool, ref), ch);
+}
+
 // Grid form: a block takes 256 consecutive parents; their child entries are
 // one contiguous range of the height below, read coalesced by the whole block
 // into per-parent LDS slots (a per-thread walk over each parent's children
@@ -1271,7 +1303,7 @@ __global__ __launch_bounds__(kBuildBlock) void node_build_kernel(
     const bool live = j < n;
     const uint32_t* pool = sc.of(live ? j : j0).node_pool;
     const uint32_t o = live ? old[j] : kEmpty;
-    const uint32_t hdr = gld(pool + (o > kFull ? o : 0u));
+    const uint32_t hdr = old_header(pool, o);
 #pragma unroll
     for (uint32_t c = 0; c < 8; ++c) s_ch[c * kBuildBlock + t] = kNone;
     __syncthreads();
@@ -1304,20 +1336,7 @@ __global__ __launch_bounds__(kBuildBlock) void node_build_kernel(
     __syncthreads();
     if (live) {
       uint32_t ch[8];
-      {
-        const uint32_t cm = o > kFull ? hdr & 0xffu : 0u, fm = (hdr >> 8) & 0xffu;
-        uint32_t w[8];
-#pragma unroll
-        for (uint32_t c = 0; c < 8; ++c) {
-          const uint32_t bit = 1u << c;
-          w[c] = gld(pool + (o > kFull ? o + ((cm & bit) ? 1u + __popc(cm & (bit - 1u)) : 0u) : 0u));
-        }
-#pragma unroll
-        for (uint32_t c = 0; c < 8; ++c) {
-          const uint32_t bit = 1u << c;
-          ch[c] = o <= kFull ? o : (cm & bit) ? w[c] : (fm & bit) ? kFull : kEmpty;
-        }
-      }
+      old_children_of_header(pool, o, hdr, ch);
 #pragma unroll
       for (uint32_t c = 0; c < 8; ++c) {
         const uint32_t v = s_ch[c * kBuildBlock + t];
@@ -1489,18 +1508,37 @@ __global__ __launch_bounds__(scan::kFusedBlock) void tail_nodes_kernel(TailArgs 
     uint32_t w = 0;
     uint64_t r = 0, hash = 0;
     if (act) {
+      // Reads in rounds without branches on loaded values (as in
+      // node_build_kernel): the old ref, the range of child entries (at most
+      // 8), the old node's header; its child words and the entries' links
+      // and results; the representatives of duplicates.
       uint32_t ch[8];
-      children_of(sh.of(keys, t).node_pool, a.hs.old[h][t], ch);
-      const uint32_t k1 = a.hs.first[h][t + 1];
-      for (uint32_t k = a.hs.first[h][t]; k < k1; ++k) {
-        const uint32_t link = a.hs.parent[h - 1][k];
-        if (link_parent(link) != t) continue;
-        uint64_t cr = lds_children ? cres_lds[k] : cres_glb[k];
-        if (res_kind(cr) == kResDup) cr = lds_children ? cres_lds[uint32_t(cr)] : cres_glb[uint32_t(cr)];
-        const uint32_t slot = link_slot(link);
+      const uint32_t* pool = sh.of(keys, t).node_pool;
+      const uint32_t o = a.hs.old[h][t];
+      const uint32_t k0 = a.hs.first[h][t], k1 = a.hs.first[h][t + 1];
+      old_children(pool, o, ch);
+      uint32_t link[8];
+      uint64_t cr[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        const uint32_t k = k0 + u < k1 ? k0 + u : k0;
+        link[u] = a.hs.parent[h - 1][k];
+        cr[u] = lds_children ? cres_lds[k] : cres_glb[k];
+      }
+      uint64_t rep[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        const uint32_t k = res_kind(cr[u]) == kResDup ? uint32_t(cr[u]) : (k0 + u < k1 ? k0 + u : k0);
+        rep[u] = lds_children ? cres_lds[k] : cres_glb[k];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        if (k0 + u >= k1 || link_parent(link[u]) != t) continue;
+        const uint32_t v = uint32_t(res_kind(cr[u]) == kResDup ? rep[u] : cr[u]);
+        const uint32_t slot = link_slot(link[u]);
 #pragma unroll
         for (uint32_t c = 0; c < 8; ++c)
-          if (c == slot) ch[c] = uint32_t(cr);  // constant indices: ch stays in registers
+          if (c == slot) ch[c] = v;  // constant indices: ch stays in registers
       }
       uint32_t ref = 0;
       w = make_node_item_regs(ch, it, &ref);
