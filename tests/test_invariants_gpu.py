@@ -20,19 +20,33 @@ pytestmark = pytest.mark.gpu
 M64 = (1 << 64) - 1
 
 
-def _mix64(z):
-    """splitmix64 finalizer (published constants), the tables' word mixer."""
-    z = (z + 0x9E3779B97F4A7C15) & M64
-    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
-    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
-    return z ^ (z >> 31)
+def _chunk(h, c):
+    """One 64-bit chunk (two item words) folded into the item hash."""
+    h = ((h ^ c) * 0x9E3779B97F4A7C15) & M64
+    return h ^ (h >> 32)
+
+
+def _final(h):
+    """MurmurHash3's fmix64 (published constants)."""
+    h ^= h >> 33
+    h = (h * 0xFF51AFD7ED558CCD) & M64
+    h ^= h >> 33
+    h = (h * 0xC4CEB9FE1A85EC53) & M64
+    return h ^ (h >> 33)
 
 
 def _hash(words):
-    h = 0x243F6A8885A308D3 ^ len(words)
-    for w in words:
-        h = _mix64(h ^ int(w))
-    return h
+    """The tables' item hash: the words two at a time as 64-bit chunks (low
+    word first), a last odd word alone, then the final avalanche."""
+    w = [int(x) for x in words]
+    h = 0x243F6A8885A308D3 ^ len(w)
+    k = 0
+    while k + 1 < len(w):
+        h = _chunk(h, w[k] | (w[k + 1] << 32))
+        k += 2
+    if k < len(w):
+        h = _chunk(h, w[k])
+    return _final(h)
 
 
 def _build(items, n_slots=64):

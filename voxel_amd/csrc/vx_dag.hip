@@ -1342,13 +1342,17 @@ __global__ __launch_bounds__(kBuildBlock) void node_build_kernel(
         const uint32_t v = s_ch[c * kBuildBlock + t];
         if (v != kNone) ch[c] = v;
       }
-      uint32_t* it = items + size_t(j) * kNodeStride;
+      // The item in registers (hashed there, not read back from memory).
+      uint32_t item[kNodeStride];
       uint32_t ref = 0;
-      const uint32_t w = make_node_item(ch, it, &ref);
+      const uint32_t w = make_node_item_regs(ch, item, &ref);
       if (!w) {
         res[j] = res_make(kResFinal, ref);
       } else {
-        hashes[j] = hash_words(it, w);
+        uint32_t* it = items + size_t(j) * kNodeStride;
+#pragma unroll
+        for (uint32_t q = 0; q < kNodeStride; ++q) it[q] = item[q];
+        hashes[j] = hash_words_regs(item, w);
         res[j] = res_make(kResPending, 0);
       }
     }

@@ -156,10 +156,31 @@ __device__ __forceinline__ uint32_t make_node_item_regs(const uint32_t (&ch)[8],
   return 1 + __popc(cm);
 }
 
+// Item hash (slot index from the low bits, fingerprint from the high 32):
+// the words two at a time as 64-bit chunks, each folded in with one
+// multiplication (h = (h ^ chunk) * K, then h ^= h >> 32: a bijection of h per
+// chunk), and MurmurHash3's fmix64 avalanche at the end. 64-bit
+// multiplications are the cost here (each is several quarter-rate 32-bit
+// multiplies on this part): 10 for a 16-word leaf instead of 32 with a full
+// mix per word.
+__host__ __device__ __forceinline__ uint64_t hash_chunk(uint64_t h, uint64_t c) {
+  h = (h ^ c) * 0x9E3779B97F4A7C15ull;
+  return h ^ (h >> 32);
+}
+__host__ __device__ __forceinline__ uint64_t hash_final(uint64_t h) {
+  h ^= h >> 33;
+  h *= 0xFF51AFD7ED558CCDull;
+  h ^= h >> 33;
+  h *= 0xC4CEB9FE1A85EC53ull;
+  return h ^ (h >> 33);
+}
+
 __device__ __forceinline__ uint64_t hash_words(const uint32_t* w, uint32_t n) {
   uint64_t h = 0x243F6A8885A308D3ull ^ n;
-  for (uint32_t k = 0; k < n; ++k) h = mix64(h ^ w[k]);
-  return h;
+  uint32_t k = 0;
+  for (; k + 1 < n; k += 2) h = hash_chunk(h, w[k] | (uint64_t(w[k + 1]) << 32));
+  if (k < n) h = hash_chunk(h, w[k]);
+  return hash_final(h);
 }
 
 // hash_words of a register array of up to kN words (constant indices).
@@ -167,9 +188,12 @@ template <uint32_t kN>
 __device__ __forceinline__ uint64_t hash_words_regs(const uint32_t (&w)[kN], uint32_t n) {
   uint64_t h = 0x243F6A8885A308D3ull ^ n;
 #pragma unroll
-  for (uint32_t k = 0; k < kN; ++k)
-    if (k < n) h = mix64(h ^ w[k]);
-  return h;
+  for (uint32_t k = 0; k < kN; k += 2) {
+    const uint64_t hi = k + 1 < kN ? uint64_t(w[k + 1 < kN ? k + 1 : k]) << 32 : 0ull;
+    if (k + 1 < n) h = hash_chunk(h, w[k] | hi);
+    else if (k < n) h = hash_chunk(h, w[k]);
+  }
+  return hash_final(h);
 }
 
 __device__ __forceinline__ bool words_equal(const uint32_t* a, const uint32_t* b, uint32_t n) {
@@ -180,9 +204,11 @@ __device__ __forceinline__ bool words_equal(const uint32_t* a, const uint32_t* b
 
 // Leaf payload helpers. Material leaves: byte i of the 64 is voxel bit i
 // (bit = x + 4y + 16z, detail/node.cuh:87-94), word i/4, byte i%4.
+// (Bit b of the nibble lands on bit 8b of nib * 0x204081, no two partial
+// products overlapping; x * 255 = (x << 8) - x turns each 0x01 byte into 0xff.)
 __device__ __forceinline__ uint32_t nibble_bytes(uint32_t nib) {
-  return ((nib & 1u) ? 0x000000ffu : 0u) | ((nib & 2u) ? 0x0000ff00u : 0u) | ((nib & 4u) ? 0x00ff0000u : 0u) |
-         ((nib & 8u) ? 0xff000000u : 0u);
+  const uint32_t x = (nib * 0x204081u) & 0x01010101u;
+  return (x << 8) - x;
 }
 
 __host__ __device__ __forceinline__ uint64_t spread3(uint32_t v) {
