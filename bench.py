@@ -775,14 +775,18 @@ def kernel_bytes(prof, leaf_words):
                                                         # per parent: its old ref and node word
         "leaf_probe": (40 + lw) * Q + L // 16,          # pending entry, list entry, item, hash, result read and
                                                         # written; slice counts and bases
-        "leaf_commit": 28 * Q,                          # list entry, result, hash read; slot value written
-        "leaf_place": (20 + 2 * lw) * Q,                # list entry, result, slot value; the new item read from
-                                                        # the batch and written to the pool (Q also holds the
-                                                        # batch's duplicates of a new item, which are not copied)
-        "node_build": 64 * N + 20 * (L + N),            # old ref, starts, item, hash, result; child entries
+        "leaf_commit": 36 * Q,                          # list entry, result, hash read; slot value, result and
+                                                        # the noted slot written
+        "leaf_place": (16 + 2 * lw) * Q,                # (launched with height 1's build, counted there)
+        "node_build": 64 * N + 20 * (L + N) + (16 + 2 * lw) * Q + 8 * N,
+                                                        # old ref, starts, item, hash, result; child entries;
+                                                        # and, in the same launches, the place of the height
+                                                        # below: noted slot, list entry, slot value, the new
+                                                        # item read from the batch and written to the pool (Q
+                                                        # also holds the batch's duplicates, not copied)
         "node_probe": 16 * N,
-        "node_commit": 16 * N,
-        "node_place": 8 * N,
+        "node_commit": 24 * N,
+        "node_place": 8 * N,                            # the last grid height's place, launched alone
     }
 
 
@@ -806,7 +810,9 @@ def _kind_of(name, last_probe):
             return kind
     if "probe_kernel" in name:
         return "node_probe" if "<9u>" in name else "leaf_probe"
-    if "place_kernel" in name:
+    if "build_and_place_kernel" in name:  # a height's place launched with the next height's build
+        return "node_build"
+    if "place_kernel" in name or "place_noted_kernel" in name:
         return "node_place" if "<9u>" in name else "leaf_place"
     if "CommitLoad" in name:  # the leaf commit scans 8 items per thread, the node heights 4
         return "leaf_commit" if ", 8u," in name else "node_commit"

@@ -370,17 +370,24 @@ struct CommitLoad {
   }
 };
 
+// With slot_idx (final mode) the store half also makes each representative
+// FINAL at once (its ref is its pool offset) and notes, per position t of
+// the scan, the slot its place step publishes (kNone: nothing to place), so
+// that the next height's build can run beside the place.
 struct CommitStore {
   uint64_t* res;
   uint64_t* slot_vals;  // the hashes array, reused
   Scenes sc;
   bool leaves;
-  __device__ void operator()(uint32_t, const CommitPayload& p, uint64_t v, uint64_t, uint64_t incl, uint32_t) const {
+  uint32_t* slot_idx = nullptr;
+  __device__ void operator()(uint32_t pos, const CommitPayload& p, uint64_t v, uint64_t, uint64_t incl,
+                             uint32_t) const {
     const uint32_t i = p.i;
     const Dev& d = sc.of(i);
     const Table t = table_of(d, leaves);
     const uint32_t base = p.base;
     const uint32_t w = uint32_t(v), seg_incl = uint32_t(incl);
+    uint32_t place = kNone;
     if (w) {
       const uint64_t fp_hi = uint64_t(p.fp) << 32;
       const uint64_t off = uint64_t(base) + seg_incl - w;
@@ -390,8 +397,13 @@ struct CommitStore {
         res[i] = res_make(kResFinal, kEmpty);
       } else {
         slot_vals[i] = fp_hi | uint32_t(off);
+        if (slot_idx) {
+          res[i] = res_make(kResFinal, uint32_t(off));
+          place = uint32_t(p.r);
+        }
       }
     }
+    if (slot_idx) slot_idx[pos] = place;
     if (p.last) {
       const uint64_t used = uint64_t(base) + seg_incl;
       *t.used = used > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(used);
@@ -430,6 +442,60 @@ __global__ void place_kernel(ItemSet set, const uint64_t* __restrict__ slot_vals
   const uint32_t n = *n_ptr;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
     place_entry<kMax>(set, slot_vals, res, sc, leaves, list ? list[t] : t);
+}
+
+// Place after a final-mode commit: position t of the commit scan copies its
+// entry's item to the pool and publishes the slot the commit noted (results
+// are already FINAL).
+struct PlaceArgs {
+  ItemSet set;
+  const uint64_t* slot_vals;
+  const uint32_t* slot_idx;
+  const uint32_t* n_ptr;
+  const uint32_t* list;
+  Scenes sc;
+  bool leaves;
+};
+
+template <uint32_t kMax>
+__device__ __forceinline__ void place_noted(const PlaceArgs& a, uint32_t block, uint32_t blocks) {
+  const uint32_t n = *a.n_ptr;
+  for (uint32_t t = block * blockDim.x + threadIdx.x; t < n; t += blocks * blockDim.x) {
+    // One round of reads: the noted slot, the entry, its slot value and item
+    // slot (set.stride == kMax words); the entry is t without a list.
+    const uint32_t slot = a.slot_idx[t];
+    const uint32_t i = a.list ? a.list[t] : t;
+    if (slot == kNone) continue;
+    const uint64_t sv = a.slot_vals[i];
+    uint32_t words[kMax];
+    load_item<kMax>(a.set.items + size_t(i) * a.set.stride, kMax, words);
+    const Table tb = table_of(a.sc.of(i), a.leaves);
+    const uint32_t w = a.set.fixed_width ? a.set.fixed_width : 1u + __popc(words[0] & 0xffu);
+    store_item<kMax>(tb.pool + uint32_t(sv), w, words);
+    st_agent(&tb.slots[slot], sv);
+  }
+}
+
+template <uint32_t kMax>
+__global__ void place_noted_kernel(PlaceArgs a) {
+  place_noted<kMax>(a, blockIdx.x, gridDim.x);
+}
+
+// The commit scan alone, in final mode (the place runs later, in
+// place_noted_kernel or beside the next height's build).
+template <uint32_t kMax>
+vx_status commit_final(const Scenes& sc, uint32_t n_scenes, bool leaves, ItemSet set, uint64_t* hashes,
+                       const uint32_t* n_ptr, const uint32_t* list, uint64_t n_ub, uint64_t* res, uint32_t* slot_idx,
+                       void* fused_ws, hipStream_t s, bool clear_ws) {
+  TimedLaunch tl(leaves ? VX_K_LEAF_COMMIT : VX_K_NODE_COMMIT, s);
+  const CommitLoad ld{res, hashes, n_ptr, list, set, sc, leaves};
+  const CommitStore sto{res, hashes, sc, leaves, slot_idx};
+  constexpr uint32_t kBig = scan::kFusedItems, kSmall = scan::kFusedItemsSmall;
+  if (n_scenes == 1)
+    return leaves ? scan::fused_dev_m<scan::SumOp, kBig>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws)
+                  : scan::fused_dev_m<scan::SumOp, kSmall>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws);
+  return leaves ? scan::fused_dev_m<scan::SegSumOp, kBig>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws)
+                : scan::fused_dev_m<scan::SegSumOp, kSmall>(n_ptr, n_ub, ld, sto, fused_ws, s, clear_ws);
 }
 
 // Commit and place of the entries list[0 .. *n_ptr) (0 .. *n_ptr without a
@@ -1290,14 +1356,32 @@ __device__ __forceinline__ void old_children(const uint32_t* pool, uint32_t ref,
 constexpr uint32_t kBuildBlock = 256;
 constexpr uint32_t kBuildBatch = 8;
 
-__global__ __launch_bounds__(kBuildBlock) void node_build_kernel(
-    Scenes sc, const uint32_t* n_ptr, const uint32_t* __restrict__ old, const uint32_t* __restrict__ first,
-    const uint32_t* __restrict__ cparent, const uint64_t* __restrict__ cres,
-    uint32_t* __restrict__ items, uint64_t* __restrict__ hashes, uint64_t* __restrict__ res) {
+struct BuildArgs {
+  Scenes sc;
+  const uint32_t* n_ptr;
+  const uint32_t* old;
+  const uint32_t* first;
+  const uint32_t* cparent;
+  const uint64_t* cres;
+  uint32_t* items;
+  uint64_t* hashes;
+  uint64_t* res;
+};
+
+// Blocks `block` of `blocks` (grid stride) of a height's build.
+__device__ __forceinline__ void build_nodes(const BuildArgs& a, uint32_t block, uint32_t blocks) {
   __shared__ uint32_t s_ch[kBuildBlock * 8];
-  const uint32_t n = *n_ptr;
+  const Scenes& sc = a.sc;
+  const uint32_t* __restrict__ old = a.old;
+  const uint32_t* __restrict__ first = a.first;
+  const uint32_t* __restrict__ cparent = a.cparent;
+  const uint64_t* __restrict__ cres = a.cres;
+  uint32_t* __restrict__ items = a.items;
+  uint64_t* __restrict__ hashes = a.hashes;
+  uint64_t* __restrict__ res = a.res;
+  const uint32_t n = *a.n_ptr;
   const uint32_t t = threadIdx.x;
-  for (uint32_t j0 = blockIdx.x * kBuildBlock; j0 < n; j0 += gridDim.x * kBuildBlock) {
+  for (uint32_t j0 = block * kBuildBlock; j0 < n; j0 += blocks * kBuildBlock) {
     const uint32_t jn = min(j0 + kBuildBlock, n);
     const uint32_t j = j0 + t;
     const bool live = j < n;
@@ -1360,6 +1444,22 @@ __global__ __launch_bounds__(kBuildBlock) void node_build_kernel(
   }
 }
 
+__global__ __launch_bounds__(kBuildBlock) void node_build_kernel(BuildArgs a) { build_nodes(a, blockIdx.x, gridDim.x); }
+
+// One launch for two independent steps: the place of height h (its first
+// place_blocks blocks; results are already FINAL after the final-mode
+// commit) and the build of height h + 1 (the other blocks), which reads
+// only those results. The heights' items and hashes sit in separate
+// buffers (even and odd heights).
+template <uint32_t kMax>
+__global__ __launch_bounds__(kBuildBlock) void build_and_place_kernel(PlaceArgs pa, uint32_t place_blocks,
+                                                                      BuildArgs ba) {
+  if (blockIdx.x < place_blocks)
+    place_noted<kMax>(pa, blockIdx.x, place_blocks);
+  else
+    build_nodes(ba, blockIdx.x - place_blocks, gridDim.x - place_blocks);
+}
+
 struct ResByHeight {
   const uint64_t* p[kMaxHeights];
 };
@@ -1411,8 +1511,13 @@ struct EditWs {
   sort::Tuning sort_tn;  // the tuning sort_ws was sized for
   uint32_t index_bits;  // > 0: records sort as packed u64 keys (RecordOut)
   Heights hs;
-  uint32_t* items;
-  uint64_t* hashes;
+  uint32_t* items;       // even heights (leaves included)
+  uint32_t* items_odd;   // odd heights
+  uint64_t* hashes;      // even heights
+  uint64_t* hashes_odd;  // odd heights
+  uint32_t* slot_idx;    // per commit position: the slot its place publishes (CommitStore)
+  uint32_t* items_at(uint32_t h) const { return (h & 1) ? items_odd : items; }
+  uint64_t* hashes_at(uint32_t h) const { return (h & 1) ? hashes_odd : hashes; }
   // Workspaces of the pipeline's fused scans, one per scan so that a single
   // fill at the start of an edit clears them all.
   uint32_t* fused_all;
@@ -1694,7 +1799,10 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
   w.sort_tn = w.index_bits ? sort::keys_u64_dev_tuning(std::max(p.key_bits, 1u)) : sort::pairs_u64_dev_tuning();
   w.sort_ws = c.take<char>(w.index_bits ? sort::keys_u64_dev_workspace_bytes(R, std::max(p.key_bits, 1u), w.sort_tn)
                                         : sort::pairs_u64_dev_workspace_bytes(R, p.key_bits, w.sort_tn));
-  uint64_t max_cap = 1, max_items = 1;
+  // Items and hashes of even heights (the leaves included) and of odd
+  // heights in separate buffers: a height's place runs beside the next
+  // height's build.
+  uint64_t max_cap[2] = {1, 1}, max_items[2] = {1, 1}, max_cap_all = 1;
   const uint32_t H = p.max_depth;
   for (uint32_t h = 0; h <= H + 1; ++h) {
     uint64_t cap = R;
@@ -1708,10 +1816,13 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
     w.hs.parent[h] = c.take<uint32_t>(cap);
     w.hs.old[h] = c.take<uint32_t>(cap);
     w.hs.res[h] = c.take<uint64_t>(cap);
-    max_cap = std::max(max_cap, cap);
-    max_items = std::max<uint64_t>(max_items, cap * (h == 0 ? p.leaf_words : kNodeStride));
+    max_cap[h & 1] = std::max(max_cap[h & 1], cap);
+    max_cap_all = std::max(max_cap_all, cap);
+    max_items[h & 1] = std::max<uint64_t>(max_items[h & 1], cap * (h == 0 ? p.leaf_words : kNodeStride));
   }
-  w.items = c.take<uint32_t>(max_items);
+  w.items = c.take<uint32_t>(max_items[0]);
+  w.items_odd = c.take<uint32_t>(max_items[1]);
+  w.slot_idx = c.take<uint32_t>(max_cap_all);
   const uint64_t slices = (w.hs.cap[0] + kWave - 1) / kWave;
   w.pend = c.take<uint32_t>(slices * kWave);
   w.pend_list = c.take<uint32_t>(w.hs.cap[0]);
@@ -1724,7 +1835,8 @@ EditWs carve(void* base, const Plan& p, uint64_t r_ub, uint32_t n_jobs, uint64_t
   w.chunk_row = c.take<uint32_t>(R / kRecChunk + 2);
   w.slice_count = c.take<uint32_t>(slices);
   w.slice_base = c.take<uint32_t>(slices);
-  w.hashes = c.take<uint64_t>(max_cap);
+  w.hashes = c.take<uint64_t>(max_cap[0]);
+  w.hashes_odd = c.take<uint64_t>(max_cap[1]);
   {
     char* start = c.take<char>(0);
     w.rec_ws = c.take<char>(scan::fused_dev_workspace_bytes(std::max<uint64_t>(candidates, 1)));
@@ -1823,6 +1935,7 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
   }
   if ((st = last_launch()) != VX_OK) return st;
   mark(g, 4, s);
+  PlaceArgs pending{};  // the place step not launched yet (results already FINAL)
   {
     Scenes sh = sc;
     sh.keys = L.keys[0];
@@ -1855,27 +1968,54 @@ vx_status rebuild(vx_dag* g, const Plan& p, const Scenes& sc, EditWs& w, uint64_
     }
     if ((st = last_launch()) != VX_OK) return st;
     const ItemSet leaf_set{w.items, p.leaf_words, p.leaf_words};
-    st = p.leaf_words == 2 ? commit_place<2>(sh, p.n_scenes, true, leaf_set, w.hashes, ls.total, ls.list, L.cap[0],
-                                             L.res[0], w.commit_ws[0], s, false)
-                           : commit_place<16>(sh, p.n_scenes, true, leaf_set, w.hashes, ls.total, ls.list,
-                                              L.cap[0], L.res[0], w.commit_ws[0], s, false);
+    st = p.leaf_words == 2 ? commit_final<2>(sh, p.n_scenes, true, leaf_set, w.hashes, ls.total, ls.list, L.cap[0],
+                                             L.res[0], w.slot_idx, w.commit_ws[0], s, false)
+                           : commit_final<16>(sh, p.n_scenes, true, leaf_set, w.hashes, ls.total, ls.list,
+                                              L.cap[0], L.res[0], w.slot_idx, w.commit_ws[0], s, false);
     if (st != VX_OK) return st;
+    pending = PlaceArgs{leaf_set, w.hashes, w.slot_idx, ls.total, ls.list, sh, true};
   }
   mark(g, 5, s);
+  // Each height's place runs in one launch with the next height's build
+  // (build_and_place_kernel); the last grid height's place runs alone.
+  uint32_t pending_kmax = p.leaf_words;
+  uint64_t pending_ub = L.cap[0];
+  auto place_alone = [&]() {
+    TimedLaunch tl(pending.leaves ? VX_K_LEAF_PLACE : VX_K_NODE_PLACE, s);
+    const uint32_t grid = grid_for(pending_ub);
+    if (pending_kmax == 2) place_noted_kernel<2><<<grid, 256, 0, s>>>(pending);
+    else if (pending_kmax == 16) place_noted_kernel<16><<<grid, 256, 0, s>>>(pending);
+    else place_noted_kernel<kNodeStride><<<grid, 256, 0, s>>>(pending);
+    return last_launch();
+  };
   for (uint32_t h = 1; h < hu; ++h) {
     Scenes sh = sc;
     sh.keys = L.keys[h];
     {
       TimedLaunch tl(VX_K_NODE_BUILD, s);
-      node_build_kernel<<<grid_for(L.cap[h]), kBuildBlock, 0, s>>>(sh, &cnt[1 + h], L.old[h], L.first[h],
-                                                                   L.parent[h - 1], L.res[h - 1], w.items, w.hashes,
-                                                                   L.res[h]);
+      const BuildArgs ba{sh, &cnt[1 + h], L.old[h], L.first[h], L.parent[h - 1], L.res[h - 1], w.items_at(h),
+                         w.hashes_at(h), L.res[h]};
+      const uint32_t pb = grid_for(pending_ub), grid = pb + grid_for(L.cap[h]);
+      if (pending_kmax == 2) build_and_place_kernel<2><<<grid, kBuildBlock, 0, s>>>(pending, pb, ba);
+      else if (pending_kmax == 16) build_and_place_kernel<16><<<grid, kBuildBlock, 0, s>>>(pending, pb, ba);
+      else build_and_place_kernel<kNodeStride><<<grid, kBuildBlock, 0, s>>>(pending, pb, ba);
     }
     if ((st = last_launch()) != VX_OK) return st;
-    if ((st = intern_scenes(sh, p.n_scenes, false, ItemSet{w.items, kNodeStride, 0}, w.hashes, &cnt[1 + h], L.cap[h],
-                            L.res[h], w.commit_ws[h], s, false)) != VX_OK)
+    const ItemSet set{w.items_at(h), kNodeStride, 0};
+    {
+      TimedLaunch tl(VX_K_NODE_PROBE, s);
+      probe_kernel<kNodeStride><<<grid_for(L.cap[h]), 256, 0, s>>>(set, w.hashes_at(h), &cnt[1 + h], L.res[h], sh,
+                                                                    p.n_scenes, false);
+    }
+    if ((st = last_launch()) != VX_OK) return st;
+    if ((st = commit_final<kNodeStride>(sh, p.n_scenes, false, set, w.hashes_at(h), &cnt[1 + h], nullptr, L.cap[h],
+                                        L.res[h], w.slot_idx, w.commit_ws[h], s, false)) != VX_OK)
       return st;
+    pending = PlaceArgs{set, w.hashes_at(h), w.slot_idx, &cnt[1 + h], nullptr, sh, false};
+    pending_kmax = kNodeStride;
+    pending_ub = L.cap[h];
   }
+  if ((st = place_alone()) != VX_OK) return st;
   if (hu <= H) {
     TimedLaunch tl(VX_K_TAIL_NODES, s);
     tail_nodes_kernel<<<1, scan::kFusedBlock, 0, s>>>(ta);
