@@ -188,23 +188,6 @@ __global__ __launch_bounds__(kHistBlock, 8) void histogram_kernel(const K* __res
   }
 }
 
-// Exclusive scan of each pass's R counters (one block of max(R, 256)
-// threads per pass).
-template <uint32_t RB>
-__global__ __launch_bounds__((1u << RB) > 256 ? (1u << RB) : 256) void hist_scan_kernel(uint32_t* hist) {
-  constexpr uint32_t R = 1u << RB;
-  __shared__ uint32_t s_wave[(R > 256 ? R : 256) / 64];
-  uint32_t* h = hist + blockIdx.x * R;
-  const uint32_t t = threadIdx.x;
-  const uint32_t v = t < R ? h[t] : 0u;
-  const uint32_t inc = wave_inclusive_sum(v);
-  if (lane_id() == 63) s_wave[wave_id()] = inc;
-  __syncthreads();
-  uint32_t before = 0;
-  for (uint32_t w = 0; w < wave_id(); ++w) before += s_wave[w];
-  if (t < R) h[t] = before + inc - v;
-}
-
 // Loads of a wave's KPT keys (lane-contiguous rounds) and their ranks among
 // the wave's earlier keys of the same digit, kept in the wave's LDS
 // histogram `wh`. FULL: the whole tile lies below n (no per-key guards).
@@ -243,7 +226,7 @@ template <class K, bool VALS, uint32_t RB, uint32_t BLOCK, uint32_t KPT>
 __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ kin, K* __restrict__ kout,
                                                           const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                           const uint32_t* n_ptr, uint32_t n_arg, uint32_t shift,
-                                                          uint32_t dmask, const uint32_t* __restrict__ pass_offsets,
+                                                          uint32_t dmask, const uint32_t* __restrict__ pass_counts,
                                                           uint32_t* tile_counter, uint32_t* rec, uint32_t* rec_next) {
   using C = Cfg<K, VALS, RB, BLOCK, KPT>;
   constexpr uint32_t R = C::R, WAVES = C::WAVES, TILE = C::TILE;
@@ -261,6 +244,20 @@ __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ 
   const uint32_t tiles = (n + TILE - 1) / TILE;
   const uint32_t t = threadIdx.x, lane = lane_id(), wave = wave_id();
   uint32_t* wh = s_hist + wave * R;
+  // The pass's digit offsets, the exclusive scan of its histogram, computed
+  // by every block (2 KB read) instead of by a launch of its own; thread t
+  // keeps digit t's.
+  uint32_t digit_base = 0;
+  {
+    const uint32_t c = t < R ? pass_counts[t] : 0u;
+    const uint32_t inc = wave_inclusive_sum(c);
+    if (lane == kWave - 1) s_cnt[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; ++w) before += s_cnt[w];
+    digit_base = before + inc - c;
+    __syncthreads();
+  }
 
   for (;;) {
     for (uint32_t i = t; i < WAVES * R; i += BLOCK) s_hist[i] = 0;
@@ -360,7 +357,7 @@ __global__ __launch_bounds__(BLOCK) void partition_kernel(const K* __restrict__ 
         }
         st_agent(&rec[size_t(tile) * R + t], kStatusPrefix | (excl + s_cnt[t]));
       }
-      s_global[t] = pass_offsets[t] + excl - s_start[t];
+      s_global[t] = digit_base + excl - s_start[t];
       rec_next[size_t(tile) * R + t] = 0u;
     }
     __syncthreads();
@@ -524,13 +521,6 @@ vx_status drive(K* keys, uint32_t* vals, K* alt_keys, uint32_t* alt_vals, const 
       histogram_kernel<K, 9, COHERENT><<<hblocks, kHistBlock, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
     else
       histogram_kernel<K, 4, COHERENT><<<hblocks, kHistBlock, 0, s>>>(keys, n_ptr, n, begin_bit, key_bits, lay.hist);
-  }
-  if ((st = last_launch()) != VX_OK) return st;
-  {
-    TimedLaunch tl(VX_K_SORT_SCAN, s);
-    if (tn.rb == 8) hist_scan_kernel<8><<<lay.passes, 256, 0, s>>>(lay.hist);
-    else if (tn.rb == 9) hist_scan_kernel<9><<<lay.passes, 512, 0, s>>>(lay.hist);
-    else hist_scan_kernel<4><<<lay.passes, 256, 0, s>>>(lay.hist);
   }
   if ((st = last_launch()) != VX_OK) return st;
 
