@@ -241,13 +241,22 @@ __device__ __forceinline__ void probe_entry(const ItemSet& set, const uint64_t* 
 // the decision to the probe.
 constexpr uint32_t kFindProbes = 64;
 
+// *empty_at: the empty slot that ended the walk (else the home slot): every
+// slot before it on the probe sequence holds an item that is not this one,
+// and stays so while the level's probe only tags empty slots, so the probe
+// may start there.
 template <uint32_t kMax>
-__device__ __forceinline__ uint32_t find_interned(const Table& t, const uint32_t (&w)[kMax], uint32_t n, uint64_t h) {
+__device__ __forceinline__ uint32_t find_interned(const Table& t, const uint32_t (&w)[kMax], uint32_t n, uint64_t h,
+                                                  uint64_t* empty_at) {
   const uint32_t fp = uint32_t(h >> 32);
   uint64_t idx = h & t.mask;
+  *empty_at = idx;
   for (uint32_t p = 0; p < kFindProbes; ++p) {
     const uint64_t s = gld(t.slots + idx);
-    if (s == 0) return 0;
+    if (s == 0) {
+      *empty_at = idx;
+      return 0;
+    }
     if (uint32_t(s >> 32) == fp) {
       const uint32_t v = uint32_t(s);
       if (v & kTag) return 0;
@@ -1232,7 +1241,8 @@ __device__ __forceinline__ bool build_leaf(const Dev d, uint32_t j, const uint32
     return false;
   }
   const uint64_t h = hash_words(w, LW);
-  const uint32_t v = find_interned<LW>(table_of(d, true), w, LW, h);
+  uint64_t start = 0;
+  const uint32_t v = find_interned<LW>(table_of(d, true), w, LW, h, &start);
   if (v) {
     res[j] = res_make(kResFinal, v);
     return false;
@@ -1240,7 +1250,9 @@ __device__ __forceinline__ bool build_leaf(const Dev d, uint32_t j, const uint32
   uint32_t* it = items + size_t(j) * LW;
 #pragma unroll
   for (uint32_t k = 0; k < LW; ++k) it[k] = w[k];
-  hashes[j] = h;
+  // The probe starts where the lookup's walk ended (the slot index in the
+  // low word, which the probe masks; the fingerprint stays in the high word).
+  hashes[j] = (h & 0xFFFFFFFF00000000ull) | uint32_t(start);
   res[j] = res_make(kResPending, 0);
   return true;
 }
