@@ -52,14 +52,23 @@ __host__ __device__ __forceinline__ uint64_t sphere_mask32(int32_t dx0, int32_t 
 // operations per voxel pair.
 constexpr int32_t kSphere16MaxR = 100;
 
-__host__ __device__ __forceinline__ uint64_t sphere_mask16(int32_t dx0, int32_t dy0, int32_t dz0, int32_t r2) {
+// The dy, dz part of sphere_mask16, which every brick of one brick row (same
+// y and z) shares: t[4 P + ly] holds r^2 - dy^2 - dz^2 + bias of voxel row
+// (ly, lz = 2P) in its low half and (ly, lz = 2P + 1) in its high half.
+__host__ __device__ __forceinline__ void sphere_row16(int32_t dy0, int32_t dz0, int32_t r2, uint32_t (&t)[8]) {
   constexpr uint32_t kBias = 32768u, kPair = 0x10001u;
-  uint32_t zz[2];  // r^2 - dz^2 + bias of lz = 2P (low half) and 2P + 1 (high half)
 #pragma unroll
   for (int P = 0; P < 2; ++P) {
     const int32_t z0 = dz0 + 2 * P, z1 = z0 + 1;
-    zz[P] = (uint32_t(r2 - z1 * z1 + int32_t(kBias)) << 16) | uint32_t(r2 - z0 * z0 + int32_t(kBias));
+    const uint32_t zz = (uint32_t(r2 - z1 * z1 + int32_t(kBias)) << 16) | uint32_t(r2 - z0 * z0 + int32_t(kBias));
+#pragma unroll
+    for (int ly = 0; ly < 4; ++ly) t[4 * P + ly] = zz - uint32_t((dy0 + ly) * (dy0 + ly)) * kPair;
   }
+}
+
+// The brick at x offset dx0 of a row with terms t (sphere_row16).
+__host__ __device__ __forceinline__ uint64_t sphere_mask16_row(int32_t dx0, const uint32_t (&t)[8]) {
+  constexpr uint32_t kPair = 0x10001u;
   uint32_t qx[4];  // dx^2 in both halves
 #pragma unroll
   for (int l = 0; l < 4; ++l) qx[l] = uint32_t((dx0 + l) * (dx0 + l)) * kPair;
@@ -67,15 +76,19 @@ __host__ __device__ __forceinline__ uint64_t sphere_mask16(int32_t dx0, int32_t 
 #pragma unroll
   for (int P = 0; P < 2; ++P)
 #pragma unroll
-    for (int ly = 0; ly < 4; ++ly) {
-      const uint32_t t = zz[P] - uint32_t((dy0 + ly) * (dy0 + ly)) * kPair;
+    for (int ly = 0; ly < 4; ++ly)
 #pragma unroll
       for (int lx = 0; lx < 4; ++lx) {
         const uint32_t pos = uint32_t(lx + 4 * ly);
-        out[P] |= ((t - qx[lx]) >> (15u - pos)) & (kPair << pos);
+        out[P] |= ((t[4 * P + ly] - qx[lx]) >> (15u - pos)) & (kPair << pos);
       }
-    }
   return (uint64_t(out[1]) << 32) | out[0];
+}
+
+__host__ __device__ __forceinline__ uint64_t sphere_mask16(int32_t dx0, int32_t dy0, int32_t dz0, int32_t r2) {
+  uint32_t t[8];
+  sphere_row16(dy0, dz0, r2, t);
+  return sphere_mask16_row(dx0, t);
 }
 
 // The same test in 64-bit arithmetic, for radii of 2^14 and more (out of

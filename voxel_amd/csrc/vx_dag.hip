@@ -789,15 +789,16 @@ struct RowStore {
 
 // What a record needs of its row, kept in LDS for the 64 rows of a window
 // (four 16-byte words per row): the brick x of the batch's record 0 (record
-// i has gx = gx0 + i), the row's brick y and z, the key bits of its scene, y
-// and z (key = those | spread3(gx)), and the brush.
+// i has gx = gx0 + i), the key bits of its scene, y and z (key = those |
+// spread3(gx)), the brush; and w: for a sphere of radius up to
+// kSphere16MaxR, the row's sphere_row16 terms (what the row's bricks share
+// of their masks), otherwise the row's brick y and z and the brush's
+// cy, cz, hy, hz.
 struct alignas(16) RowFields {
-  uint32_t gx0, gy, gz, key_lo;
-  uint32_t key_hi;
-  int32_t cx, cy, cz;
-  int32_t hx, hy, hz;
-  uint32_t kind;
-  uint32_t material, pad[3];
+  uint32_t gx0, key_lo, key_hi, kind;
+  int32_t cx, hx;
+  uint32_t material, pad;
+  uint32_t w[8];
 };
 static_assert(sizeof(RowFields) == 64, "RowFields layout");
 
@@ -836,20 +837,26 @@ __global__ __launch_bounds__(kRecBlock) void brush_records_kernel(BrushCands c, 
         const uint4 q0 = gld4(jp), q1 = gld4(jp + 4), q2 = gld4(jp + 8), q3 = gld4(jp + 12);
         RowFields f;
         f.gx0 = q2.x + x0 - start;  // BrushJob: bx0, by0, bz0 at words 8..10, scene at word 15
-        f.gy = q2.y + (yz & 0xffffu);
-        f.gz = q2.z + (yz >> 16);
-        const uint64_t key = (uint64_t(q3.w) << c.codec.shift) | (spread3(f.gy) << 1) | (spread3(f.gz) << 2);
+        const uint32_t gy = q2.y + (yz & 0xffffu), gz = q2.z + (yz >> 16);
+        const uint64_t key = (uint64_t(q3.w) << c.codec.shift) | (spread3(gy) << 1) | (spread3(gz) << 2);
         f.key_lo = uint32_t(key);
         f.key_hi = uint32_t(key >> 32);
-        f.cx = int32_t(q0.x);
-        f.cy = int32_t(q0.y);
-        f.cz = int32_t(q0.z);
-        f.hx = int32_t(q0.w);
-        f.hy = int32_t(q1.x);
-        f.hz = int32_t(q1.y);
         f.kind = q1.z;
+        f.cx = int32_t(q0.x);
+        f.hx = int32_t(q0.w);
         f.material = c.materials ? q1.w : (q1.w ? 1u : 0u);
-        f.pad[0] = f.pad[1] = f.pad[2] = 0u;
+        f.pad = 0u;
+        if (f.kind == 0 && f.hx <= kSphere16MaxR) {  // brick_mask's choice for this brush
+          sphere_row16(int32_t(gy * 4) - int32_t(q0.y), int32_t(gz * 4) - int32_t(q0.z), f.hx * f.hx, f.w);
+        } else {
+          f.w[0] = gy;
+          f.w[1] = gz;
+          f.w[2] = q0.y;
+          f.w[3] = q0.z;
+          f.w[4] = q1.x;
+          f.w[5] = q1.y;
+          f.w[6] = f.w[7] = 0u;
+        }
         wrows[lane] = f;  // read by other lanes of this wave only (LDS is in order within a wave)
       }
       const uint32_t stop = min(wend, r1);
@@ -866,8 +873,15 @@ __global__ __launch_bounds__(kRecBlock) void brush_records_kernel(BrushCands c, 
         if (i < stop) {
           const uint32_t gx = g.gx0 + i;
           out.put_key(i, ((uint64_t(g.key_hi) << 32) | g.key_lo) | spread3(gx));
-          const vx_brush b{g.cx, g.cy, g.cz, g.hx, g.hy, g.hz, g.kind, g.material};
-          out.put_payload(i, brick_mask(b, int32_t(gx * 4), int32_t(g.gy * 4), int32_t(g.gz * 4)), g.material);
+          uint64_t mask;
+          if (g.kind == 0 && g.hx <= kSphere16MaxR) {
+            mask = sphere_mask16_row(int32_t(gx * 4) - g.cx, g.w);
+          } else {
+            const vx_brush b{g.cx, int32_t(g.w[2]), int32_t(g.w[3]), g.hx, int32_t(g.w[4]), int32_t(g.w[5]), g.kind,
+                             g.material};
+            mask = brick_mask(b, int32_t(gx * 4), int32_t(g.w[0] * 4), int32_t(g.w[1] * 4));
+          }
+          out.put_payload(i, mask, g.material);
         }
       }
       i0 = max(i0, stop);  // [i0, stop) done
